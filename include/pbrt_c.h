@@ -1,0 +1,339 @@
+/*
+ * pbrt_c.h - C ABI of the MI355X (gfx950) frame path: BVH build, path tracing,
+ * temporal accumulation, BMFR/BFR denoising, TAA and format conversion.
+ *
+ * Every entry point returns an int32_t status (PBRT_OK == 0, negative on error)
+ * and never throws.  pbrt_last_error() returns the message of the last failure
+ * on the calling thread.  Images are caller-owned device pointers, row-major,
+ * tightly packed (pitch == width), pixel (x, y) at index y * width + x.
+ * `stream` is a hipStream_t passed as an opaque pointer (NULL = null stream).
+ *
+ * The reference has no C ABI: its "FFI" is the Vulkan command buffer recorded by
+ * the render modules in source/renderModules.  Each entry point below names the
+ * reference pipeline/dispatch it replaces.
+ */
+#ifndef PBRT_C_H
+#define PBRT_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBRT_ABI_VERSION 1
+
+enum pbrt_status {
+    PBRT_OK = 0,
+    PBRT_ERR_INVALID_ARG = -1,
+    PBRT_ERR_UNSUPPORTED = -2,
+    PBRT_ERR_OUT_OF_MEMORY = -3,
+    PBRT_ERR_HIP = -4,
+    PBRT_ERR_INTERNAL = -5
+};
+
+/* ------------------------------------------------------------------------- */
+/* Plain data contracts (std430 layouts of the reference's SSBOs)            */
+/* ------------------------------------------------------------------------- */
+
+/* RayTracingPushConstants (source/renderModules/PipelineStructs.hpp:6-13)
+ * == layoutPTPushConstants.glsl:4-11.  Matrices are column-major (vsg/GLSL). */
+typedef struct pbrt_push_constants {
+    float view_inverse[16];
+    float proj_inverse[16];
+    float prev_view[16];
+    uint32_t frame_number;
+    uint32_t sample_number;
+} pbrt_push_constants; /* 200 bytes */
+
+/* vsg::Light::PackedLight (external/vsg/include/vsg/nodes/Light.h:23-37)
+ * == struct Light in shaders/ptStructures.glsl:54-63.  v0_type[3] holds the
+ * light type (1 directional, 2 point, 5 area), strengths[3] the inclusive
+ * strength prefix sum written by scene packing. */
+typedef struct pbrt_light {
+    float v0_type[4];
+    float v1_strength[4];
+    float v2_angle[4];
+    float dir_angle2[4];
+    float col_ambient[4];
+    float col_diffuse[4];
+    float col_specular[4];
+    float strengths[4];
+} pbrt_light; /* 128 bytes */
+
+/* WaveFrontMaterialPacked (source/scene/RayTracingVisitor.hpp:70-79)
+ * == shaders/ptStructures.glsl:18-27. */
+typedef struct pbrt_material {
+    float ambient_roughness[4];
+    float diffuse_ior[4];
+    float specular_dissolve[4];
+    float transmittance_illum[4];
+    float emission_texture_id[4];
+    uint32_t category_id;
+    float pad[3];
+} pbrt_material; /* 96 bytes */
+
+/* RGBA8 UNORM texture, row-major, sampled LINEAR / REPEAT / mip 0
+ * (external/vsg/include/vsg/state/Sampler.h:29-43). */
+typedef struct pbrt_texture_desc {
+    uint32_t width;
+    uint32_t height;
+    const uint8_t* rgba8;
+} pbrt_texture_desc;
+
+/* One mesh (a vsg::VertexIndexDraw plus its descriptor set).  A mesh's
+ * material and texture set are indexed by the mesh id, as the reference's
+ * closest-hit shader does (shaders/ptClosesthit.rchit:21-22,32,48,50,57,84). */
+typedef struct pbrt_mesh_desc {
+    uint32_t vertex_count;
+    const float* positions; /* 3 * vertex_count */
+    const float* normals;   /* 3 * vertex_count; a zero first normal requests area-weighted
+                               normals (RayTracingVisitor.cpp:52-118); NULL == all zero */
+    const float* uvs;       /* 2 * vertex_count, or NULL for the default per-triangle
+                               uvs (RayTracingVisitor.cpp:122-159) */
+    uint32_t index_count;   /* multiple of 3 */
+    uint32_t index_stride;  /* 2 (uint16) or 4 (uint32) */
+    const void* indices;
+    pbrt_material material;
+    int32_t diffuse_texture;  /* texture index or -1 for the 1x1 white default */
+    int32_t normal_texture;
+    int32_t specular_texture;
+    int32_t emissive_texture;
+} pbrt_mesh_desc;
+
+/* One TLAS instance: object-to-world transform (column-major 4x4) and mesh. */
+typedef struct pbrt_instance_desc {
+    float transform[16];
+    uint32_t mesh;
+    uint32_t pad[3];
+} pbrt_instance_desc;
+
+enum pbrt_bvh_builder {
+    PBRT_BVH_SAH_HOST = 0,  /* binned SAH on the host */
+    PBRT_BVH_LBVH_GPU = 1   /* Morton-code LBVH built by gfx950 kernels */
+};
+
+typedef struct pbrt_scene_desc {
+    uint32_t mesh_count;
+    const pbrt_mesh_desc* meshes;
+    uint32_t instance_count;
+    const pbrt_instance_desc* instances;
+    uint32_t texture_count;
+    const pbrt_texture_desc* textures;
+    uint32_t light_count;     /* explicit lights, appended after the emissive-triangle lights */
+    const pbrt_light* lights;
+    uint32_t bvh_builder;     /* enum pbrt_bvh_builder */
+    uint32_t reserved;
+} pbrt_scene_desc;
+
+typedef struct pbrt_scene_info {
+    uint32_t triangle_count;   /* world-space triangles over all instances */
+    uint32_t instance_count;
+    uint32_t mesh_count;
+    uint32_t light_count;      /* Infos.lightCount (layoutPTUniform.glsl:4-9) */
+    float light_strength_sum;  /* Infos.lightStrengthSum */
+    uint32_t bvh_node_count;
+    uint32_t bvh_depth;
+    uint32_t light_sampling;   /* enum pbrt_light_sampling chosen at packing */
+    uint64_t device_bytes;
+} pbrt_scene_info;
+
+typedef struct pbrt_scene pbrt_scene;
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                    */
+/* ------------------------------------------------------------------------- */
+
+int32_t pbrt_abi_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* pbrt_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Scene packing + BVH build                                                  */
+/* Replaces RayTracingSceneDescriptorCreationVisitor (source/scene/           */
+/* RayTracingVisitor.cpp:3-560), PBRTPipeline::setup_pipeline                  */
+/* (source/renderModules/PBRTPipeline.cpp:85-170) and the BLAS/TLAS build      */
+/* (external/vsg/src/vsg/raytracing/BuildAccelerationStructureTraversal.cpp:   */
+/* 63-90, vsg/vk/Context.cpp:214-222).                                         */
+/* ------------------------------------------------------------------------- */
+
+int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** out_scene);
+int32_t pbrt_scene_destroy(pbrt_scene* scene);
+int32_t pbrt_scene_get_info(const pbrt_scene* scene, pbrt_scene_info* out_info);
+/* Copies the packed light list (PBRTPipeline.cpp:154-162 data) to host memory. */
+int32_t pbrt_scene_get_lights(const pbrt_scene* scene, pbrt_light* out_lights, uint32_t capacity);
+
+/* ------------------------------------------------------------------------- */
+/* Path tracing: replaces vkCmdTraceRaysKHR over ptRaygen.rgen /              */
+/* ptClosesthit.rchit / ptAlphaHit.rahit / ptMiss.rmiss / shadow.rmiss        */
+/* (PBRTPipeline::add_trace_rays_to_command_graph, PBRTPipeline.cpp:65-80).    */
+/* ------------------------------------------------------------------------- */
+
+enum pbrt_light_sampling {
+    PBRT_LIGHT_SAMPLE_SURFACE_STRENGTH = 0, /* lighting.glsl:18-96  */
+    PBRT_LIGHT_SAMPLE_LIGHT_STRENGTH = 1,   /* lighting.glsl:98-168 */
+    PBRT_LIGHT_SAMPLE_UNIFORM = 2,          /* lighting.glsl:170-227 */
+    PBRT_LIGHT_SAMPLE_AUTO = 0xFFFFFFFFu    /* >800 lights => uniform (PBRTPipeline.cpp:92-96) */
+};
+
+enum pbrt_illumination_output {
+    PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT = 0, /* IlluminationBufferDemodulatedFloat, ptRaygen.rgen:83-89 */
+    PBRT_OUTPUT_FINAL_IMAGE = 1               /* IlluminationBufferFinalFloat, ptRaygen.rgen:91-103 */
+};
+
+enum pbrt_tracer_kind {
+    PBRT_TRACER_MEGAKERNEL = 0, /* one thread per path, whole path in one launch */
+    PBRT_TRACER_WAVEFRONT = 1   /* ray queues: extend / shade / shadow launches per bounce */
+};
+
+typedef struct pbrt_trace_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t max_recursion_depth; /* Infos.maxRecursionDepth, 2 in the reference (PBRTPipeline.cpp:27) */
+    uint32_t min_recursion_depth; /* Infos.minRecursionDepth, 0 in the reference */
+    uint32_t light_sampling;      /* enum pbrt_light_sampling */
+    uint32_t output;              /* enum pbrt_illumination_output */
+    uint32_t write_gbuffer;       /* GBUFFER define */
+    uint32_t tracer;              /* enum pbrt_tracer_kind */
+} pbrt_trace_params;
+
+typedef struct pbrt_trace_images {
+    float* depth;          /* R32F     (GBuffer.cpp, layoutPTImages.glsl:9) */
+    float* normal;         /* RG32F    spherical (acos z, atan(y, x)) */
+    uint8_t* material;     /* RGBA8    category / 255 */
+    uint8_t* albedo;       /* RGBA8    diffuse + specular colour */
+    float* illumination;   /* RGBA32F  demodulated radiance, or the progressive final image */
+} pbrt_trace_images;
+
+/* Optional per-launch counters (device pointer to 4 x uint64, accumulated with
+ * atomics; NULL disables counting): [0] closest-hit rays, [1] shadow rays,
+ * [2] BVH nodes visited, [3] triangles tested. */
+int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* params,
+                   const pbrt_push_constants* push_constants, const pbrt_trace_images* images,
+                   unsigned long long* counters, void* stream);
+
+/* Closest-hit query over the scene BVH (the traversal traceRayEXT performs).
+ * rays: n x 8 floats (origin xyz, tmin, direction xyz, tmax).
+ * hits: n x 4 words (t as float bits, instance index, primitive index, 0xFFFFFFFF on miss)
+ * and barycentrics: n x 2 floats (u, v) as in hitAttributeEXT. */
+int32_t pbrt_intersect(const pbrt_scene* scene, const float* rays, uint32_t n, uint32_t* hits,
+                       float* barycentrics, void* stream);
+/* Any-hit shadow query (gl_RayFlagsTerminateOnFirstHit | SkipClosestHit with the
+ * alpha test of ptAlphaHit.rahit on non-opaque instances); out: n bytes, 1 = occluded. */
+int32_t pbrt_occluded(const pbrt_scene* scene, const float* rays, uint32_t n, uint8_t* occluded,
+                      void* stream);
+
+/* RNG streams of shaders/random.glsl (rEInit + randomFloat) for a width x height
+ * launch: out[(y * width + x) * count + k] = bit pattern of the k-th randomFloat. */
+int32_t pbrt_rng_sequence(uint32_t width, uint32_t height, uint32_t frame_index, uint32_t count,
+                          uint32_t* out, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Temporal accumulation: replaces accumulator.comp (SEPARATE_MATRICES form,   */
+/* shaders/accumulator.comp:33-105) dispatched by Accumulator::                */
+/* add_dispatch_to_command_graph (source/renderModules/Accumulator.cpp:72-83). */
+/* ------------------------------------------------------------------------- */
+
+/* Accumulator push constants as set by Accumulator::set_camera_matrices
+ * (Accumulator.cpp:85-117) with separate matrices. */
+typedef struct pbrt_accumulate_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t frame_number;
+    uint32_t reserved;
+    float inv_proj[16];    /* current inverse projection ("view" push constant) */
+    float inv_view[16];    /* current inverse view */
+    float prev_view[16];   /* previous frame's view matrix */
+    float prev_origin[4];  /* inverse(prev_view)[3] with w = 1 */
+} pbrt_accumulate_params;
+
+typedef struct pbrt_accumulate_images {
+    const float* illumination;         /* RGBA32F  noisy demodulated radiance (tracer output) */
+    const float* depth;                /* R32F     current G-buffer depth */
+    const float* prev_depth;           /* R32F     previous depth (sampled bilinear) */
+    const uint8_t* prev_spp;           /* R8       previous sample counts (sampled bilinear) */
+    const uint16_t* prev_accumulated;  /* RGBA16F  previous accumulated illumination (sampled bilinear) */
+    uint16_t* motion;                  /* RG16F    out: previous-frame uv, or (-1, -1) */
+    uint8_t* spp;                      /* R8       out: sample count / 256 */
+    uint16_t* accumulated;             /* RGBA16F  out: accumulated demodulated radiance */
+} pbrt_accumulate_images;
+
+int32_t pbrt_accumulate(const pbrt_accumulate_params* params, const pbrt_accumulate_images* images, void* stream);
+
+/* AccumulationBuffer::copy_to_back_images (source/buffers/AccumulationBuffer.cpp:72-244):
+ * current depth, spp and accumulated illumination copied to their previous-frame images.
+ * Any destination may be NULL to skip it (a frame graph may ping-pong instead). */
+int32_t pbrt_copy_back(uint32_t width, uint32_t height, const float* depth, const uint8_t* spp,
+                       const uint16_t* accumulated, float* prev_depth, uint8_t* prev_spp,
+                       uint16_t* prev_accumulated, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* TAA: replaces taa.comp (shaders/taa.comp:48-104) plus Taa::copy_final_image */
+/* (source/renderModules/Taa.cpp:99-143).                                     */
+/* ------------------------------------------------------------------------- */
+typedef struct pbrt_taa_images {
+    const uint16_t* motion;     /* RG16F    previous-frame uv (accumulator output) */
+    const uint8_t* denoised;    /* B8G8R8A8 denoiser final image */
+    uint8_t* history;           /* R8G8B8A8 accumulation image; receives a raw byte copy of `final` */
+    uint8_t* final_image;       /* B8G8R8A8 out */
+} pbrt_taa_images;
+
+int32_t pbrt_taa(uint32_t width, uint32_t height, uint32_t frame_number, const pbrt_taa_images* images,
+                 void* stream);
+
+/* formatConverter.comp:9-14 with FORMAT rgba8 on a B8G8R8A8 image
+ * (source/renderModules/FormatConverter.cpp:4-82): RGBA32F -> B8G8R8A8. */
+int32_t pbrt_convert(uint32_t width, uint32_t height, const float* source, uint8_t* bgra8, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* BMFR denoiser: replaces bmfrPre/bmfrFit/bmfrPost.comp (shaders/bmfrPre.comp */
+/* :5-98, bmfrFit.comp:5-92, bmfrPost.comp:5-124) dispatched by               */
+/* BMFR::add_dispatch_to_command_graph (source/renderModules/denoisers/        */
+/* BMFR.cpp:203-230).  Blocks are block_size^2 pixels on the padded grid       */
+/* ((W / b + 2) * b) x ((H / b + 2) * b), shifted by pixelOffsets[frame % 16]. */
+/* ------------------------------------------------------------------------- */
+
+enum pbrt_bmfr_fit {
+    /* the reference algorithm: FP32 Householder QR of the fp16 feature block
+     * (bmfrFit.comp:27-81) with wave64 reductions */
+    PBRT_BMFR_FIT_HOUSEHOLDER = 0,
+    /* per-block projection of the features against the noised constant column,
+     * Gram matrix on FP16 MFMA (hi/lo split, FP32 accumulation), FP64 Cholesky
+     * solve; pre and fit fused (no feature buffer round trip) */
+    PBRT_BMFR_FIT_GRAM_MFMA = 1
+};
+
+typedef struct pbrt_bmfr_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t block_size;     /* 8, 16 or 32 (DenoisingBlockSize X8 / X16 / X32) */
+    uint32_t frame_number;   /* push constant frameNumber */
+    uint32_t fit;            /* enum pbrt_bmfr_fit */
+    uint32_t reserved[3];
+} pbrt_bmfr_params;
+
+typedef struct pbrt_bmfr_images {
+    const float* depth;         /* R32F G-buffer depth */
+    const float* normal;        /* RG32F spherical normal */
+    const uint8_t* albedo;      /* RGBA8 */
+    const uint16_t* motion;     /* RG16F accumulator motion */
+    const uint8_t* spp;         /* R8 accumulator sample count / 256 */
+    const uint16_t* noisy;      /* RGBA16F accumulated demodulated illumination */
+    uint16_t* denoised;         /* RGBA16F x 2 layers ([2][H][W][4]); reads layer frame & 1,
+                                   writes layer (frame & 1) ^ 1 */
+    uint8_t* final_image;       /* B8G8R8A8 out: remodulated, gamma 1/2.2 */
+    void* workspace;            /* device scratch of pbrt_bmfr_workspace_size() bytes */
+    size_t workspace_bytes;
+} pbrt_bmfr_images;
+
+/* Workspace layout: fp16 feature buffer [13][Hp][Wp] (Householder fit only),
+ * then float weights [blocks_y][blocks_x][30] (feature-major, rgb inner). */
+int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
+int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* PBRT_C_H */

@@ -1,0 +1,1103 @@
+// CPU oracle of the path tracing stage (test infrastructure only).
+//
+// Restates, per pixel, shaders/ptRaygen.rgen:24-104 with ptClosesthit.rchit,
+// ptAlphaHit.rahit, ptMiss.rmiss, shadow.rmiss, lighting.glsl, sampling.glsl,
+// brdf.glsl, color.glsl, geometry.glsl, camera.glsl and random.glsl, plus the
+// scene packing rules of source/scene/RayTracingVisitor.cpp.  Ray/triangle
+// queries are answered in FP64 (Moller-Trumbore on world-space triangles
+// transformed in double), either by brute force over every triangle or through
+// a median-split BVH that returns the same nearest hit.
+//
+// Semantics fixed where the reference is undefined (shared with the product):
+//   * a path ends when indirectLighting returns without tracing or the traced
+//     ray misses (ptRaygen.rgen:70 would otherwise normalize a zero vector);
+//   * alpha testing applies to every ray on alpha-tested instances;
+//   * 1x1 textures return their texel exactly; other fetches are bilinear with
+//     FP32 weights, REPEAT addressing, mip 0.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "../include/pbrt_c.h"
+#include "oracle_common.hpp"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// random.glsl
+// ---------------------------------------------------------------------------
+struct RandomEngine {
+    uint32_t x, y, z, w;
+};
+
+// wangHash (random.glsl:74-81)
+inline uint32_t wang_hash(uint32_t seed) {
+    seed = (seed ^ 61u) ^ (seed >> 16);
+    seed *= 9u;
+    seed = seed ^ (seed >> 4);
+    seed *= 0x27d4eb2du;
+    seed = seed ^ (seed >> 15);
+    return seed;
+}
+
+// rEInit (random.glsl:84-96)
+inline RandomEngine re_init(uint32_t x, uint32_t y, uint32_t frame) {
+    RandomEngine r;
+    r.x = wang_hash(x);
+    r.y = wang_hash(y);
+    r.z = wang_hash(frame);
+    r.w = wang_hash((x + y) * frame);
+    return r;
+}
+
+// TausStep (random.glsl:17-21)
+inline uint32_t taus(uint32_t z, int s1, int s2, int s3, uint32_t m) {
+    uint32_t b = ((z << s1) ^ z) >> s2;
+    return ((z & m) << s3) ^ b;
+}
+
+// randomNumber / randomFloat (random.glsl:39-58): the float literal
+// 2.3283064365387e-10 is kept as written and rounded to float by the compiler.
+inline float random_float(RandomEngine& e) {
+    e.x = taus(e.x, 13, 19, 12, 4294967294u);
+    e.y = taus(e.y, 2, 25, 4, 4294967288u);
+    e.z = taus(e.z, 3, 11, 17, 4294967280u);
+    e.w = 1664525u * e.w + 1013904223u;
+    const float scale = 2.3283064365387e-10f;
+    return scale * (float)(e.x ^ e.y ^ e.z ^ e.w);
+}
+
+// ---------------------------------------------------------------------------
+// Scene (RayTracingVisitor.cpp restatement)
+// ---------------------------------------------------------------------------
+struct Tex {
+    uint32_t w = 1, h = 1;
+    std::vector<uint8_t> rgba;
+};
+
+struct MeshO {
+    std::vector<float> pos, nor, uv;
+    std::vector<uint32_t> idx;
+    int tex[4];  // diffuse, normal, specular, emissive: index into textures (0 = white)
+    pbrt_material mat;
+    bool opaque;
+};
+
+struct InstO {
+    float m[16];
+    float nm[9];  // transpose(inverse(upper 3x3)), column-major
+    uint32_t mesh;
+};
+
+struct SceneO {
+    std::vector<Tex> textures;
+    std::vector<MeshO> meshes;
+    std::vector<InstO> insts;
+    std::vector<pbrt_light> lights;
+    float strength_sum = 0;
+    // world triangles (double), instance/primitive per triangle
+    std::vector<double> wt;  // 9 per triangle
+    std::vector<uint32_t> tri_inst, tri_prim;
+    // BVH (median split) over world triangles
+    struct Node {
+        double lo[3], hi[3];
+        int32_t left, right;  // left < 0: leaf [first, first + count)
+        uint32_t first, count;
+    };
+    std::vector<Node> nodes;
+    std::vector<uint32_t> order;
+};
+
+static void normal_mat(const float* m, float* out) {
+    double a[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) a[r][c] = m[4 * c + r];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2]) - a[0][1] * (a[1][0] * a[2][2] - a[2][0] * a[1][2]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[2][0] * a[1][1]);
+    // cofactor matrix / det == transpose(inverse)
+    double cof[3][3];
+    cof[0][0] = (a[1][1] * a[2][2] - a[2][1] * a[1][2]);
+    cof[0][1] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2]);
+    cof[0][2] = (a[1][0] * a[2][1] - a[2][0] * a[1][1]);
+    cof[1][0] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2]);
+    cof[1][1] = (a[0][0] * a[2][2] - a[2][0] * a[0][2]);
+    cof[1][2] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1]);
+    cof[2][0] = (a[0][1] * a[1][2] - a[1][1] * a[0][2]);
+    cof[2][1] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2]);
+    cof[2][2] = (a[0][0] * a[1][1] - a[1][0] * a[0][1]);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) out[3 * c + r] = (float)(cof[r][c] / det);
+}
+
+static uint32_t idx_at(const pbrt_mesh_desc& m, uint32_t i) {
+    return m.index_stride == 2 ? ((const uint16_t*)m.indices)[i] : ((const uint32_t*)m.indices)[i];
+}
+
+static void build_bvh(SceneO& s) {
+    uint32_t n = (uint32_t)s.tri_inst.size();
+    s.order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) s.order[i] = i;
+    std::vector<double> cen(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a)
+            cen[3 * i + a] = (s.wt[9 * (size_t)i + a] + s.wt[9 * (size_t)i + 3 + a] + s.wt[9 * (size_t)i + 6 + a]) / 3.0;
+    s.nodes.clear();
+    struct Task {
+        uint32_t node, b, e;
+    };
+    std::vector<Task> st;
+    s.nodes.push_back({});
+    st.push_back({0, 0, n});
+    while (!st.empty()) {
+        Task t = st.back();
+        st.pop_back();
+        SceneO::Node nd{};
+        for (int a = 0; a < 3; ++a) {
+            nd.lo[a] = 1e300;
+            nd.hi[a] = -1e300;
+        }
+        double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+        for (uint32_t i = t.b; i < t.e; ++i) {
+            uint32_t k = s.order[i];
+            for (int v = 0; v < 3; ++v)
+                for (int a = 0; a < 3; ++a) {
+                    double x = s.wt[9 * (size_t)k + 3 * v + a];
+                    nd.lo[a] = std::min(nd.lo[a], x);
+                    nd.hi[a] = std::max(nd.hi[a], x);
+                }
+            for (int a = 0; a < 3; ++a) {
+                clo[a] = std::min(clo[a], cen[3 * k + a]);
+                chi[a] = std::max(chi[a], cen[3 * k + a]);
+            }
+        }
+        if (t.e - t.b <= 4) {
+            nd.left = -1;
+            nd.first = t.b;
+            nd.count = t.e - t.b;
+            s.nodes[t.node] = nd;
+            continue;
+        }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+        uint32_t mid = (t.b + t.e) / 2;
+        std::nth_element(s.order.begin() + t.b, s.order.begin() + mid, s.order.begin() + t.e,
+                         [&](uint32_t x, uint32_t y) { return cen[3 * x + ax] < cen[3 * y + ax]; });
+        nd.left = (int32_t)s.nodes.size();
+        nd.right = nd.left + 1;
+        s.nodes[t.node] = nd;
+        s.nodes.push_back({});
+        s.nodes.push_back({});
+        st.push_back({(uint32_t)nd.left, t.b, mid});
+        st.push_back({(uint32_t)nd.right, mid, t.e});
+    }
+}
+
+SceneO* pack(const pbrt_scene_desc& d) {
+    auto s = std::make_unique<SceneO>();
+    Tex white;
+    white.rgba = {255, 255, 255, 255};
+    s->textures.push_back(white);
+    std::vector<bool> tex_opaque(1, true);
+    for (uint32_t t = 0; t < d.texture_count; ++t) {
+        Tex x;
+        x.w = d.textures[t].width;
+        x.h = d.textures[t].height;
+        x.rgba.assign(d.textures[t].rgba8, d.textures[t].rgba8 + 4 * (size_t)x.w * x.h);
+        bool op = true;
+        // RayTracingVisitor.cpp:330-335: any alpha < .01 * 255 => not opaque
+        for (size_t i = 0; i < (size_t)x.w * x.h; ++i)
+            if (x.rgba[4 * i + 3] < .01F * 255) {
+                op = false;
+                break;
+            }
+        tex_opaque.push_back(op);
+        s->textures.push_back(std::move(x));
+    }
+    for (uint32_t mi = 0; mi < d.mesh_count; ++mi) {
+        const pbrt_mesh_desc& m = d.meshes[mi];
+        MeshO o;
+        o.pos.assign(m.positions, m.positions + 3 * (size_t)m.vertex_count);
+        o.nor.assign(3 * (size_t)m.vertex_count, 0.0f);
+        if (m.normals) o.nor.assign(m.normals, m.normals + 3 * (size_t)m.vertex_count);
+        for (uint32_t i = 0; i < m.index_count; ++i) o.idx.push_back(idx_at(m, i));
+        // RayTracingVisitor.cpp:52-118: area-weighted running average of face normals
+        float n0 = o.nor[0], n1 = o.nor[1], n2 = o.nor[2];
+        if (n0 * n0 + n1 * n1 + n2 * n2 == 0) {
+            std::vector<float> wsum(m.vertex_count, 0.0f);
+            for (size_t tri = 0; tri < o.idx.size() / 3; ++tri) {
+                uint32_t id[3] = {o.idx[3 * tri], o.idx[3 * tri + 1], o.idx[3 * tri + 2]};
+                V3<float> a(o.pos[3 * id[0]], o.pos[3 * id[0] + 1], o.pos[3 * id[0] + 2]);
+                V3<float> b(o.pos[3 * id[1]], o.pos[3 * id[1] + 1], o.pos[3 * id[1] + 2]);
+                V3<float> c(o.pos[3 * id[2]], o.pos[3 * id[2] + 1], o.pos[3 * id[2] + 2]);
+                V3<float> fn = cross(b - a, c - a);
+                float w = length(fn);
+                fn = fn / w;
+                for (uint32_t k : id) {
+                    float nw = w + wsum[k];
+                    V3<float> old(o.nor[3 * k], o.nor[3 * k + 1], o.nor[3 * k + 2]);
+                    V3<float> nn = old * (wsum[k] / nw) + fn * (w / nw);
+                    o.nor[3 * k] = nn.x;
+                    o.nor[3 * k + 1] = nn.y;
+                    o.nor[3 * k + 2] = nn.z;
+                    wsum[k] = nw;
+                }
+            }
+        }
+        // RayTracingVisitor.cpp:122-159: default uvs (0,0), (0,1), (1,0) per triangle corner
+        if (m.uvs) {
+            o.uv.assign(m.uvs, m.uvs + 2 * (size_t)m.vertex_count);
+        } else {
+            o.uv.assign(2 * (size_t)m.vertex_count, 0.0f);
+            for (size_t tri = 0; tri < o.idx.size() / 3; ++tri) {
+                o.uv[2 * o.idx[3 * tri] + 0] = 0;
+                o.uv[2 * o.idx[3 * tri] + 1] = 0;
+                o.uv[2 * o.idx[3 * tri + 1] + 0] = 0;
+                o.uv[2 * o.idx[3 * tri + 1] + 1] = 1;
+                o.uv[2 * o.idx[3 * tri + 2] + 0] = 1;
+                o.uv[2 * o.idx[3 * tri + 2] + 1] = 0;
+            }
+        }
+        o.tex[0] = m.diffuse_texture + 1;
+        o.tex[1] = m.normal_texture + 1;
+        o.tex[2] = m.specular_texture + 1;
+        o.tex[3] = m.emissive_texture + 1;
+        o.mat = m.material;
+        o.opaque = tex_opaque[o.tex[0]];
+        s->meshes.push_back(std::move(o));
+    }
+    for (uint32_t ii = 0; ii < d.instance_count; ++ii) {
+        const pbrt_instance_desc& in = d.instances[ii];
+        InstO o;
+        std::memcpy(o.m, in.transform, sizeof(o.m));
+        normal_mat(in.transform, o.nm);
+        o.mesh = in.mesh;
+        s->insts.push_back(o);
+        const MeshO& m = s->meshes[in.mesh];
+        size_t ntri = m.idx.size() / 3;
+        auto world = [&](uint32_t vi, double* w) {
+            const float* p = &m.pos[3 * vi];
+            for (int r = 0; r < 3; ++r)
+                w[r] = (double)in.transform[r] * p[0] + (double)in.transform[4 + r] * p[1] +
+                       (double)in.transform[8 + r] * p[2] + (double)in.transform[12 + r];
+        };
+        for (size_t t = 0; t < ntri; ++t) {
+            for (int k = 0; k < 3; ++k) {
+                double w[3];
+                world(m.idx[3 * t + k], w);
+                s->wt.insert(s->wt.end(), w, w + 3);
+            }
+            s->tri_inst.push_back(ii);
+            s->tri_prim.push_back((uint32_t)t);
+        }
+        // RayTracingVisitor.cpp:170-221: one area light per emissive triangle
+        const float* em = m.mat.emission_texture_id;
+        if (em[0] + em[1] + em[2] != 0) {
+            for (size_t t = 0; t < ntri; ++t) {
+                pbrt_light l{};
+                for (int a = 0; a < 3; ++a) l.col_ambient[a] = l.col_diffuse[a] = l.col_specular[a] = em[a];
+                l.strengths[2] = 1;
+                float* corner[3] = {l.v0_type, l.v1_strength, l.v2_angle};
+                for (int k = 0; k < 3; ++k) {
+                    double w[3];
+                    world(m.idx[3 * t + k], w);
+                    for (int a = 0; a < 3; ++a) corner[k][a] = (float)w[a];
+                }
+                l.v0_type[3] = 5;
+                s->lights.push_back(l);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < d.light_count; ++i) s->lights.push_back(d.lights[i]);
+    if (s->lights.empty()) {
+        // RayTracingVisitor.cpp:414-427
+        pbrt_light l{};
+        l.v0_type[3] = 1;
+        V3<float> dir = normalize(V3<float>(0.1f, 1.0f, -5.1f));
+        l.dir_angle2[0] = dir.x;
+        l.dir_angle2[1] = dir.y;
+        l.dir_angle2[2] = dir.z;
+        for (int a = 0; a < 3; ++a) {
+            l.col_ambient[a] = 2;
+            l.col_diffuse[a] = 2;
+        }
+        l.strengths[0] = .5f;
+        s->lights.push_back(l);
+    }
+    // RayTracingVisitor.cpp:428-441
+    float sum = 0;
+    for (auto& l : s->lights) {
+        sum += l.col_ambient[0] + l.col_ambient[1] + l.col_ambient[2] + l.col_diffuse[0] + l.col_diffuse[1] +
+               l.col_diffuse[2] + l.col_specular[0] + l.col_specular[1] + l.col_specular[2];
+        l.strengths[3] = sum;
+    }
+    s->strength_sum = sum;
+    build_bvh(*s);
+    return s.release();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 ray queries
+// ---------------------------------------------------------------------------
+struct HitO {
+    int32_t tri = -1;
+    double t = 0, u = 0, v = 0;
+};
+
+template <class T>
+V4<T> tex_fetch(const SceneO& s, int tex, V2<T> uv);
+
+// alpha test of ptAlphaHit.rahit:14-52 (FP32 texture fetch)
+static bool alpha_rejects(const SceneO& s, uint32_t tri, double u, double v) {
+    const InstO& in = s.insts[s.tri_inst[tri]];
+    const MeshO& m = s.meshes[in.mesh];
+    if (m.opaque) return false;
+    uint32_t prim = s.tri_prim[tri];
+    const uint32_t* id = &m.idx[3 * (size_t)prim];
+    float uf = (float)u, vf = (float)v;
+    float b0 = 1.0f - uf - vf;
+    V2<float> tc;
+    tc.x = m.uv[2 * id[0]] * b0 + m.uv[2 * id[1]] * uf + m.uv[2 * id[2]] * vf;
+    tc.y = m.uv[2 * id[0] + 1] * b0 + m.uv[2 * id[1] + 1] * uf + m.uv[2 * id[2] + 1] * vf;
+    return tex_fetch<float>(s, m.tex[0], tc).w < .01f;
+}
+
+// Moller-Trumbore in double; hit if t in [tmin, tmax] and barycentrics inside.
+static bool tri_test(const SceneO& s, uint32_t k, const double* o, const double* d, double tmin, double tmax,
+                     double& t, double& u, double& v) {
+    const double* p = &s.wt[9 * (size_t)k];
+    double e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+    double e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+    double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det == 0) return false;
+    double inv = 1.0 / det;
+    double tv[3] = {o[0] - p[0], o[1] - p[1], o[2] - p[2]};
+    u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+    if (u < 0 || u > 1) return false;
+    double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    v = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * inv;
+    if (v < 0 || u + v > 1) return false;
+    t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+    return t >= tmin && t <= tmax;
+}
+
+static bool box_test(const SceneO::Node& n, const double* o, const double* inv, double tmin, double tmax) {
+    double t0 = tmin, t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        double ta = (n.lo[a] - o[a]) * inv[a], tb = (n.hi[a] - o[a]) * inv[a];
+        if (std::isnan(ta) || std::isnan(tb)) continue;  // ray in the slab plane
+        if (ta > tb) std::swap(ta, tb);
+        t0 = std::max(t0, ta);
+        t1 = std::min(t1, tb * (1 + 1e-12));
+    }
+    return t0 <= t1;
+}
+
+// Closest accepted hit (any_hit: first accepted hit).  brute: test every triangle.
+static HitO query(const SceneO& s, const double* o, const double* d, double tmin, double tmax, bool any_hit,
+                  bool brute) {
+    HitO best;
+    double tbest = tmax;
+    auto consider = [&](uint32_t k) {
+        double t, u, v;
+        if (!tri_test(s, k, o, d, tmin, tbest, t, u, v)) return false;
+        if (best.tri >= 0 && t == tbest && (int32_t)k > best.tri) return false;  // ties: lowest index
+        if (alpha_rejects(s, k, u, v)) return false;
+        best.tri = (int32_t)k;
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        tbest = t;
+        return true;
+    };
+    if (brute) {
+        for (uint32_t k = 0; k < s.tri_inst.size(); ++k)
+            if (consider(k) && any_hit) return best;
+        return best;
+    }
+    double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const SceneO::Node& n = s.nodes[stack[--sp]];
+        if (!box_test(n, o, inv, tmin, tbest)) continue;
+        if (n.left < 0) {
+            for (uint32_t i = n.first; i < n.first + n.count; ++i)
+                if (consider(s.order[i]) && any_hit) return best;
+        } else {
+            stack[sp++] = (uint32_t)n.right;
+            stack[sp++] = (uint32_t)n.left;
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Textures (sampler LINEAR / REPEAT / mip 0)
+// ---------------------------------------------------------------------------
+template <class T>
+V4<T> tex_fetch(const SceneO& s, int tex, V2<T> uv) {
+    const Tex& t = s.textures[tex];
+    auto texel = [&](int x, int y) {
+        const uint8_t* c = &t.rgba[4 * ((size_t)y * t.w + x)];
+        return V4<float>{(float)c[0] / 255.0f, (float)c[1] / 255.0f, (float)c[2] / 255.0f, (float)c[3] / 255.0f};
+    };
+    if (t.w == 1 && t.h == 1) {
+        V4<float> c = texel(0, 0);
+        return {T(c.x), T(c.y), T(c.z), T(c.w)};
+    }
+    float x = (float)uv.x * (float)t.w - 0.5f, y = (float)uv.y * (float)t.h - 0.5f;
+    float x0f = std::floor(x), y0f = std::floor(y);
+    float fx = x - x0f, fy = y - y0f;
+    auto wrap = [](int i, int n) {
+        int r = i % n;
+        return r < 0 ? r + n : r;
+    };
+    int x0 = wrap((int)x0f, (int)t.w), x1 = wrap((int)x0f + 1, (int)t.w);
+    int y0 = wrap((int)y0f, (int)t.h), y1 = wrap((int)y0f + 1, (int)t.h);
+    V4<float> a = texel(x0, y0), b = texel(x1, y0), c = texel(x0, y1), e = texel(x1, y1);
+    auto lerp = [](V4<float> p, V4<float> q, float w) {
+        float k = 1.0f - w;
+        return V4<float>{p.x * k + q.x * w, p.y * k + q.y * w, p.z * k + q.z * w, p.w * k + q.w * w};
+    };
+    V4<float> r = lerp(lerp(a, b, fx), lerp(c, e, fx), fy);
+    return {T(r.x), T(r.y), T(r.z), T(r.w)};
+}
+
+// ---------------------------------------------------------------------------
+// Shading (T = float mirror, T = double oracle)
+// ---------------------------------------------------------------------------
+template <class T>
+struct Surface {
+    T perceptual_roughness, metalness, alpha_roughness;
+    int illum;
+    V3<T> r0, r90, diffuse, specular, emissive, transmissive, normal;
+    V3<T> basis[3];
+    T ior;
+};
+
+template <class T> struct K {
+    static constexpr T pi = T(3.14159265359);
+    static constexpr T rpi = T(0.31830988618);
+    static constexpr T eps = T(1e-6);
+    static constexpr T min_rough = T(0.04);
+    static constexpr T max_rad = T(1e1);
+    static constexpr T min_term = T(0.05);
+};
+
+template <class T> inline T luminance(V3<T> c) { return dot(c, V3<T>(T(0.2126), T(0.7152), T(0.0722))); }
+template <class T> inline V3<T> srgb2lin(V3<T> c) { return vpow(c, T(2.2)); }
+
+// convertMetallic (color.glsl:30-47)
+template <class T>
+T convert_metallic(V3<T> diffuse, V3<T> specular, T max_spec) {
+    T pd = std::sqrt(T(0.299) * diffuse.x * diffuse.x + T(0.587) * diffuse.y * diffuse.y +
+                     T(0.114) * diffuse.z * diffuse.z);
+    T ps = std::sqrt(T(0.299) * specular.x * specular.x + T(0.587) * specular.y * specular.y +
+                     T(0.114) * specular.z * specular.z);
+    if (ps < K<T>::min_rough) return T(0);
+    T a = K<T>::min_rough;
+    T b = pd * (T(1) - max_spec) / (T(1) - K<T>::min_rough) + ps - T(2) * K<T>::min_rough;
+    T c = K<T>::min_rough - ps;
+    T D = gmax(b * b - T(4) * a * c, T(0));
+    return gclamp((-b + std::sqrt(D)) / (T(2) * a), T(0), T(1));
+}
+
+template <class T>
+void miss(Surface<T>& s, V3<T>& pos, uint32_t& cat) {
+    pos = V3<T>(T(1.0e10));
+    cat = 0xFFFFFFFFu;
+    s = Surface<T>{};
+    s.normal = V3<T>(T(1));
+    s.ior = T(1);
+    s.illum = 0;
+}
+
+// ptClosesthit.rchit:18-92
+template <class T>
+void closest_hit(const SceneO& s, const HitO& h, V3<T> ray_dir, Surface<T>& si, V3<T>& position, uint32_t& cat) {
+    const InstO& in = s.insts[s.tri_inst[h.tri]];
+    const MeshO& m = s.meshes[in.mesh];
+    uint32_t prim = s.tri_prim[h.tri];
+    const uint32_t* id = &m.idx[3 * (size_t)prim];
+    auto P = [&](uint32_t i) { return V3<T>(T(m.pos[3 * i]), T(m.pos[3 * i + 1]), T(m.pos[3 * i + 2])); };
+    auto N = [&](uint32_t i) { return V3<T>(T(m.nor[3 * i]), T(m.nor[3 * i + 1]), T(m.nor[3 * i + 2])); };
+    auto U = [&](uint32_t i) { return V2<T>{T(m.uv[2 * i]), T(m.uv[2 * i + 1])}; };
+    V3<T> p0 = P(id[0]), p1 = P(id[1]), p2 = P(id[2]);
+    V3<T> n0 = N(id[0]), n1 = N(id[1]), n2 = N(id[2]);
+    V2<T> uv0 = U(id[0]), uv1 = U(id[1]), uv2 = U(id[2]);
+    T hu = T(h.u), hv = T(h.v);
+    T bx = T(1) - hu - hv, by = hu, bz = hv;
+    V2<T> tc{uv0.x * bx + uv1.x * by + uv2.x * bz, uv0.y * bx + uv1.y * by + uv2.y * bz};
+    V4<T> dif = tex_fetch<T>(s, m.tex[0], tc);
+    V3<T> drgb = srgb2lin(V3<T>(dif.x, dif.y, dif.z)) * dif.w;
+    V3<T> pos = p0 * bx + p1 * by + p2 * bz;
+    V4<T> wp = mat4_mul<T>(in.m, V4<T>{pos.x, pos.y, pos.z, T(1)});
+    position = V3<T>(wp.x, wp.y, wp.z);
+    V3<T> normal = normalize(n0 * bx + n1 * by + n2 * bz);
+    if (std::isinf(normal.x) || std::isnan(normal.x)) normal = V3<T>(T(0), T(1), T(0));
+    auto nm = [&](V3<T> v) {
+        return V3<T>(T(in.nm[0]) * v.x + T(in.nm[3]) * v.y + T(in.nm[6]) * v.z,
+                     T(in.nm[1]) * v.x + T(in.nm[4]) * v.y + T(in.nm[7]) * v.z,
+                     T(in.nm[2]) * v.x + T(in.nm[5]) * v.y + T(in.nm[8]) * v.z);
+    };
+    normal = normalize(nm(normal));
+    const T e = T(1e-6);
+    if (uv0 == uv1) uv1.x += e;
+    if (uv0 == uv2) uv2.y += e;
+    if (uv1 == uv2) {
+        uv2.x += e;
+        uv2.y += e;
+    }
+    // getTangent / getBitangent (geometry.glsl:59-73)
+    auto tangent = [](V3<T> A, V3<T> B, V3<T> Cc, V2<T> Auv, V2<T> Buv, V2<T> Cuv) {
+        T bvcv = Buv.y - Cuv.y;
+        if (bvcv == T(0)) return (B - Cc) / (Buv.x - Cuv.x);
+        T q = (Auv.y - Cuv.y) / bvcv;
+        V3<T> D = Cc + (B - Cc) * q;
+        T duvx = Cuv.x + (Buv.x - Cuv.x) * q;
+        return (D - A) / (duvx - Auv.x);
+    };
+    V3<T> Tn = nm(tangent(p0, p1, p2, uv0, uv1, uv2));
+    V3<T> Bn = nm(tangent(p0, p2, p1, V2<T>{uv0.y, uv0.x}, V2<T>{uv2.y, uv2.x}, V2<T>{uv1.y, uv1.x}));
+    auto proj = [](V3<T> a, V3<T> b) { return dot(a, b) / dot(b, b) * b; };
+    V3<T> t = Tn - proj(Tn, normal);
+    V3<T> b = Bn - proj(Bn, t) - proj(Bn, normal);
+    si.basis[0] = normalize(t);
+    si.basis[1] = normalize(b);
+    si.basis[2] = normal;
+    const Tex& ntex = s.textures[m.tex[1]];
+    if (!(ntex.w == 1 && ntex.h == 1)) {
+        V4<T> tn = tex_fetch<T>(s, m.tex[1], tc);
+        V3<T> tsn(tn.x * T(2) - T(1), tn.y * T(2) - T(1), tn.z * T(2) - T(1));
+        normal = normalize(si.basis[0] * tsn.x + si.basis[1] * tsn.y + si.basis[2] * tsn.z);
+    } else {
+        normal = si.basis[2];
+    }
+    const pbrt_material& mat = m.mat;
+    drgb = drgb * V3<T>(T(mat.diffuse_ior[0]), T(mat.diffuse_ior[1]), T(mat.diffuse_ior[2]));
+    V3<T> spec;
+    T rough;
+    const Tex& stex = s.textures[m.tex[2]];
+    if (stex.w == 1 && stex.h == 1) {
+        spec = V3<T>(T(mat.specular_dissolve[0]), T(mat.specular_dissolve[1]), T(mat.specular_dissolve[2]));
+        rough = T(mat.ambient_roughness[3]);
+    } else {
+        V4<T> st = tex_fetch<T>(s, m.tex[2], tc);
+        spec = srgb2lin(V3<T>(st.x, st.y, st.z));
+        rough = st.w;
+    }
+    T max_spec = max3(spec);
+    T metallic = convert_metallic(drgb, spec, max_spec);
+    V3<T> dpart = drgb * ((T(1) - max_spec) / (T(1) - K<T>::min_rough) / gmax(T(1) - metallic, e));
+    V3<T> spart = spec - (V3<T>(K<T>::min_rough) * (T(1) - metallic) * (T(1) / gmax(metallic, e)));
+    V3<T> base = vmix(dpart, spart, metallic * metallic);
+    V3<T> f0(T(0.04));
+    V3<T> diffuse_color = base * (V3<T>(T(1)) - f0);
+    diffuse_color = diffuse_color * (T(1) - metallic);
+    V3<T> specular_color = vmix(f0, base, metallic);
+    T refl = max3(specular_color);
+    T r90 = gclamp(refl * T(25), T(0), T(1));
+    V3<T> v = normalize(-ray_dir);
+    V4<T> em = tex_fetch<T>(s, m.tex[3], tc);
+    V3<T> emissive = V3<T>(T(mat.emission_texture_id[0]), T(mat.emission_texture_id[1]),
+                           T(mat.emission_texture_id[2])) *
+                     srgb2lin(V3<T>(em.x, em.y, em.z));
+    if (dot(v, normal) < T(0)) emissive = V3<T>(T(0));
+    si.perceptual_roughness = rough;
+    si.metalness = metallic;
+    si.alpha_roughness = rough * rough;
+    si.illum = (int)mat.transmittance_illum[3];
+    si.r0 = specular_color;
+    si.r90 = V3<T>(r90);
+    si.diffuse = diffuse_color;
+    si.specular = specular_color;
+    si.emissive = emissive;
+    si.transmissive = V3<T>(T(mat.transmittance_illum[0]), T(mat.transmittance_illum[1]), T(mat.transmittance_illum[2]));
+    si.normal = normal;
+    si.ior = T(mat.diffuse_ior[3]);
+    cat = mat.category_id;
+}
+
+// brdf.glsl:71-84
+template <class T>
+T spec_weight(const Surface<T>& s) {
+    T wd = gmix(luminance(s.diffuse), T(0), s.metalness);
+    T ws = luminance(s.specular);
+    return gmin(T(1), ws / (wd + wd));
+}
+template <class T>
+T pdf_brdf(const Surface<T>& s, V3<T> l, V3<T> h) {
+    T pd = K<T>::rpi * dot(l, s.normal);
+    T a2 = s.alpha_roughness * s.alpha_roughness;
+    T nh = dot(h, s.normal);
+    T q = nh * nh * (a2 - T(1)) + T(1);
+    T ps = a2 / (K<T>::pi * (q * q));
+    ps *= dot(h, s.normal);
+    return gmix(pd, ps, spec_weight(s));
+}
+// brdf.glsl:132-164
+template <class T>
+V3<T> brdf(V3<T> v, V3<T> l, V3<T> h, const Surface<T>& s) {
+    V3<T> n = s.normal;
+    T NdotL = gclamp(dot(n, l), T(0.001), T(1));
+    T NdotV = gclamp(std::fabs(dot(n, v)), T(0.001), T(1));
+    T NdotH = gclamp(dot(n, h), T(0), T(1));
+    T VdotH = gclamp(dot(v, h), T(0), T(1));
+    V3<T> F = s.r0 + (s.r90 - s.r90 * s.r0) * std::exp2((T(-5.55473) * VdotH - T(6.98316)) * VdotH);
+    T r = s.alpha_roughness * s.alpha_roughness;
+    T aL = T(2) * NdotL / (NdotL + std::sqrt(r + (T(1) - r) * (NdotL * NdotL)));
+    T aV = T(2) * NdotV / (NdotV + std::sqrt(r + (T(1) - r) * (NdotV * NdotV)));
+    T G = aL * aV;
+    T f = (NdotH * r - NdotH) * NdotH + T(1);
+    T D = r / (K<T>::pi * f * f);
+    T Fd90 = T(0.5) + T(2) * s.perceptual_roughness * VdotH * VdotH;
+    const T dim = T(1) - T(0.1);  // min over (1 - f0) with f0 = vec3(0.1)
+    T res = ((T(1) + (Fd90 - T(1)) * std::pow(T(1) - NdotL, T(5))) * (T(1) + (Fd90 - T(1)) * std::pow(T(1) - NdotV, T(5)))) *
+            K<T>::rpi * dim;
+    V3<T> diff = (T(1) - F) * (s.diffuse * res);
+    V3<T> spec = F * G * D / (T(4) * NdotL * NdotV);
+    if (dot(n, l) <= T(0)) spec = V3<T>(T(0));
+    return diff + spec;
+}
+
+// sampling.glsl:11-78; the engine arrives by value.
+template <class T>
+V3<T> sample_brdf(const Surface<T>& s, RandomEngine re, V3<T> v, V3<T>& l, T& pdf) {
+    T ux = (T)random_float(re), uy = (T)random_float(re), uz = (T)random_float(re);
+    T sw = spec_weight(s);
+    V3<T> basis[3] = {s.basis[0], s.basis[1], s.basis[2]};
+    bool entering = dot(v, s.normal) >= T(0);
+    if (s.illum == 7 && !entering) basis[2] = basis[2] * T(-1);
+    V3<T> h;
+    auto mul = [&](V3<T> x) { return basis[0] * x.x + basis[1] * x.y + basis[2] * x.z; };
+    if (uz < sw) {
+        T a2 = s.alpha_roughness * s.alpha_roughness;
+        T phi = T(2) * K<T>::pi * ux;
+        T cosH = std::sqrt((T(1) - uy) / (T(1) + (a2 - T(1)) * uy));
+        T sinH = std::sqrt(T(1) - cosH * cosH);
+        h = mul(V3<T>(sinH * std::cos(phi), sinH * std::sin(phi), cosH));
+        l = -reflect(v, h);
+    } else {
+        T t = T(2) * K<T>::pi * uy;
+        T sr = std::sqrt(ux);
+        T dx = sr * std::cos(t), dy = sr * std::sin(t);
+        l = mul(V3<T>(dx, dy, std::sqrt(gmax(T(0), T(1) - dx * dx - dy * dy))));
+        h = normalize(v + l);
+    }
+    pdf = pdf_brdf(s, l, h);
+    V3<T> b = brdf(v, l, h, s);
+    if (s.illum == 7) {
+        T f = T(1) + (T(0) - T(1)) * std::pow(gclamp(T(1) - dot(v, h), T(0), T(1)), T(5));
+        if ((T)random_float(re) < f || !entering) {
+            T t = !entering ? s.ior : T(1) / s.ior;
+            T cosI = dot(basis[2], l);
+            T sinT2 = t * t * (T(1) - cosI * cosI);
+            if (sinT2 <= T(1)) {
+                l = l - T(2) * cosI * basis[2];
+                T cosT = std::sqrt(T(1) - sinT2);
+                l = t * l + (t * cosI - cosT) * basis[2];
+            }
+            b = s.transmissive;
+            pdf = T(1);
+        } else {
+            pdf = T(1);
+            b = V3<T>(T(1));
+        }
+    }
+    return b;
+}
+
+template <class T>
+V3<T> sky(V3<T> d) {
+    V3<T> up = srgb2lin(V3<T>(T(0.3), T(0.5), T(0.92)));
+    up = vmix(V3<T>(T(1)), up, gmax(d.z, T(0)));
+    V3<T> lo(T(0.2));
+    T t = gclamp((d.z - T(-0.02)) / (T(0.02) - T(-0.02)), T(0), T(1));
+    T w = t * t * (T(3) - T(2) * t);
+    return vmix(lo, up, w);
+}
+
+struct Params {
+    uint32_t width, height, max_depth, min_depth, light_mode, output, write_gbuffer;
+    float inv_view[16], inv_proj[16];
+    uint32_t frame, sample;
+};
+
+// sampleLight, three variants (lighting.glsl:18-227).  Returns false when the
+// function returns vec3(0) before tracing a shadow ray.
+template <class T>
+bool sample_light(const SceneO& sc, uint32_t mode, V3<T> pos, V3<T> n, RandomEngine& re, V3<T>& col, V3<T>& l,
+                  T& pdf, T& tmax) {
+    const T tmin = T(0.001);
+    auto P3 = [](const float* f) { return V3<T>(T(f[0]), T(f[1]), T(f[2])); };
+    auto att = [&](const pbrt_light& li, T d) {
+        return T(1) / (T(li.strengths[0]) + T(li.strengths[1]) * d + T(li.strengths[2]) * d * d);
+    };
+    const uint32_t count = (uint32_t)sc.lights.size();
+    if (mode == 0) {
+        tmax = T(1000);
+        l = V3<T>(T(0));
+        T ssum = 0, picked = 0;
+        V3<T> picked_col;
+        for (uint32_t i = 0; i < count; ++i) {
+            const pbrt_light& li = sc.lights[i];
+            T power = (T(li.col_ambient[0]) + T(li.col_diffuse[0]) + T(li.col_specular[0])) +
+                      (T(li.col_ambient[1]) + T(li.col_diffuse[1]) + T(li.col_specular[1])) +
+                      (T(li.col_ambient[2]) + T(li.col_diffuse[2]) + T(li.col_specular[2])) +
+                      (T(li.col_ambient[3]) + T(li.col_diffuse[3]) + T(li.col_specular[3]));
+            T strength = 0;
+            V3<T> ls = P3(li.col_ambient) + P3(li.col_diffuse) + P3(li.col_specular);
+            T cur_tmax = T(1000);
+            V3<T> cur_l;
+            int type = (int)li.v0_type[3];
+            if (type == 1) {
+                T d = length(pos - P3(li.v0_type));
+                T a = att(li, d);
+                V3<T> nd = -P3(li.dir_angle2);
+                strength = gmax(dot(n, nd), T(0)) * power * a;
+                ls = ls * (dot(n, nd) * a);
+                cur_l = normalize(nd);
+            } else if (type == 2) {
+                T d = length(pos - P3(li.v0_type));
+                T a = att(li, d);
+                V3<T> to = normalize(P3(li.v0_type) - pos);
+                strength = gmax(dot(n, to), T(0)) * power * a;
+                ls = ls * (dot(n, to) * a);
+                cur_l = to;
+            } else if (type == 5) {
+                T r0 = (T)random_float(re), r1 = (T)random_float(re);
+                T sx = std::sqrt(r0);
+                T bx = T(1) - sx, by = r1 * sx;
+                V3<T> p1 = P3(li.v0_type), p2 = P3(li.v1_strength), p3 = P3(li.v2_angle);
+                V3<T> lp = (T(1) - bx - by) * p1 + bx * p2 + by * p3;
+                V3<T> ld = lp - pos;
+                V3<T> ln = normalize(cross(p2 - p1, p3 - p1));
+                T d = length(ld);
+                ld = ld / d;
+                T a = att(li, d);
+                T cs = gmax(dot(n, ld), T(0)), cl = gmax(dot(-ld, ln), T(0));
+                strength = cs * cl * power * a;
+                ls = ls * (cs * cl * a);
+                cur_l = ld;
+                cur_tmax = d - tmin;
+            }
+            ssum += strength;
+            if ((T)random_float(re) < strength / ssum) {
+                picked = strength;
+                picked_col = ls;
+                tmax = cur_tmax;
+                l = cur_l;
+            }
+        }
+        if (ssum < T(1e-6) || picked < T(1e-6)) return false;
+        pdf = picked / ssum;
+        col = picked_col;
+        return true;
+    }
+    int i;
+    T lstrength = 1;
+    if (mode == 1) {
+        T rnd = (T)random_float(re);
+        T pk = rnd * T(sc.strength_sum);
+        int b = 0, e = (int)count, mid = (b + e) / 2, c = 0;
+        while (e - b > 1 && ++c < 100) {
+            if (pk <= T(sc.lights[mid].strengths[3]))
+                e = mid + 1;
+            else
+                b = mid;
+            mid = (b + e) / 2;
+        }
+        lstrength = T(sc.lights[b].strengths[3]);
+        if (b > 0) lstrength -= T(sc.lights[b - 1].strengths[3]);
+        i = b;
+    } else {
+        float rnd = random_float(re);
+        i = (int)(rnd * (float)count) - (int)rnd;
+    }
+    const pbrt_light& li = sc.lights[i];
+    V3<T> ls = P3(li.col_ambient) + P3(li.col_diffuse) + P3(li.col_specular);
+    tmax = T(1000);
+    int type = (int)li.v0_type[3];
+    l = V3<T>(T(0));
+    if (type == 1) {
+        T d = length(pos - P3(li.v0_type));
+        ls = ls * (gmax(dot(n, -P3(li.dir_angle2)), T(0)) * att(li, d));
+        l = normalize(-P3(li.dir_angle2));
+    } else if (type == 2) {
+        T d = length(pos - P3(li.v0_type));
+        ls = ls * (gmax(dot(n, normalize(P3(li.v0_type) - pos)), T(0)) * att(li, d));
+        l = normalize(P3(li.v0_type) - pos);
+    } else if (type == 5) {
+        T r0 = (T)random_float(re), r1 = (T)random_float(re);
+        T sx = std::sqrt(r0);
+        T bx = T(1) - sx, by = r1 * sx;
+        V3<T> p1 = P3(li.v0_type), p2 = P3(li.v1_strength), p3 = P3(li.v2_angle);
+        V3<T> lp = (T(1) - bx - by) * p1 + bx * p2 + by * p3;
+        V3<T> ld = lp - pos;
+        V3<T> lnr = cross(p2 - p1, p3 - p1);
+        T area = T(0.5) * length(lnr);
+        V3<T> ln = normalize(lnr);
+        T d = length(ld);
+        ld = ld / d;
+        ls = ls * (gmax(dot(n, ld), T(0)) * gmax(dot(-ld, ln), T(0)) * att(li, d) * area);
+        l = ld;
+        tmax = d - tmin;
+    }
+    if (length(ls) < T(1e-6)) return false;
+    pdf = T(1);
+    col = mode == 1 ? ls * T(sc.strength_sum) / lstrength : ls * T(count);
+    return true;
+}
+
+struct Stats {
+    uint64_t rays = 0, shadow_rays = 0;
+};
+
+template <class T>
+HitO trace_ray(const SceneO& sc, V3<T> o, V3<T> d, T tmin, T tmax, bool any, bool brute) {
+    double od[3] = {(double)o.x, (double)o.y, (double)o.z}, dd[3] = {(double)d.x, (double)d.y, (double)d.z};
+    return query(sc, od, dd, (double)tmin, (double)tmax, any, brute);
+}
+
+// nextEventEsitmation (lighting.glsl:246-261)
+template <class T>
+V3<T> nee(const SceneO& sc, const Params& p, V3<T> pos, V3<T> o, const Surface<T>& s, V3<T> tp, RandomEngine& re,
+          bool brute, Stats& st) {
+    if (s.normal == V3<T>(T(1))) return sky(-o) * tp;
+    V3<T> col, l;
+    T pdf, tmax;
+    if (!sample_light(sc, p.light_mode, pos, s.normal, re, col, l, pdf, tmax)) return V3<T>(T(0));
+    st.shadow_rays++;
+    HitO sh = trace_ray<T>(sc, pos, l, T(0.001), tmax, true, brute);
+    if (sh.tri >= 0) return V3<T>(T(0));
+    if (col == V3<T>(T(0))) return col;
+    V3<T> lh = normalize(l + o);
+    V3<T> b = brdf(o, l, lh, s);
+    T bp = pdf_brdf(s, l, lh);
+    T f = pdf * pdf, g = bp * bp;
+    T w = f / (f + g);
+    V3<T> light = col * b * w / pdf;
+    return vmin(tp * light, K<T>::max_rad);
+}
+
+struct PixelOut {
+    float depth, nx, ny;
+    uint8_t material[4], albedo[4];
+    float illum[4];
+    double radiance[3];  // clamped radiance before demodulation
+};
+
+// ptRaygen.rgen main
+template <class T>
+void trace_pixel(const SceneO& sc, const Params& p, uint32_t px, uint32_t py, bool brute, const float* prev_final,
+                 PixelOut& out, Stats& st) {
+    RandomEngine re = re_init(px, py, p.frame);
+    V3<T> tp(T(1));
+    bool aa = p.output == 1 && p.sample > 0;
+    // createRay (camera.glsl:6-16)
+    T cx = T(px) + T(0.5), cy = T(py) + T(0.5);
+    if (aa) {
+        T jx = (T)random_float(re), jy = (T)random_float(re);
+        cx = cx + (jx - T(0.5));
+        cy = cy + (jy - T(0.5));
+    }
+    T nx = cx / T(p.width), ny = cy / T(p.height);
+    T clx = nx * T(2) - T(1), cly = ny * T(2) - T(1);
+    V3<T> cam(T(p.inv_view[12]), T(p.inv_view[13]), T(p.inv_view[14]));
+    V4<T> dv = mat4_mul<T>(p.inv_proj, V4<T>{clx, cly, T(1), T(1)});
+    V3<T> dn = normalize(V3<T>(dv.x, dv.y, dv.z));
+    V4<T> dw = mat4_mul<T>(p.inv_view, V4<T>{dn.x, dn.y, dn.z, T(0)});
+    V3<T> dir(dw.x, dw.y, dw.z);
+    Surface<T> s;
+    V3<T> hp;
+    uint32_t cat;
+    st.rays++;
+    HitO h = trace_ray<T>(sc, cam, dir, T(0.001), T(10000), false, brute);
+    if (h.tri >= 0)
+        closest_hit<T>(sc, h, dir, s, hp, cat);
+    else
+        miss<T>(s, hp, cat);
+    V3<T> color(T(0));
+    color = color + nee<T>(sc, p, hp, -normalize(dir), s, tp, re, brute, st);
+    color = color + s.emissive;
+    V3<T> albedo = s.diffuse + s.specular;
+    out.depth = (float)length(hp - cam);
+    out.nx = (float)std::acos(s.normal.z);
+    out.ny = (float)std::atan2(s.normal.y, s.normal.x);
+    out.material[0] = unorm8((float)cat / 255.0f);
+    out.material[1] = out.material[2] = out.material[3] = 0;
+    out.albedo[0] = unorm8((float)albedo.x);
+    out.albedo[1] = unorm8((float)albedo.y);
+    out.albedo[2] = unorm8((float)albedo.z);
+    out.albedo[3] = 255;
+    if (!(s.normal == V3<T>(T(1)))) {
+        int trans = 0;
+        V3<T> wpos = cam;
+        for (int i = 0; i < (int)p.max_depth && trans < 10; ++i) {
+            if (s.illum == 7) {
+                --i;
+                ++trans;
+            }
+            V3<T> v = normalize(wpos - hp);
+            wpos = hp;
+            V3<T> l;
+            T pdf;
+            V3<T> b = sample_brdf<T>(s, re, v, l, pdf);
+            if (b == V3<T>(T(0)) || pdf < K<T>::eps) break;
+            T t = dot(l, s.normal);
+            if (s.illum == 7) t = T(1);
+            V3<T> ptp = tp * (b * t) / pdf;
+            if ((uint32_t)i > p.min_depth) {
+                if ((T)random_float(re) < K<T>::min_term) break;
+                ptp = ptp / (T(1) - K<T>::min_term);
+            }
+            tp = ptp;
+            st.rays++;
+            HitO hh = trace_ray<T>(sc, wpos, l, T(0.001), T(10000), false, brute);
+            if (hh.tri >= 0)
+                closest_hit<T>(sc, hh, l, s, hp, cat);
+            else
+                miss<T>(s, hp, cat);
+            color = color + nee<T>(sc, p, hp, -l, s, tp, re, brute, st) + s.emissive * tp;
+            if (s.normal == V3<T>(T(1))) break;
+        }
+    }
+    color = vclamp(color, T(0), K<T>::max_rad);
+    out.radiance[0] = (double)color.x;
+    out.radiance[1] = (double)color.y;
+    out.radiance[2] = (double)color.z;
+    if (p.output == 0) {
+        V3<T> dm = color;
+        if (!std::isinf(hp.x)) dm = vmin(color / (albedo + V3<T>(K<T>::eps)), T(1e3));
+        out.illum[0] = (float)dm.x;
+        out.illum[1] = (float)dm.y;
+        out.illum[2] = (float)dm.z;
+        out.illum[3] = 1;
+    } else {
+        V3<T> fc = color;
+        if (p.sample > 0) {
+            const float* pr = prev_final;
+            T alpha = T(1) / T(p.sample + 1);
+            fc = vmix(srgb2lin(V3<T>(T(pr[0]), T(pr[1]), T(pr[2]))), fc, alpha);
+        }
+        fc = vclamp(vpow(fc, T(1) / T(2.2)), T(0), T(1));
+        out.illum[0] = (float)fc.x;
+        out.illum[1] = (float)fc.y;
+        out.illum[2] = (float)fc.z;
+        out.illum[3] = 1;
+    }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// random.glsl stream: out[(y*w + x)*count + k] = bits of the k-th randomFloat
+void orc_rng_sequence(uint32_t w, uint32_t h, uint32_t frame, uint32_t count, uint32_t* out) {
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            RandomEngine re = re_init(x, y, frame);
+            for (uint32_t k = 0; k < count; ++k) out[((size_t)y * w + x) * count + k] = f2u(random_float(re));
+        }
+}
+
+uint32_t orc_wang_hash(uint32_t s) { return wang_hash(s); }
+
+void* orc_scene_create(const pbrt_scene_desc* d) { return pack(*d); }
+void orc_scene_destroy(void* s) { delete (SceneO*)s; }
+uint32_t orc_scene_light_count(void* s) { return (uint32_t)((SceneO*)s)->lights.size(); }
+float orc_scene_strength_sum(void* s) { return ((SceneO*)s)->strength_sum; }
+uint32_t orc_scene_triangle_count(void* s) { return (uint32_t)((SceneO*)s)->tri_inst.size(); }
+void orc_scene_lights(void* s, pbrt_light* out) {
+    SceneO* sc = (SceneO*)s;
+    std::memcpy(out, sc->lights.data(), sc->lights.size() * sizeof(pbrt_light));
+}
+
+// rays: n x 8 floats (o, tmin, d, tmax).  out_tri: instance, primitive (or -1),
+// out_tuv: t, u, v in double, out_edge: min(u, v, 1-u-v) of the accepted hit.
+void orc_intersect(void* s, const float* rays, uint32_t n, int32_t brute, int32_t any_hit, int32_t* out_ip,
+                   double* out_tuv) {
+    SceneO* sc = (SceneO*)s;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * (size_t)i;
+        double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+        HitO h = query(*sc, o, d, r[3], r[7], any_hit != 0, brute != 0);
+        out_ip[2 * i] = h.tri >= 0 ? (int32_t)sc->tri_inst[h.tri] : -1;
+        out_ip[2 * i + 1] = h.tri >= 0 ? (int32_t)sc->tri_prim[h.tri] : -1;
+        out_tuv[3 * i] = h.t;
+        out_tuv[3 * i + 1] = h.u;
+        out_tuv[3 * i + 2] = h.v;
+    }
+}
+
+// Trace rows [y0, y1) of a frame.  precision 32 = FP32 mirror, 64 = FP64.
+// Outputs are full-size images (only the traced rows are written):
+// depth W*H, normal 2*W*H, material/albedo 4*W*H bytes, illumination 4*W*H,
+// radiance 3*W*H doubles (may be NULL).  stats[0] rays, stats[1] shadow rays.
+void orc_trace(void* s, const uint32_t* params, const float* inv_view, const float* inv_proj, int32_t precision,
+               int32_t brute, uint32_t y0, uint32_t y1, int32_t threads, float* depth, float* normal,
+               uint8_t* material, uint8_t* albedo, float* illum, double* radiance, uint64_t* stats) {
+    SceneO* sc = (SceneO*)s;
+    Params p{};
+    p.width = params[0];
+    p.height = params[1];
+    p.max_depth = params[2];
+    p.min_depth = params[3];
+    p.light_mode = params[4];
+    p.output = params[5];
+    p.write_gbuffer = params[6];
+    p.frame = params[7];
+    p.sample = params[8];
+    std::memcpy(p.inv_view, inv_view, sizeof(p.inv_view));
+    std::memcpy(p.inv_proj, inv_proj, sizeof(p.inv_proj));
+    if (threads < 1) threads = 1;
+    std::atomic<uint32_t> next_row{y0};
+    std::atomic<uint64_t> rays{0}, shadow{0};
+    auto worker = [&]() {
+        Stats st;
+        for (;;) {
+            uint32_t y = next_row.fetch_add(1);
+            if (y >= y1) break;
+            for (uint32_t x = 0; x < p.width; ++x) {
+                size_t pix = (size_t)y * p.width + x;
+                PixelOut o{};
+                const float* prev = illum ? illum + 4 * pix : nullptr;
+                if (precision == 64)
+                    trace_pixel<double>(*sc, p, x, y, brute != 0, prev, o, st);
+                else
+                    trace_pixel<float>(*sc, p, x, y, brute != 0, prev, o, st);
+                if (depth) depth[pix] = o.depth;
+                if (normal) {
+                    normal[2 * pix] = o.nx;
+                    normal[2 * pix + 1] = o.ny;
+                }
+                if (material) std::memcpy(material + 4 * pix, o.material, 4);
+                if (albedo) std::memcpy(albedo + 4 * pix, o.albedo, 4);
+                if (illum) std::memcpy(illum + 4 * pix, o.illum, 16);
+                if (radiance) std::memcpy(radiance + 3 * pix, o.radiance, 24);
+            }
+        }
+        rays += st.rays;
+        shadow += st.shadow_rays;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    if (stats) {
+        stats[0] = rays.load();
+        stats[1] = shadow.load();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+"""The C ABI library loads on a machine without a GPU and exports every
+function declared in include/*.h; struct layouts match the reference's
+std430 sizes (PipelineStructs.hpp:6-13, Light.h:23-37, ptStructures.glsl:18-27)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+from vulkanpbrt_amd import capi
+
+ROOT = Path(__file__).resolve().parents[1]
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(pbrt_[a-z0-9_]+)\s*\(", re.M)
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        names.update(DECL.findall(h.read_text()))
+    return sorted(names)
+
+
+def test_headers_declare_functions():
+    names = declared_functions()
+    assert "pbrt_trace" in names and "pbrt_scene_create" in names
+    assert len(names) >= 10
+
+
+def test_every_declared_symbol_is_exported(pbrt_lib):
+    missing = [n for n in declared_functions() if not hasattr(pbrt_lib, n)]
+    assert not missing, f"symbols declared in include/*.h but not exported: {missing}"
+
+
+def test_ctypes_signatures_cover_header():
+    assert set(declared_functions()) == set(capi.SIGNATURES), \
+        set(declared_functions()) ^ set(capi.SIGNATURES)
+
+
+def test_abi_version(pbrt_lib):
+    assert pbrt_lib.pbrt_abi_version() == 1
+
+
+def test_struct_sizes():
+    assert C.sizeof(capi.PushConstants) == 200
+    assert C.sizeof(capi.Light) == 128
+    assert C.sizeof(capi.Material) == 96
+
+
+def test_null_arguments_are_rejected_without_gpu(pbrt_lib):
+    # argument validation runs before any device call
+    assert pbrt_lib.pbrt_scene_create(None, None) == -1
+    assert b"NULL" in pbrt_lib.pbrt_last_error()
+    assert pbrt_lib.pbrt_trace(None, None, None, None, None, None) == -1
+
+
+def test_generated_scene_config1_counts(pbrt_lib):
+    g = capi.GeneratedScene(1, 1)
+    a = g.arrays()
+    ntri = sum(len(m["indices"]) // 3 for m in a["meshes"])
+    # 12 quads (24 triangles) + 16 icospheres of 320 triangles
+    assert ntri == 24 + 16 * 320
+    assert g.desc.instance_count == 23
+
+
+def test_generated_scene_unknown_config(pbrt_lib):
+    h = C.c_void_p()
+    assert pbrt_lib.pbrt_generate_scene(99, 1, C.byref(h)) == -2

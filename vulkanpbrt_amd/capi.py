@@ -1,0 +1,289 @@
+"""ctypes view of the C ABI in ``include/pbrt_c.h`` and ``include/pbrt_scenegen.h``.
+
+The library ``libpbrt.so`` is built in-tree by :mod:`vulkanpbrt_amd.build`.
+Loading fails loudly when it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libpbrt.so"
+
+PBRT_OK = 0
+
+# enums (pbrt_c.h)
+PBRT_BVH_SAH_HOST = 0
+PBRT_BVH_LBVH_GPU = 1
+LIGHT_SAMPLE_SURFACE_STRENGTH = 0
+LIGHT_SAMPLE_LIGHT_STRENGTH = 1
+LIGHT_SAMPLE_UNIFORM = 2
+LIGHT_SAMPLE_AUTO = 0xFFFFFFFF
+OUTPUT_DEMOD_ILLUMINATION_FLOAT = 0
+OUTPUT_FINAL_IMAGE = 1
+TRACER_MEGAKERNEL = 0
+TRACER_WAVEFRONT = 1
+
+
+class PushConstants(C.Structure):
+    _fields_ = [
+        ("view_inverse", C.c_float * 16),
+        ("proj_inverse", C.c_float * 16),
+        ("prev_view", C.c_float * 16),
+        ("frame_number", C.c_uint32),
+        ("sample_number", C.c_uint32),
+    ]
+
+
+class Light(C.Structure):
+    _fields_ = [(n, C.c_float * 4) for n in (
+        "v0_type", "v1_strength", "v2_angle", "dir_angle2",
+        "col_ambient", "col_diffuse", "col_specular", "strengths")]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("ambient_roughness", C.c_float * 4),
+        ("diffuse_ior", C.c_float * 4),
+        ("specular_dissolve", C.c_float * 4),
+        ("transmittance_illum", C.c_float * 4),
+        ("emission_texture_id", C.c_float * 4),
+        ("category_id", C.c_uint32),
+        ("pad", C.c_float * 3),
+    ]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba8", C.c_void_p)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("vertex_count", C.c_uint32),
+        ("positions", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("uvs", C.c_void_p),
+        ("index_count", C.c_uint32),
+        ("index_stride", C.c_uint32),
+        ("indices", C.c_void_p),
+        ("material", Material),
+        ("diffuse_texture", C.c_int32),
+        ("normal_texture", C.c_int32),
+        ("specular_texture", C.c_int32),
+        ("emissive_texture", C.c_int32),
+    ]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("mesh", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("mesh_count", C.c_uint32),
+        ("meshes", C.POINTER(MeshDesc)),
+        ("instance_count", C.c_uint32),
+        ("instances", C.POINTER(InstanceDesc)),
+        ("texture_count", C.c_uint32),
+        ("textures", C.POINTER(TextureDesc)),
+        ("light_count", C.c_uint32),
+        ("lights", C.POINTER(Light)),
+        ("bvh_builder", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("triangle_count", C.c_uint32),
+        ("instance_count", C.c_uint32),
+        ("mesh_count", C.c_uint32),
+        ("light_count", C.c_uint32),
+        ("light_strength_sum", C.c_float),
+        ("bvh_node_count", C.c_uint32),
+        ("bvh_depth", C.c_uint32),
+        ("light_sampling", C.c_uint32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "width", "height", "max_recursion_depth", "min_recursion_depth",
+        "light_sampling", "output", "write_gbuffer", "tracer")]
+
+
+class TraceImages(C.Structure):
+    _fields_ = [
+        ("depth", C.c_void_p),
+        ("normal", C.c_void_p),
+        ("material", C.c_void_p),
+        ("albedo", C.c_void_p),
+        ("illumination", C.c_void_p),
+    ]
+
+
+class CameraPose(C.Structure):
+    _fields_ = [
+        ("eye", C.c_float * 3),
+        ("center", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("fovy_degrees", C.c_float),
+        ("near_distance", C.c_float),
+        ("far_distance", C.c_float),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/*.h
+SIGNATURES = {
+    "pbrt_abi_version": (C.c_int32, []),
+    "pbrt_last_error": (C.c_char_p, []),
+    "pbrt_scene_create": (C.c_int32, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "pbrt_scene_destroy": (C.c_int32, [C.c_void_p]),
+    "pbrt_scene_get_info": (C.c_int32, [C.c_void_p, C.POINTER(SceneInfo)]),
+    "pbrt_scene_get_lights": (C.c_int32, [C.c_void_p, C.POINTER(Light), C.c_uint32]),
+    "pbrt_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
+                               C.POINTER(TraceImages), C.c_void_p, C.c_void_p]),
+    "pbrt_intersect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbrt_occluded": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pbrt_rng_sequence": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pbrt_generate_scene": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pbrt_generated_scene_desc": (C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "pbrt_generated_scene_camera": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(CameraPose)]),
+    "pbrt_generated_scene_destroy": (C.c_int32, [C.c_void_p]),
+}
+
+_lib: C.CDLL | None = None
+
+
+class PbrtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pbrt status {status}: {message}")
+        self.status = status
+
+
+def lib() -> C.CDLL:
+    """Load ``libpbrt.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m vulkanpbrt_amd.build` "
+                "(there is no CPU fallback for the HIP path)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != PBRT_OK:
+        raise PbrtError(status, lib().pbrt_last_error().decode())
+
+
+def ptr(t) -> int:
+    """Device (torch tensor) or host (numpy array) address as an int."""
+    if t is None:
+        return 0
+    if hasattr(t, "data_ptr"):
+        return int(t.data_ptr())
+    return int(t.ctypes.data)
+
+
+# --------------------------------------------------------------------------
+# Scenes
+# --------------------------------------------------------------------------
+def _np_from(ptr_value, count, dtype):
+    if not ptr_value or count == 0:
+        return None
+    buf = (C.c_byte * (count * np.dtype(dtype).itemsize)).from_address(ptr_value)
+    return np.frombuffer(buf, dtype=dtype).copy()
+
+
+class GeneratedScene:
+    """A seed-generated workload (pbrt_generate_scene)."""
+
+    def __init__(self, config: int, seed: int):
+        h = C.c_void_p()
+        check(lib().pbrt_generate_scene(config, seed, C.byref(h)))
+        self.handle = h
+        self.config = config
+        self.seed = seed
+        self.desc = SceneDesc()
+        check(lib().pbrt_generated_scene_desc(self.handle, C.byref(self.desc)))
+
+    def camera(self, frame: int = 0) -> CameraPose:
+        pose = CameraPose()
+        check(lib().pbrt_generated_scene_camera(self.handle, frame, C.byref(pose)))
+        return pose
+
+    def arrays(self) -> dict:
+        """Host copies of the scene description (input of the oracle)."""
+        d = self.desc
+        meshes = []
+        for i in range(d.mesh_count):
+            m = d.meshes[i]
+            idt = np.uint16 if m.index_stride == 2 else np.uint32
+            meshes.append(dict(
+                positions=_np_from(m.positions, 3 * m.vertex_count, np.float32),
+                normals=_np_from(m.normals, 3 * m.vertex_count, np.float32),
+                uvs=_np_from(m.uvs, 2 * m.vertex_count, np.float32),
+                indices=_np_from(m.indices, m.index_count, idt),
+                material=np.frombuffer(bytes(m.material), dtype=np.uint8).copy(),
+                textures=np.array([m.diffuse_texture, m.normal_texture, m.specular_texture,
+                                   m.emissive_texture], dtype=np.int32),
+            ))
+        instances = [(np.array(d.instances[i].transform, dtype=np.float32), d.instances[i].mesh)
+                     for i in range(d.instance_count)]
+        textures = []
+        for i in range(d.texture_count):
+            t = d.textures[i]
+            textures.append(_np_from(t.rgba8, 4 * t.width * t.height, np.uint8).reshape(t.height, t.width, 4))
+        lights = np.frombuffer(bytes((Light * d.light_count).from_address(C.addressof(d.lights.contents)))
+                               if d.light_count else b"", dtype=np.float32).reshape(-1, 32).copy()
+        return dict(meshes=meshes, instances=instances, textures=textures, lights=lights)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().pbrt_generated_scene_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Scene:
+    """A packed, BVH-built scene resident in HBM (pbrt_scene_create)."""
+
+    def __init__(self, desc: SceneDesc, keepalive=None):
+        h = C.c_void_p()
+        check(lib().pbrt_scene_create(C.byref(desc), C.byref(h)))
+        self.handle = h
+        self._keepalive = keepalive
+
+    def info(self) -> SceneInfo:
+        inf = SceneInfo()
+        check(lib().pbrt_scene_get_info(self.handle, C.byref(inf)))
+        return inf
+
+    def lights(self) -> np.ndarray:
+        n = self.info().light_count
+        arr = (Light * n)()
+        check(lib().pbrt_scene_get_lights(self.handle, arr, n))
+        return np.frombuffer(bytes(arr), dtype=np.float32).reshape(n, 32).copy()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().pbrt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,526 @@
+// BMFR (blockwise multi-order feature regression) on gfx950.
+//
+// Block geometry, features, noise and the post pass follow bmfrGeneral.comp,
+// bmfrPre.comp, bmfrFit.comp and bmfrPost.comp (POSITION_TYPE 0: screen-space
+// block coordinates plus per-block normalised depth):
+//   features  [1, n.x, n.y, n.z, x, y, z, x^2, y^2, z^2 | r, g, b]
+//   fit rows  row i of block (bx, by) is padded pixel (bx*b + i / b, by*b + i % b)
+//   noise     feature j < 10 gets + 2e-4 * (hash(i + j*B^2 + frame*13*B^2) - 0.5),
+//             B = b*b, integer wrap, hash = bmfrGeneral.comp random()
+//
+// Two fits:
+//   k_bmfr_fit_householder   the reference's FP32 Householder QR over the fp16
+//                            feature buffer, subgroup reductions on wave64
+//   k_bmfr_fit_gram          pre + fit fused: features stay in registers; the
+//                            columns are projected against the noised constant
+//                            column a0 in FP64, scaled by powers of two, split into
+//                            fp16 hi + lo and their Gram matrix is accumulated with
+//                            v_mfma_f32_16x16x32_f16 (FP32 inside a 32-row chunk,
+//                            FP64 across chunks); FP64 Cholesky solves for w1..w9,
+//                            w0 = a0.(y - A'w') / a0.a0.
+#include <hip/hip_runtime.h>
+
+#include "capi_common.hpp"
+#include "image_ops.hpp"
+
+namespace pbrt {
+
+constexpr int kAlphaSize = 13;  // features + rgb
+constexpr int kFeatures = 10;
+
+struct BmfrGeom {
+    int width, height, bs, wp, hp, blocks_x, blocks_y;
+    int offx, offy;
+    uint32_t frame;
+    int fit_threads;
+    uint16_t* features;  // [13][hp][wp] fp16
+    float* weights;      // [blocks_y][blocks_x][30]
+};
+
+__device__ __forceinline__ int mirror_coord(int x, int s) {
+    if (x < 0) return -x - 1;
+    if (x >= s) return 2 * s - x - 1;
+    return x;
+}
+
+// random(uint) of bmfrGeneral.comp:83-93 (Jenkins' 6-step integer hash)
+__device__ __forceinline__ float bmfr_random(uint32_t a) {
+    a = (a + 0x7ed55d16u) + (a << 12);
+    a = (a ^ 0xc761c23cu) ^ (a >> 19);
+    a = (a + 0x165667b1u) + (a << 5);
+    a = (a + 0xd3a2646cu) ^ (a << 9);
+    a = (a + 0xfd7046c5u) + (a << 3);
+    a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+    return __fdiv_rn((float)a, 4294967296.0f);
+}
+
+// addRandom (bmfrGeneral.comp:95-97) without contraction
+__device__ __forceinline__ float add_noise(float v, uint32_t key) {
+    float r = __fsub_rn(bmfr_random(key), 0.5f);
+    return __fadd_rn(v, __fmul_rn(2e-4f, r));
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup reductions (wave64 butterfly, then thread 0 folds the waves in order)
+// ---------------------------------------------------------------------------
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce(T v, Op op) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off));
+    return v;
+}
+
+template <class T, class Op>
+__device__ T block_reduce(T v, Op op, T* scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = (blockDim.x + 63) >> 6;
+    v = wave_reduce(v, op);
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T t = scratch[0];
+        for (int i = 1; i < nwaves; ++i) t = op(t, scratch[i]);
+        scratch[0] = t;
+    }
+    __syncthreads();
+    T r = scratch[0];
+    __syncthreads();
+    return r;
+}
+
+struct OpMin {
+    __device__ float operator()(float a, float b) const { return fminf(a, b); }
+};
+struct OpMax {
+    __device__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct OpAdd {
+    template <class T>
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+
+struct PixelIn {
+    float depth;
+    vec3 normal;
+    vec3 noisy;
+    int img_x, img_y;
+    bool inside;  // curAbsolutPos == curImagePos
+};
+
+__device__ __forceinline__ PixelIn load_pixel(const BmfrGeom& g, const pbrt_bmfr_images& im, int gx, int gy) {
+    PixelIn p;
+    int ax = gx - g.offx, ay = gy - g.offy;
+    p.img_x = mirror_coord(ax, g.width);
+    p.img_y = mirror_coord(ay, g.height);
+    p.inside = ax == p.img_x && ay == p.img_y;
+    size_t pix = (size_t)p.img_y * g.width + p.img_x;
+    p.depth = im.depth[pix];
+    float2 n = reinterpret_cast<const float2*>(im.normal)[pix];
+    p.normal = decode_normal(n.x, n.y);
+    p.noisy = xyz(load_half4(im.noisy, pix));
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// bmfrPre: 13 features of every padded pixel to the fp16 feature buffer
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_bmfr_pre(BmfrGeom g, pbrt_bmfr_images im) {
+    __shared__ float scratch[16];
+    const int lx = threadIdx.x % g.bs, ly = threadIdx.x / g.bs;
+    const int gx = blockIdx.x * g.bs + lx, gy = blockIdx.y * g.bs + ly;
+    PixelIn p = load_pixel(g, im, gx, gy);
+    float mn = block_reduce(p.depth, OpMin(), scratch);
+    float mx = block_reduce(p.depth, OpMax(), scratch);
+    float z = (p.depth - mn) / (mx - mn + 1e-6f);
+    float px = (float)lx / (float)(g.bs - 1), py = (float)ly / (float)(g.bs - 1);
+    float f[kAlphaSize] = {1.0f, p.normal.x, p.normal.y, p.normal.z, px, py, z, px * px, py * py, z * z,
+                           p.noisy.x, p.noisy.y, p.noisy.z};
+    const size_t plane = (size_t)g.wp * g.hp;
+    const size_t at = (size_t)gy * g.wp + gx;
+#pragma unroll
+    for (int i = 0; i < kAlphaSize; ++i) g.features[i * plane + at] = f2h(f[i]);
+}
+
+// ---------------------------------------------------------------------------
+// bmfrFit: FP32 Householder QR of the (bs^2 x 13) block, reference algorithm
+// ---------------------------------------------------------------------------
+template <int kSub>
+__global__ void __launch_bounds__(256) k_bmfr_fit_householder(BmfrGeom g) {
+    __shared__ float scratch[16];
+    __shared__ float u_length_squared;
+    __shared__ vec3 ws[kFeatures];
+    const int id = threadIdx.x;
+    const int T = blockDim.x;
+    const int bs = g.bs;
+    const uint32_t pb = (uint32_t)(bs * bs);
+    const int basex = blockIdx.x * bs, basey = blockIdx.y * bs;
+    const size_t plane = (size_t)g.wp * g.hp;
+    float feat[kSub][kAlphaSize];
+    for (int col = 0; col < kAlphaSize; ++col)
+        for (int s = 0; s < kSub; ++s) {
+            const int index = id + s * T;
+            const int px = index / bs + basex, py = index % bs + basey;
+            float v = h2f(g.features[col * plane + (size_t)py * g.wp + px]);
+            if (col < kFeatures)
+                v = add_noise(v, (uint32_t)index + (uint32_t)col * pb * pb + g.frame * (uint32_t)kAlphaSize * pb * pb);
+            feat[s][col] = v;
+        }
+    for (int col = 0; col < kFeatures; ++col) {
+        float u[kSub];
+        float val2 = 0.0f;
+        for (int s = 0; s < kSub; ++s) {
+            const int index = id + s * T;
+            u[s] = feat[s][col];
+            if (index > col) val2 += u[s] * u[s];
+        }
+        float vec_len_squ = block_reduce(val2, OpAdd(), scratch);
+        if (id < col) {
+            u[0] = 0.0f;
+        } else if (id == col) {
+            float vec_len = sqrtf(vec_len_squ + u[0] * u[0]);
+            u[0] -= vec_len;
+            vec_len_squ += u[0] * u[0];
+            u_length_squared = vec_len_squ;
+            feat[0][col] = vec_len;
+        } else {
+            feat[0][col] = 0.0f;
+        }
+        __syncthreads();
+        const float uls = u_length_squared;
+        for (int f = col + 1; f < kAlphaSize; ++f) {
+            float v = 0.0f;
+            for (int s = 0; s < kSub; ++s) {
+                const int index = id + s * T;
+                if (index >= col) v += feat[s][f] * u[s];
+            }
+            v = block_reduce(v, OpAdd(), scratch);
+            for (int s = 0; s < kSub; ++s) {
+                const int index = id + s * T;
+                if (index >= col) feat[s][f] -= 2.0f * u[s] * v / uls;
+            }
+        }
+    }
+    for (int i = kFeatures - 1; i >= 0; --i) {
+        if (id == i) {
+            vec3 w = make_vec3(feat[0][10], feat[0][11], feat[0][12]);
+            for (int x = i + 1; x < kFeatures; ++x) w -= ws[x] * feat[0][x];
+            w /= feat[0][i];
+            ws[i] = w;
+        }
+        __syncthreads();
+    }
+    if (id < kFeatures) {
+        vec3 w = ws[id];
+        if (u_length_squared == 0.0f) w = make_vec3(0.2f);
+        float* out = g.weights + ((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30;
+        out[id * 3 + 0] = w.x;
+        out[id * 3 + 1] = w.y;
+        out[id * 3 + 2] = w.z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused pre + fit: projected features -> fp16 hi/lo -> MFMA Gram -> FP64 solve
+// ---------------------------------------------------------------------------
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+constexpr int kGramCols = 16;  // 9 projected features + 3 projected rgb + 4 zero columns
+constexpr int kFitThreads = 256;
+
+template <int kRows>
+__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
+    constexpr int kSub = (kRows + kFitThreads - 1) / kFitThreads;
+    __shared__ float fscratch[16];
+    __shared__ double dscratch[16];
+    __shared__ _Float16 lds_hi[kGramCols][kRows];
+    __shared__ _Float16 lds_lo[kGramCols][kRows];
+    __shared__ double gram[4][kGramCols * kGramCols];
+    __shared__ double dots[kAlphaSize];
+    const int tid = threadIdx.x;
+    const int bs = g.bs;
+    const uint32_t pb = (uint32_t)(bs * bs);
+    const int basex = blockIdx.x * bs, basey = blockIdx.y * bs;
+
+    // 1. load rows; block min/max of depth
+    float a[kSub][kAlphaSize];
+    float dmin = 3.4e38f, dmax = -3.4e38f;
+    PixelIn px[kSub];
+    for (int s = 0; s < kSub; ++s) {
+        const int r = tid + s * kFitThreads;
+        if (r < kRows) {
+            px[s] = load_pixel(g, im, basex + r / bs, basey + r % bs);
+            dmin = fminf(dmin, px[s].depth);
+            dmax = fmaxf(dmax, px[s].depth);
+        }
+    }
+    const float mn = block_reduce(dmin, OpMin(), fscratch);
+    const float mx = block_reduce(dmax, OpMax(), fscratch);
+    // 2. fp16-quantised features (the reference's R16F feature buffer) + noise
+    for (int s = 0; s < kSub; ++s) {
+        const int r = tid + s * kFitThreads;
+        if (r >= kRows) continue;
+        const int lx = r / bs, ly = r % bs;
+        float z = (px[s].depth - mn) / (mx - mn + 1e-6f);
+        float fx = (float)lx / (float)(bs - 1), fy = (float)ly / (float)(bs - 1);
+        float f[kAlphaSize] = {1.0f, px[s].normal.x, px[s].normal.y, px[s].normal.z, fx, fy, z, fx * fx, fy * fy,
+                               z * z, px[s].noisy.x, px[s].noisy.y, px[s].noisy.z};
+        for (int c = 0; c < kAlphaSize; ++c) {
+            float v = h2f(f2h(f[c]));
+            if (c < kFeatures)
+                v = add_noise(v, (uint32_t)r + (uint32_t)c * pb * pb + g.frame * (uint32_t)kAlphaSize * pb * pb);
+            a[s][c] = v;
+        }
+    }
+    // 3. FP64 dot products with the constant column: a0.a0, a0.A_j, a0.y_c
+    double d_local[kAlphaSize];
+    for (int c = 0; c < kAlphaSize; ++c) d_local[c] = 0.0;
+    for (int s = 0; s < kSub; ++s) {
+        const int r = tid + s * kFitThreads;
+        if (r >= kRows) continue;
+        const double a0 = a[s][0];
+        for (int c = 0; c < kAlphaSize; ++c) d_local[c] += a0 * (double)a[s][c];
+    }
+    for (int c = 0; c < kAlphaSize; ++c) {
+        double t = block_reduce(d_local[c], OpAdd(), dscratch);
+        if (tid == 0) dots[c] = t;
+    }
+    __syncthreads();
+    // 4. project against a0 (FP64), keep fp32; block max-abs per column
+    const double inv00 = 1.0 / dots[0];
+    float b[kSub][12];
+    float amax[12];
+    for (int c = 0; c < 12; ++c) amax[c] = 0.0f;
+    for (int s = 0; s < kSub; ++s) {
+        const int r = tid + s * kFitThreads;
+        if (r >= kRows) continue;
+        const double a0 = a[s][0];
+        for (int c = 0; c < 12; ++c) {
+            double coef = dots[c + 1] * inv00;
+            b[s][c] = (float)((double)a[s][c + 1] - coef * a0);
+            amax[c] = fmaxf(amax[c], fabsf(b[s][c]));
+        }
+    }
+    __shared__ float scale[12];
+    for (int c = 0; c < 12; ++c) {
+        float m = block_reduce(amax[c], OpMax(), fscratch);
+        if (tid == 0) {
+            int e = 0;
+            frexpf(m, &e);
+            scale[c] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;  // |scaled| < 1
+        }
+    }
+    __syncthreads();
+    // 5. fp16 hi/lo split into LDS, column-major [col][row]
+    for (int s = 0; s < kSub; ++s) {
+        const int r = tid + s * kFitThreads;
+        if (r >= kRows) continue;
+        for (int c = 0; c < kGramCols; ++c) {
+            float v = c < 12 ? b[s][c] * scale[c] : 0.0f;
+            _Float16 hi = (_Float16)v;
+            _Float16 lo = (_Float16)(v - (float)hi);
+            lds_hi[c][r] = hi;
+            lds_lo[c][r] = lo;
+        }
+    }
+    __syncthreads();
+    // 6. Gram on MFMA: each wave takes 32-row chunks; per chunk hi*hi + hi*lo + lo*hi
+    {
+        const int wave = tid >> 6, lane = tid & 63;
+        const int col = lane & 15, grp = lane >> 4;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        constexpr int kChunks = kRows / 32;
+        for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
+            const int row0 = ch * 32 + grp * 8;
+            half8 hi, lo;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                hi[j] = lds_hi[col][row0 + j];
+                lo[j] = lds_lo[col][row0 + j];
+            }
+            float4v d = {0.0f, 0.0f, 0.0f, 0.0f};
+            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, d, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] += (double)d[i];
+        }
+        // D layout: lane holds rows 4*grp + i of column `col`
+#pragma unroll
+        for (int i = 0; i < 4; ++i) gram[wave][(4 * grp + i) * kGramCols + col] = acc[i];
+    }
+    __syncthreads();
+    // 7. FP64 solve on one lane
+    if (tid == 0) {
+        double G[12][12];
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) {
+                double v = gram[0][i * kGramCols + j] + gram[1][i * kGramCols + j] + gram[2][i * kGramCols + j] +
+                           gram[3][i * kGramCols + j];
+                G[i][j] = v / ((double)scale[i] * (double)scale[j]);
+            }
+        // symmetrise (the D layout orientation is irrelevant for a Gram matrix)
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < i; ++j) G[i][j] = G[j][i] = 0.5 * (G[i][j] + G[j][i]);
+        constexpr int n = kFeatures - 1;  // projected features A1..A9
+        double L[n][n];
+        bool keep[n];
+        double maxdiag = 0.0;
+        for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, G[i][i]);
+        for (int j = 0; j < n; ++j) {
+            double d = G[j][j];
+            for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+            keep[j] = d > 1e-14 * maxdiag && d > 0.0;
+            double ljj = keep[j] ? sqrt(d) : 1.0;
+            L[j][j] = ljj;
+            for (int i = j + 1; i < n; ++i) {
+                double v = G[i][j];
+                for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+                L[i][j] = keep[j] ? v / ljj : 0.0;
+            }
+        }
+        float* out = g.weights + ((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30;
+        for (int c = 0; c < 3; ++c) {
+            double z[n], w[n];
+            for (int i = 0; i < n; ++i) {
+                double v = keep[i] ? G[i][9 + c] : 0.0;
+                for (int k = 0; k < i; ++k) v -= L[i][k] * z[k];
+                z[i] = keep[i] ? v / L[i][i] : 0.0;
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double v = z[i];
+                for (int k = i + 1; k < n; ++k) v -= L[k][i] * w[k];
+                w[i] = keep[i] ? v / L[i][i] : 0.0;
+            }
+            double w0 = dots[10 + c];
+            for (int i = 0; i < n; ++i) w0 -= w[i] * dots[i + 1];
+            w0 /= dots[0];
+            out[c] = (float)w0;
+            for (int i = 0; i < n; ++i) out[(i + 1) * 3 + c] = (float)w[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bmfrPost: weighted feature sum, temporal blend, remodulation, gamma
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
+    __shared__ float scratch[16];
+    __shared__ float wts[30];
+    const int lx = threadIdx.x % g.bs, ly = threadIdx.x / g.bs;
+    const int gx = blockIdx.x * g.bs + lx, gy = blockIdx.y * g.bs + ly;
+    if (threadIdx.x < 30) {
+        float w = g.weights[((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30 + threadIdx.x];
+        wts[threadIdx.x] = (isinf(w) || isnan(w)) ? 0.0f : w;
+    }
+    PixelIn p = load_pixel(g, im, gx, gy);
+    const size_t pix = (size_t)p.img_y * g.width + p.img_x;
+    float mn = block_reduce(p.depth, OpMin(), scratch);
+    float mx = block_reduce(p.depth, OpMax(), scratch);
+    if (!p.inside) return;
+    const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
+    float mux = h2f(mv.x), muy = h2f(mv.y);
+    bool accept = mux >= 0.0f;
+    float pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
+    float z = (p.depth - mn) / (mx - mn + 1e-6f);
+    float fx = (float)lx / (float)(g.bs - 1), fy = (float)ly / (float)(g.bs - 1);
+    float f[kFeatures] = {1.0f, p.normal.x, p.normal.y, p.normal.z, fx, fy, z, fx * fx, fy * fy, z * z};
+    vec3 den = make_vec3(0.0f);
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) den += make_vec3(wts[3 * i], wts[3 * i + 1], wts[3 * i + 2]) * f[i];
+    den = vclamp(den, 0.0f, 10.0f);
+    vec3 prev = make_vec3(0.0f);
+    float alpha = 1.0f;
+    const size_t plane = (size_t)g.width * g.height;
+    if (g.frame > 0 && accept) {
+        const uint16_t* layer = im.denoised + 4 * plane * (g.frame & 1u);
+        const int w = g.width;
+        prev = xyz(sample_bilinear([&](int sx, int sy) { return load_half4(layer, (size_t)sy * w + sx); }, g.width,
+                                   g.height, mux, muy));
+        alpha = fmaxf(1.0f / pixel_spp, 0.1f);
+    }
+    den = alpha * den + (1.0f - alpha) * prev;
+    store_half4(im.denoised + 4 * plane * ((g.frame & 1u) ^ 1u), pix, make_vec4(den, 1.0f));
+    vec3 alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
+    vec3 t = alb * den;
+    t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
+    t = vclamp(vpow(t, 0.454545f), 0.0f, 1.0f);
+    store_bgra8(im.final_image, pix, make_vec4(t, 1.0f));
+}
+
+// pixelOffsets of bmfrGeneral.comp:16
+static const float kPixelOffsets[16][2] = {{.7f, .85f},  {.95f, .5f},  {.43f, .76f}, {.97f, .03f},
+                                           {.37f, .58f}, {.03f, .36f}, {.81f, .46f}, {0.f, .78f},
+                                           {.36f, -.08f}, {-.06f, 0.f}, {.95f, .1f},  {.85f, .61f},
+                                           {.06f, .1f},  {.43f, .16f}, {0.f, .5f},   {.73f, .38f}};
+
+static BmfrGeom make_geom(const pbrt_bmfr_params& p) {
+    BmfrGeom g{};
+    PBRT_REQUIRE(p.block_size == 8 || p.block_size == 16 || p.block_size == 32, "pbrt_bmfr: block_size must be 8, 16 or 32");
+    PBRT_REQUIRE(p.width >= 2 * p.block_size && p.height >= 2 * p.block_size,
+                 "pbrt_bmfr: image must be at least two blocks wide and high (mirror addressing)");
+    PBRT_REQUIRE(p.fit == PBRT_BMFR_FIT_HOUSEHOLDER || p.fit == PBRT_BMFR_FIT_GRAM_MFMA, "pbrt_bmfr: unknown fit");
+    g.width = (int)p.width;
+    g.height = (int)p.height;
+    g.bs = (int)p.block_size;
+    g.wp = (g.width / g.bs + 2) * g.bs;
+    g.hp = (g.height / g.bs + 2) * g.bs;
+    g.blocks_x = g.wp / g.bs;
+    g.blocks_y = g.hp / g.bs;
+    const float* o = kPixelOffsets[p.frame_number % 16];
+    g.offx = (int)((float)g.bs * o[0]);
+    g.offy = (int)((float)g.bs * o[1]);
+    g.frame = p.frame_number;
+    g.fit_threads = g.bs == 8 ? 64 : 256;
+    return g;
+}
+
+static size_t feature_bytes(const BmfrGeom& g) { return (size_t)kAlphaSize * g.wp * g.hp * sizeof(uint16_t); }
+static size_t weight_bytes(const BmfrGeom& g) { return (size_t)g.blocks_x * g.blocks_y * 30 * sizeof(float); }
+
+}  // namespace pbrt
+
+using namespace pbrt;
+
+extern "C" int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* p, size_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(p && out, "pbrt_bmfr_workspace_size: NULL argument");
+        BmfrGeom g = make_geom(*p);
+        size_t fb = (feature_bytes(g) + 255) & ~(size_t)255;
+        *out = fb + weight_bytes(g);
+    });
+}
+
+extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* im, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(p && im, "pbrt_bmfr: NULL argument");
+        PBRT_REQUIRE(im->depth && im->normal && im->albedo && im->motion && im->spp && im->noisy && im->denoised &&
+                         im->final_image && im->workspace,
+                     "pbrt_bmfr: missing image");
+        BmfrGeom g = make_geom(*p);
+        size_t fb = (feature_bytes(g) + 255) & ~(size_t)255;
+        PBRT_REQUIRE(im->workspace_bytes >= fb + weight_bytes(g), "pbrt_bmfr: workspace too small");
+        g.features = static_cast<uint16_t*>(im->workspace);
+        g.weights = reinterpret_cast<float*>(static_cast<char*>(im->workspace) + fb);
+        hipStream_t st = (hipStream_t)stream;
+        dim3 grid(g.blocks_x, g.blocks_y);
+        const int threads = g.bs * g.bs;
+        if (p->fit == PBRT_BMFR_FIT_HOUSEHOLDER) {
+            hipLaunchKernelGGL(k_bmfr_pre, grid, dim3(threads), 0, st, g, *im);
+            if (g.bs == 32)
+                hipLaunchKernelGGL(k_bmfr_fit_householder<4>, grid, dim3(256), 0, st, g);
+            else if (g.bs == 16)
+                hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(256), 0, st, g);
+            else
+                hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
+        } else {
+            if (g.bs == 32)
+                hipLaunchKernelGGL(k_bmfr_fit_gram<1024>, grid, dim3(kFitThreads), 0, st, g, *im);
+            else if (g.bs == 16)
+                hipLaunchKernelGGL(k_bmfr_fit_gram<256>, grid, dim3(kFitThreads), 0, st, g, *im);
+            else
+                hipLaunchKernelGGL(k_bmfr_fit_gram<64>, grid, dim3(kFitThreads), 0, st, g, *im);
+        }
+        hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, *im);
+        PBRT_HIP(hipGetLastError());
+    });
+}

@@ -1,0 +1,20 @@
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace pbrt {
+
+constexpr int kLeafBits = 4;             // leaf triangle count stored as count - 1 in 4 bits
+constexpr int kMaxLeaf = 1 << kLeafBits;
+
+struct BvhHost {
+    std::vector<float> nodes;         // 16 floats per node (see scene_types.hpp)
+    std::vector<uint32_t> tri_order;  // BVH position -> input triangle index
+    uint32_t depth = 0;               // levels of inner nodes + leaf level
+};
+
+// tri_lo / tri_hi: 3 floats per triangle (axis-aligned bounds).
+BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int max_leaf, int max_depth);
+
+}  // namespace pbrt

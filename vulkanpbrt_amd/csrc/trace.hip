@@ -1,0 +1,325 @@
+// Path tracing kernels and their C ABI entry points (pbrt_trace,
+// pbrt_intersect, pbrt_occluded, pbrt_rng_sequence).
+//
+// The megakernel follows ptRaygen.rgen:24-104 per pixel: camera ray, primary
+// hit, next-event estimation (lighting.glsl:246-261), G-buffer stores, the
+// bounce loop of indirectLighting (lighting.glsl:263-287) and the final
+// clamp / demodulation.  Project semantics where the reference is undefined:
+//   * a path ends when indirectLighting returns without tracing (rejected
+//     BRDF sample or Russian roulette) or when the traced ray misses; the
+//     reference instead continues with normalize(vec3(0)) = NaN state
+//     (ptRaygen.rgen:70), whose result depends on the driver's NaN handling.
+//   * the motion/alpha/format rules of the other passes live in their files.
+#include <hip/hip_runtime.h>
+
+#include "capi_common.hpp"
+#include "scene_internal.hpp"
+#include "traverse.hpp"
+
+namespace pbrt {
+
+struct TraceConsts {
+    float inv_view[16];
+    float inv_proj[16];
+    uint32_t width, height;
+    uint32_t frame_number, sample_number;
+    uint32_t max_depth, min_depth;
+    uint32_t light_sampling;
+    uint32_t output;
+    uint32_t write_gbuffer;
+    uint32_t pad[3];
+};
+
+constexpr int kTileX = 8;
+constexpr int kTileY = 8;
+constexpr int kTileThreads = kTileX * kTileY;
+
+__device__ __forceinline__ uint8_t unorm8(float x) {
+    x = gclamp(x, 0.0f, 1.0f);
+    return (uint8_t)rintf(x * 255.0f);
+}
+
+// createRay (camera.glsl:6-16)
+__device__ __forceinline__ void create_ray(const TraceConsts& c, uint32_t px, uint32_t py, bool anti_alias, Rng& re,
+                                           vec3& pos, vec3& dir) {
+    vec2 center = make_vec2((float)px + 0.5f, (float)py + 0.5f);
+    if (anti_alias) {
+        vec2 j = rng_vec2(re);
+        center = center + (j - make_vec2(0.5f, 0.5f));
+    }
+    vec2 ndc = make_vec2(center.x / (float)c.width, center.y / (float)c.height);
+    vec2 clip = make_vec2(ndc.x * 2.0f - 1.0f, ndc.y * 2.0f - 1.0f);
+    pos = make_vec3(c.inv_view[12], c.inv_view[13], c.inv_view[14]);
+    vec4 d = mat4_mul(c.inv_proj, make_vec4(clip.x, clip.y, 1.0f, 1.0f));
+    vec4 w = mat4_mul(c.inv_view, make_vec4(normalize(xyz(d)), 0.0f));
+    dir = xyz(w);
+}
+
+__device__ __forceinline__ LightSample sample_light(const DeviceScene& sc, uint32_t mode, vec3 pos, vec3 n, Rng& re) {
+    if (mode == PBRT_LIGHT_SAMPLE_SURFACE_STRENGTH)
+        return sample_light_surface_strength(sc.lights, sc.light_count, pos, n, re);
+    if (mode == PBRT_LIGHT_SAMPLE_LIGHT_STRENGTH)
+        return sample_light_light_strength(sc.lights, sc.light_count, sc.light_strength_sum, pos, n, re);
+    return sample_light_uniform(sc.lights, sc.light_count, pos, n, re);
+}
+
+template <bool kCount>
+__device__ __forceinline__ void trace_closest(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax,
+                                              int* stack, SurfaceInfo& s, vec3& position, uint32_t& category,
+                                              TraversalCounters& cnt) {
+    Hit h;
+    if (traverse<false, kCount>(sc, o, d, tmin, tmax, h, stack, kTileThreads, cnt))
+        shade_hit(sc, h, d, s, position, category);
+    else
+        miss_surface(s, position, category);
+}
+
+// nextEventEsitmation with the shadow ray traced inline.
+template <bool kCount>
+__device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode, vec3 pos, vec3 o,
+                                           const SurfaceInfo& s, vec3 throughput, Rng& re, int* stack,
+                                           TraversalCounters& cnt, uint32_t& shadow_rays) {
+    if (is_miss(s)) return sky_color(-o) * throughput;
+    LightSample ls = sample_light(sc, mode, pos, s.normal, re);
+    if (!ls.valid) return make_vec3(0.0f);
+    ++shadow_rays;
+    Hit h;
+    bool occluded = traverse<true, kCount>(sc, pos, ls.l, 0.001f, ls.tmax, h, stack, kTileThreads, cnt);
+    if (occluded) return make_vec3(0.0f);
+    return nee_unshadowed(ls, o, s, throughput);
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, TraceConsts c, float* depth_img,
+                                                              float* normal_img, uint8_t* material_img,
+                                                              uint8_t* albedo_img, float4* illum_img,
+                                                              unsigned long long* counters) {
+    __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
+    const uint32_t lid = threadIdx.x;
+    int* stack = stack_mem + lid;
+    const uint32_t px = blockIdx.x * kTileX + (lid % kTileX);
+    const uint32_t py = blockIdx.y * kTileY + (lid / kTileX);
+    if (px >= c.width || py >= c.height) return;
+    const size_t pix = (size_t)py * c.width + px;
+    TraversalCounters cnt{0, 0};
+    uint32_t rays = 0, shadow_rays = 0;
+
+    Rng re = rng_init(px, py, c.frame_number);
+    vec3 throughput = make_vec3(1.0f);
+    bool anti_alias = c.output == PBRT_OUTPUT_FINAL_IMAGE && c.sample_number > 0;
+    vec3 cam_pos, dir;
+    create_ray(c, px, py, anti_alias, re, cam_pos, dir);
+    SurfaceInfo s;
+    vec3 hit_pos;
+    uint32_t category;
+    ++rays;
+    trace_closest<kCount>(sc, cam_pos, dir, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+    vec3 color = make_vec3(0.0f);
+    color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -normalize(dir), s, throughput, re, stack, cnt,
+                                shadow_rays);
+    color += s.emissive_color;
+    vec3 albedo = s.diffuse_color + s.specular_color;
+    if (c.write_gbuffer) {
+        depth_img[pix] = distance(hit_pos, cam_pos);
+        normal_img[2 * pix + 0] = acosf(s.normal.z);
+        normal_img[2 * pix + 1] = atan2f(s.normal.y, s.normal.x);
+        uint8_t* mat = material_img + 4 * pix;
+        mat[0] = unorm8((float)category / 255.0f);
+        mat[1] = 0;
+        mat[2] = 0;
+        mat[3] = 0;
+        uint8_t* alb = albedo_img + 4 * pix;
+        alb[0] = unorm8(albedo.x);
+        alb[1] = unorm8(albedo.y);
+        alb[2] = unorm8(albedo.z);
+        alb[3] = 255;
+    }
+    if (!is_miss(s)) {
+        int trans_depth = 0;
+        vec3 wpos = cam_pos;
+        for (int i = 0; i < (int)c.max_depth && trans_depth < 10; ++i) {
+            if (s.illumination_type == 7) {
+                --i;
+                ++trans_depth;
+            }
+            vec3 v = normalize(wpos - hit_pos);
+            wpos = hit_pos;
+            // indirectLighting (lighting.glsl:263-287)
+            vec3 l;
+            float pdf;
+            vec3 brdf = sample_brdf(s, re, v, l, pdf);
+            if (brdf == make_vec3(0.0f) || pdf < kEpsilon) break;
+            float tc = dot(l, s.normal);
+            if (s.illumination_type == 7) tc = 1.0f;
+            vec3 path_tp = throughput * (brdf * tc) / pdf;
+            if ((uint32_t)i > c.min_depth) {
+                if (rng_float(re) < kMinTermination) break;
+                path_tp /= 1.0f - kMinTermination;
+            }
+            throughput = path_tp;
+            ++rays;
+            trace_closest<kCount>(sc, wpos, l, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+            color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -l, s, throughput, re, stack, cnt,
+                                        shadow_rays) +
+                     s.emissive_color * throughput;
+            if (is_miss(s)) break;
+        }
+    }
+    color = vclamp(color, 0.0f, kMaxRadiance);
+    if (c.output == PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT) {
+        vec3 demod = color;
+        if (!isinf(hit_pos.x)) demod = vmin(color / (albedo + make_vec3(kEpsilon)), 1e3f);
+        illum_img[pix] = make_float4(demod.x, demod.y, demod.z, 1.0f);
+    } else {
+        vec3 fc = color;
+        if (c.sample_number > 0) {
+            float4 prev = illum_img[pix];
+            float alpha = 1.0f / (float)(c.sample_number + 1);
+            vec3 pl = srgb_to_linear(make_vec3(prev.x, prev.y, prev.z));
+            fc = vmix(pl, fc, alpha);
+        }
+        fc = vpow(fc, 1.0f / 2.2f);
+        fc = vclamp(fc, 0.0f, 1.0f);
+        illum_img[pix] = make_float4(fc.x, fc.y, fc.z, 1.0f);
+    }
+    if (kCount) {
+        atomicAdd(&counters[0], (unsigned long long)rays);
+        atomicAdd(&counters[1], (unsigned long long)shadow_rays);
+        atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
+        atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+    }
+}
+
+__global__ void __launch_bounds__(kTileThreads) k_intersect(DeviceScene sc, const float* rays, uint32_t n,
+                                                            uint32_t* hits, float* bary) {
+    __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
+    uint32_t i = blockIdx.x * kTileThreads + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    TraversalCounters cnt{0, 0};
+    Hit h;
+    bool any = traverse<false, false>(sc, make_vec3(r[0], r[1], r[2]), make_vec3(r[4], r[5], r[6]), r[3], r[7], h,
+                                      stack_mem + threadIdx.x, kTileThreads, cnt);
+    uint32_t* o = hits + 4 * (size_t)i;
+    if (any) {
+        const float4 t0 = sc.tris[3 * h.tri + 0];
+        const float4 t1 = sc.tris[3 * h.tri + 1];
+        o[0] = __float_as_uint(h.t);
+        o[1] = __float_as_uint(t0.w);
+        o[2] = __float_as_uint(t1.w);
+        o[3] = h.tri;
+        bary[2 * (size_t)i] = h.u;
+        bary[2 * (size_t)i + 1] = h.v;
+    } else {
+        o[0] = __float_as_uint(r[7]);
+        o[1] = kNoHit;
+        o[2] = kNoHit;
+        o[3] = kNoHit;
+        bary[2 * (size_t)i] = 0.0f;
+        bary[2 * (size_t)i + 1] = 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kTileThreads) k_occluded(DeviceScene sc, const float* rays, uint32_t n,
+                                                           uint8_t* occ) {
+    __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
+    uint32_t i = blockIdx.x * kTileThreads + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    TraversalCounters cnt{0, 0};
+    Hit h;
+    bool any = traverse<true, false>(sc, make_vec3(r[0], r[1], r[2]), make_vec3(r[4], r[5], r[6]), r[3], r[7], h,
+                                     stack_mem + threadIdx.x, kTileThreads, cnt);
+    occ[i] = any ? 1 : 0;
+}
+
+__global__ void k_rng_sequence(uint32_t width, uint32_t height, uint32_t frame, uint32_t count, uint32_t* out) {
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t py = blockIdx.y;
+    if (px >= width || py >= height) return;
+    Rng re = rng_init(px, py, frame);
+    uint32_t* o = out + ((size_t)py * width + px) * count;
+    for (uint32_t k = 0; k < count; ++k) o[k] = float_as_uint(rng_float(re));
+}
+
+}  // namespace pbrt
+
+using namespace pbrt;
+
+extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* p, const pbrt_push_constants* pc,
+                              const pbrt_trace_images* img, unsigned long long* counters, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(scene && p && pc && img, "pbrt_trace: NULL argument");
+        PBRT_REQUIRE(p->width > 0 && p->height > 0, "pbrt_trace: zero-size image");
+        PBRT_REQUIRE(p->output == PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT || p->output == PBRT_OUTPUT_FINAL_IMAGE,
+                     "pbrt_trace: illumination buffer not supported");
+        PBRT_REQUIRE(img->illumination, "pbrt_trace: illumination image is NULL");
+        PBRT_REQUIRE(!p->write_gbuffer || (img->depth && img->normal && img->material && img->albedo),
+                     "pbrt_trace: G-buffer requested but an image is NULL");
+        PBRT_REQUIRE(p->tracer == PBRT_TRACER_MEGAKERNEL || p->tracer == PBRT_TRACER_WAVEFRONT,
+                     "pbrt_trace: unknown tracer");
+        const DeviceScene& sc = scene_device_view(scene);
+        TraceConsts c{};
+        std::memcpy(c.inv_view, pc->view_inverse, sizeof(c.inv_view));
+        std::memcpy(c.inv_proj, pc->proj_inverse, sizeof(c.inv_proj));
+        c.width = p->width;
+        c.height = p->height;
+        c.frame_number = pc->frame_number;
+        c.sample_number = pc->sample_number;
+        c.max_depth = p->max_recursion_depth;
+        c.min_depth = p->min_recursion_depth;
+        uint32_t mode = p->light_sampling;
+        if (mode == PBRT_LIGHT_SAMPLE_AUTO)
+            mode = sc.light_count > 800 ? PBRT_LIGHT_SAMPLE_UNIFORM : PBRT_LIGHT_SAMPLE_SURFACE_STRENGTH;
+        PBRT_REQUIRE(mode <= PBRT_LIGHT_SAMPLE_UNIFORM, "pbrt_trace: unknown light sampling mode");
+        c.light_sampling = mode;
+        c.output = p->output;
+        c.write_gbuffer = p->write_gbuffer;
+        dim3 grid((p->width + kTileX - 1) / kTileX, (p->height + kTileY - 1) / kTileY);
+        hipStream_t st = (hipStream_t)stream;
+        float4* illum = reinterpret_cast<float4*>(img->illumination);
+        if (counters)
+            hipLaunchKernelGGL(pt_megakernel<true>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth, img->normal,
+                               img->material, img->albedo, illum, counters);
+        else
+            hipLaunchKernelGGL(pt_megakernel<false>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
+                               img->normal, img->material, img->albedo, illum, counters);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_intersect(const pbrt_scene* scene, const float* rays, uint32_t n, uint32_t* hits,
+                                  float* barycentrics, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(scene && (n == 0 || (rays && hits && barycentrics)), "pbrt_intersect: NULL argument");
+        if (n == 0) return;
+        const DeviceScene& sc = scene_device_view(scene);
+        hipLaunchKernelGGL(k_intersect, dim3((n + kTileThreads - 1) / kTileThreads), dim3(kTileThreads), 0,
+                           (hipStream_t)stream, sc, rays, n, hits, barycentrics);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_occluded(const pbrt_scene* scene, const float* rays, uint32_t n, uint8_t* occluded,
+                                 void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(scene && (n == 0 || (rays && occluded)), "pbrt_occluded: NULL argument");
+        if (n == 0) return;
+        const DeviceScene& sc = scene_device_view(scene);
+        hipLaunchKernelGGL(k_occluded, dim3((n + kTileThreads - 1) / kTileThreads), dim3(kTileThreads), 0,
+                           (hipStream_t)stream, sc, rays, n, occluded);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_rng_sequence(uint32_t width, uint32_t height, uint32_t frame_index, uint32_t count,
+                                     uint32_t* out, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(out != nullptr, "pbrt_rng_sequence: NULL output");
+        if (width == 0 || height == 0 || count == 0) return;
+        dim3 grid((width + 255) / 256, height);
+        hipLaunchKernelGGL(k_rng_sequence, grid, dim3(256), 0, (hipStream_t)stream, width, height, frame_index, count,
+                           out);
+        PBRT_HIP(hipGetLastError());
+    });
+}

@@ -1,0 +1,321 @@
+// BVH traversal and hit shading on gfx950.
+//
+// traverse<>() is the software replacement of the driver traversal behind
+// traceRayEXT: stack-based BVH2 walk (near child first, far child pushed to a
+// per-thread LDS stack), Moller-Trumbore triangle tests, and the alpha test of
+// ptAlphaHit.rahit:14-52 applied to every candidate on alpha-tested instances
+// (the project rule for the undefined Opaque|NoOpaque ray flags,
+// ptRaygen.rgen:16).  Closest-hit mode keeps the nearest t in [tmin, tmax];
+// any-hit mode (shadow rays: TerminateOnFirstHit | SkipClosestHit) returns on
+// the first accepted candidate.
+#pragma once
+
+#include "pt_device.hpp"
+#include "scene_types.hpp"
+
+namespace pbrt {
+
+struct Hit {
+    float t;
+    uint32_t tri;  // BVH-ordered triangle index, kNoHit on miss
+    float u, v;    // hitAttributeEXT barycentrics
+};
+
+struct TraversalCounters {
+    uint32_t nodes;
+    uint32_t tris;
+};
+
+// ---------------------------------------------------------------------------
+// Texture fetch: RGBA8 UNORM, LINEAR filter, REPEAT addressing, mip 0.
+// 1x1 textures return their texel exactly (all four taps coincide).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ vec4 unpack_rgba8(uint32_t c) {
+    return make_vec4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
+                     (float)((c >> 16) & 0xFFu) / 255.0f, (float)(c >> 24) / 255.0f);
+}
+
+__device__ __forceinline__ int wrap_repeat(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ vec4 lerp4(vec4 a, vec4 b, float t) {
+    float s = 1.0f - t;
+    return make_vec4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
+}
+
+__device__ __forceinline__ vec4 texture_sample(const DeviceScene& sc, int tex, vec2 uv) {
+    const DeviceTexture t = sc.textures[tex];
+    const uint32_t* texels = sc.texels + t.offset;
+    if (t.width == 1 && t.height == 1) return unpack_rgba8(texels[0]);
+    float x = uv.x * (float)t.width - 0.5f;
+    float y = uv.y * (float)t.height - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y);
+    float fx = x - fx0, fy = y - fy0;
+    int x0 = wrap_repeat((int)fx0, (int)t.width), x1 = wrap_repeat((int)fx0 + 1, (int)t.width);
+    int y0 = wrap_repeat((int)fy0, (int)t.height), y1 = wrap_repeat((int)fy0 + 1, (int)t.height);
+    vec4 c00 = unpack_rgba8(texels[(size_t)y0 * t.width + x0]);
+    vec4 c10 = unpack_rgba8(texels[(size_t)y0 * t.width + x1]);
+    vec4 c01 = unpack_rgba8(texels[(size_t)y1 * t.width + x0]);
+    vec4 c11 = unpack_rgba8(texels[(size_t)y1 * t.width + x1]);
+    return lerp4(lerp4(c00, c10, fx), lerp4(c01, c11, fx), fy);
+}
+
+// ---------------------------------------------------------------------------
+// Index / vertex unpacking (geometry.glsl:4-40)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unpack_index(const DeviceScene& sc, const DeviceMesh& m, uint32_t prim,
+                                             uint32_t& ix, uint32_t& iy, uint32_t& iz) {
+    const uint32_t* w = sc.index_words + m.index_word_offset;
+    if (m.index_stride == 4) {
+        ix = w[3 * prim];
+        iy = w[3 * prim + 1];
+        iz = w[3 * prim + 2];
+    } else {
+        uint32_t full = 3 * prim;
+        uint32_t p = full >> 1;
+        if (full & 1u) {
+            ix = w[p] >> 16;
+            uint32_t t = w[p + 1];
+            iz = t >> 16;
+            iy = t & 0xFFFFu;
+        } else {
+            uint32_t t = w[p];
+            iy = t >> 16;
+            ix = t & 0xFFFFu;
+            iz = w[p + 1] & 0xFFFFu;
+        }
+    }
+}
+
+__device__ __forceinline__ vec3 load3(const float* p, uint32_t i) { return make_vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ vec2 load2(const float* p, uint32_t i) { return make_vec2(p[2 * i], p[2 * i + 1]); }
+
+// Alpha test of ptAlphaHit.rahit:14-52 (diffuse alpha, no sRGB conversion).
+__device__ __forceinline__ bool alpha_rejects(const DeviceScene& sc, uint32_t inst, uint32_t prim, float u, float v) {
+    const DeviceInstance& in = sc.instances[inst];
+    const DeviceMesh m = sc.meshes[in.mesh_id];
+    uint32_t ix, iy, iz;
+    unpack_index(sc, m, prim, ix, iy, iz);
+    const float* uvs = sc.uvs + 2 * (size_t)m.vertex_offset;
+    vec2 uv0 = load2(uvs, ix), uv1 = load2(uvs, iy), uv2 = load2(uvs, iz);
+    float b0 = 1.0f - u - v;
+    vec2 tc = uv0 * b0 + uv1 * u + uv2 * v;
+    return texture_sample(sc, m.diffuse_tex, tc).w < 0.01f;
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float safe_rcp(float d) {
+    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
+}
+
+template <bool kAnyHit, bool kCount>
+__device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax, Hit& hit, int* stack,
+                         int stride, TraversalCounters& cnt) {
+    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    hit.tri = kNoHit;
+    hit.t = tmax;
+    int sp = 0;
+    int node = 0;
+    const float4* __restrict__ nodes = sc.nodes;
+    const float4* __restrict__ tris = sc.tris;
+    while (true) {
+        if (node >= 0) {
+            if (kCount) cnt.nodes++;
+            const float4 n0 = nodes[4 * node + 0];
+            const float4 n1 = nodes[4 * node + 1];
+            const float4 n2 = nodes[4 * node + 2];
+            const float4 n3 = nodes[4 * node + 3];
+            float tcur = hit.t;
+            // child 0
+            float a0 = n0.x * ix - ox, a1 = n0.y * ix - ox;
+            float b0 = n0.z * iy - oy, b1 = n0.w * iy - oy;
+            float c0 = n2.x * iz - oz, c1 = n2.y * iz - oz;
+            float near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+            float far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
+            // child 1
+            float d0 = n1.x * ix - ox, d1 = n1.y * ix - ox;
+            float e0 = n1.z * iy - oy, e1 = n1.w * iy - oy;
+            float f0 = n2.z * iz - oz, f1 = n2.w * iz - oz;
+            float near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), tmin));
+            float far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
+            bool h0 = near0 <= far0;
+            bool h1 = near1 <= far1;
+            int ch0 = __float_as_int(n3.x);
+            int ch1 = __float_as_int(n3.y);
+            if (h0 && h1) {
+                bool swap = near1 < near0;
+                int first = swap ? ch1 : ch0;
+                int second = swap ? ch0 : ch1;
+                stack[sp * stride] = second;
+                ++sp;
+                node = first;
+                continue;
+            } else if (h0) {
+                node = ch0;
+                continue;
+            } else if (h1) {
+                node = ch1;
+                continue;
+            }
+        } else {
+            uint32_t code = ~(uint32_t)node;
+            uint32_t first = code >> kLeafCountBits;
+            uint32_t count = (code & (kMaxLeafSize - 1)) + 1;
+            for (uint32_t k = first; k < first + count; ++k) {
+                if (kCount) cnt.tris++;
+                const float4 t0 = tris[3 * k + 0];
+                const float4 t1 = tris[3 * k + 1];
+                const float4 t2 = tris[3 * k + 2];
+                vec3 v0 = make_vec3(t0.x, t0.y, t0.z);
+                vec3 e1 = make_vec3(t1.x, t1.y, t1.z);
+                vec3 e2 = make_vec3(t2.x, t2.y, t2.z);
+                vec3 pv = cross(d, e2);
+                float det = dot(e1, pv);
+                if (det == 0.0f) continue;
+                float inv = 1.0f / det;
+                vec3 tv = o - v0;
+                float u = dot(tv, pv) * inv;
+                if (!(u >= 0.0f && u <= 1.0f)) continue;
+                vec3 qv = cross(tv, e1);
+                float v = dot(d, qv) * inv;
+                if (!(v >= 0.0f && u + v <= 1.0f)) continue;
+                float t = dot(e2, qv) * inv;
+                if (!(t >= tmin && t < hit.t)) continue;
+                uint32_t flags = __float_as_uint(t2.w);
+                if (flags & kTriFlagAlphaTested) {
+                    if (alpha_rejects(sc, __float_as_uint(t0.w), __float_as_uint(t1.w), u, v)) continue;
+                }
+                hit.t = t;
+                hit.tri = k;
+                hit.u = u;
+                hit.v = v;
+                if (kAnyHit) return true;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stack[sp * stride];
+    }
+    return hit.tri != kNoHit;
+}
+
+// ---------------------------------------------------------------------------
+// Closest-hit shading: ptClosesthit.rchit:18-92 with geometry.glsl helpers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ vec3 get_tangent(vec3 a, vec3 b, vec3 c, vec2 auv, vec2 buv, vec2 cuv) {
+    float bv_cv = buv.y - cuv.y;
+    if (bv_cv == 0.0f) return (b - c) / (buv.x - cuv.x);
+    float quotient = (auv.y - cuv.y) / bv_cv;
+    vec3 dd = c + (b - c) * quotient;
+    vec2 duv = cuv + (buv - cuv) * quotient;
+    return (dd - a) / (duv.x - auv.x);
+}
+
+__device__ __forceinline__ vec3 project_onto(vec3 a, vec3 b) { return dot(a, b) / dot(b, b) * b; }
+
+__device__ __forceinline__ vec3 mat3_mul(const float* m, vec3 v) {
+    return make_vec3(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+                     m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+
+__device__ void shade_hit(const DeviceScene& sc, const Hit& hit, vec3 ray_dir, SurfaceInfo& s, vec3& position,
+                          uint32_t& category) {
+    const float4 t0 = sc.tris[3 * hit.tri + 0];
+    const float4 t1 = sc.tris[3 * hit.tri + 1];
+    uint32_t inst_id = __float_as_uint(t0.w);
+    uint32_t prim = __float_as_uint(t1.w);
+    const DeviceInstance& inst = sc.instances[inst_id];
+    const uint32_t obj = (uint32_t)inst.mesh_id;
+    const DeviceMesh m = sc.meshes[obj];
+    uint32_t ix, iy, iz;
+    unpack_index(sc, m, prim, ix, iy, iz);
+    const float* P = sc.positions + 3 * (size_t)m.vertex_offset;
+    const float* N = sc.normals + 3 * (size_t)m.vertex_offset;
+    const float* UV = sc.uvs + 2 * (size_t)m.vertex_offset;
+    vec3 p0 = load3(P, ix), p1 = load3(P, iy), p2 = load3(P, iz);
+    vec3 n0 = load3(N, ix), n1 = load3(N, iy), n2 = load3(N, iz);
+    vec2 uv0 = load2(UV, ix), uv1 = load2(UV, iy), uv2 = load2(UV, iz);
+    const float bx = 1.0f - hit.u - hit.v, by = hit.u, bz = hit.v;
+    vec2 tc = uv0 * bx + uv1 * by + uv2 * bz;
+    vec4 diffuse = texture_sample(sc, m.diffuse_tex, tc);
+    vec3 drgb = srgb_to_linear(make_vec3(diffuse.x, diffuse.y, diffuse.z));
+    drgb *= diffuse.w;
+    vec3 pos = p0 * bx + p1 * by + p2 * bz;
+    position = xyz(mat4_mul(inst.object_mat, make_vec4(pos, 1.0f)));
+    vec3 normal = normalize(n0 * bx + n1 * by + n2 * bz);
+    if (isinf(normal.x) || isnan(normal.x)) normal = make_vec3(0.0f, 1.0f, 0.0f);
+    normal = normalize(mat3_mul(inst.normal_mat, normal));
+    const float eps = 1e-6f;
+    if (uv0 == uv1) uv1 = uv1 + make_vec2(eps, 0.0f);
+    if (uv0 == uv2) uv2 = uv2 + make_vec2(0.0f, eps);
+    if (uv1 == uv2) uv2 = uv2 + make_vec2(eps, eps);
+    vec3 T = mat3_mul(inst.normal_mat, get_tangent(p0, p1, p2, uv0, uv1, uv2));
+    vec3 B = mat3_mul(inst.normal_mat, get_tangent(p0, p2, p1, make_vec2(uv0.y, uv0.x), make_vec2(uv2.y, uv2.x),
+                                                   make_vec2(uv1.y, uv1.x)));
+    // gramSchmidt (geometry.glsl:78-82)
+    vec3 tt = T - project_onto(T, normal);
+    vec3 bb = B - project_onto(B, tt) - project_onto(B, normal);
+    mat3 tbn;
+    tbn.c[0] = normalize(tt);
+    tbn.c[1] = normalize(bb);
+    tbn.c[2] = normal;
+    // getNormal (geometry.glsl:84-92)
+    const DeviceTexture ntex = sc.textures[m.normal_tex];
+    if (!(ntex.width == 1 && ntex.height == 1)) {
+        vec4 tn = texture_sample(sc, m.normal_tex, tc);
+        vec3 tangent_normal = make_vec3(tn.x * 2.0f - 1.0f, tn.y * 2.0f - 1.0f, tn.z * 2.0f - 1.0f);
+        normal = normalize(tbn * tangent_normal);
+    } else {
+        normal = tbn.c[2];
+    }
+    const DeviceMaterial& mat = sc.materials[obj];
+    drgb = drgb * xyz(mat.diffuse_ior);
+    vec4 specular;
+    const DeviceTexture stex = sc.textures[m.specular_tex];
+    if (stex.width == 1 && stex.height == 1) {
+        specular = make_vec4(xyz(mat.specular_dissolve), mat.ambient_roughness.w);
+    } else {
+        vec4 st = texture_sample(sc, m.specular_tex, tc);
+        vec3 sl = srgb_to_linear(make_vec3(st.x, st.y, st.z));
+        specular = make_vec4(sl, st.w);
+    }
+    float perceptual_roughness = specular.w;
+    vec3 srgb = make_vec3(specular.x, specular.y, specular.z);
+    float max_specular = max3(srgb);
+    float metallic = convert_metallic(drgb, srgb, max_specular);
+    vec3 diff_part = drgb * ((1.0f - max_specular) / (1.0f - kMinRoughness) / fmaxf(1.0f - metallic, eps));
+    vec3 spec_part = srgb - (make_vec3(kMinRoughness) * (1.0f - metallic) * (1.0f / fmaxf(metallic, eps)));
+    vec3 base = vmix(diff_part, spec_part, metallic * metallic);
+    const vec3 f0 = make_vec3(0.04f);
+    vec3 diffuse_color = base * (make_vec3(1.0f) - f0);
+    diffuse_color *= 1.0f - metallic;
+    float alpha_roughness = perceptual_roughness * perceptual_roughness;
+    vec3 specular_color = vmix(f0, base, metallic);
+    float reflectance = max3(specular_color);
+    float reflectance90 = gclamp(reflectance * 25.0f, 0.0f, 1.0f);
+    vec3 v = normalize(-ray_dir);
+    vec4 em = texture_sample(sc, m.emissive_tex, tc);
+    vec3 emissive = xyz(mat.emission_texture_id) * srgb_to_linear(make_vec3(em.x, em.y, em.z));
+    if (dot(v, normal) < 0.0f) emissive = make_vec3(0.0f);
+    s.perceptual_roughness = perceptual_roughness;
+    s.metalness = metallic;
+    s.alpha_roughness = alpha_roughness;
+    s.illumination_type = (int)mat.transmittance_illum.w;
+    s.reflectance0 = specular_color;
+    s.reflectance90 = make_vec3(reflectance90);
+    s.diffuse_color = diffuse_color;
+    s.specular_color = specular_color;
+    s.emissive_color = emissive;
+    s.transmissive_color = xyz(mat.transmittance_illum);
+    s.normal = normal;
+    s.basis = tbn;
+    s.index_of_refraction = mat.diffuse_ior.w;
+    category = mat.category_id;
+}
+
+}  // namespace pbrt
