@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Frame benchmark: 1080p trace + denoise on one MI355X.
+
+A step is one frame of the atrium workload (seed-generated ~267k triangles,
+1920x1080, 1 spp, 4 bounces, SURFACE_STRENGTH light sampling): path tracing,
+temporal accumulation, BMFR (32x32 blocks) and TAA, with the camera moving along
+the scene's 64-frame spline.  Inputs (scene, BVH, textures, images) are resident
+in HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--config 2] [--tracer megakernel|wavefront]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--tracer", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+WORKLOADS = {
+    1: dict(name="cornell box + 16 icospheres (5.1k tris), 256x256, 1 spp, 2 bounces, accumulator + BMFR + TAA",
+            width=256, height=256, depth=2, seed=1),
+    2: dict(name="atrium (~267k tris, 8 textures, 3 area quads + sun), 1920x1080, 1 spp, 4 bounces, "
+                 "accumulator + BMFR 32x32 + TAA over the 64-frame camera spline",
+            width=1920, height=1080, depth=4, seed=2),
+}
+
+
+class Frame:
+    """Device images and the per-frame launch sequence (all on torch's current stream)."""
+
+    def __init__(self, torch, capi, camera, gen, scene, w, h, depth, tracer, bmfr_fit):
+        self.torch, self.capi, self.camera = torch, capi, camera
+        self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
+        self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+        self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
+        d = "cuda"
+        t = torch
+        self.img = dict(
+            depth=t.zeros((h, w), dtype=t.float32, device=d),
+            normal=t.zeros((h, w, 2), dtype=t.float32, device=d),
+            material=t.zeros((h, w, 4), dtype=t.uint8, device=d),
+            albedo=t.zeros((h, w, 4), dtype=t.uint8, device=d),
+            illumination=t.zeros((h, w, 4), dtype=t.float32, device=d),
+        )
+        # ping-pong history: index = frame parity (replaces the copy-back image copies)
+        self.depth_hist = [t.zeros((h, w), dtype=t.float32, device=d) for _ in range(2)]
+        self.spp = [t.zeros((h, w), dtype=t.uint8, device=d) for _ in range(2)]
+        self.acc = [t.zeros((h, w, 4), dtype=t.int16, device=d) for _ in range(2)]
+        self.motion = t.zeros((h, w, 2), dtype=t.int16, device=d)
+        self.denoised = t.zeros((2, h, w, 4), dtype=t.int16, device=d)
+        self.bmfr_final = t.zeros((h, w, 4), dtype=t.uint8, device=d)
+        self.taa_hist = t.zeros((h, w, 4), dtype=t.uint8, device=d)
+        self.final = t.zeros((h, w, 4), dtype=t.uint8, device=d)
+        bp = capi.BmfrParams(w, h, 32, 0, self.fit)
+        n = C.c_size_t()
+        capi.check(capi.lib().pbrt_bmfr_workspace_size(C.byref(bp), C.byref(n)))
+        self.ws = t.zeros(n.value, dtype=t.uint8, device=d)
+        self.ws_bytes = n.value
+        tn = C.c_size_t()
+        capi.check(capi.lib().pbrt_trace_workspace_size(w, h, C.byref(tn)))
+        self.trace_ws = t.zeros(tn.value, dtype=t.uint8, device=d)
+        self.prev_view = None
+
+    def stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def push_constants(self, frame):
+        pose = self.gen.camera(frame)
+        view, proj = self.camera.pose_matrices(pose, self.w, self.h)
+        prev = self.prev_view if self.prev_view is not None else view
+        return self.camera.push_constants(view, proj, prev, frame), view, proj
+
+    def trace(self, pc, counters=None):
+        capi = self.capi
+        p = capi.TraceParams(self.w, self.h, self.depth, 0, capi.LIGHT_SAMPLE_AUTO,
+                             capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, 1, self.tracer)
+        ti = capi.TraceImages(*(capi.ptr(self.img[k]) for k in ("depth", "normal", "material", "albedo",
+                                                                 "illumination")),
+                              capi.ptr(self.trace_ws), self.trace_ws.numel())
+        cnt = capi.ptr(counters) if counters is not None else None
+        capi.check(capi.lib().pbrt_trace(self.scene.handle, C.byref(p), C.byref(pc), C.byref(ti), cnt,
+                                         self.stream()))
+
+    def accumulate(self, frame, view, proj):
+        capi, cam = self.capi, self.camera
+        cur, prv = frame & 1, (frame & 1) ^ 1
+        p = capi.AccumulateParams(self.w, self.h, frame, 0)
+        p.inv_proj[:] = cam.colmajor(np.linalg.inv(proj)).tolist()
+        p.inv_view[:] = cam.colmajor(np.linalg.inv(view)).tolist()
+        pv = self.prev_view if self.prev_view is not None else view
+        p.prev_view[:] = cam.colmajor(pv).tolist()
+        o = np.linalg.inv(pv)[:, 3].astype(np.float32)
+        o[3] = 1.0
+        p.prev_origin[:] = o.tolist()
+        # the G-buffer depth of this frame is written into depth_hist[cur] after the pass
+        im = capi.AccumulateImages(capi.ptr(self.img["illumination"]), capi.ptr(self.img["depth"]),
+                                   capi.ptr(self.depth_hist[prv]), capi.ptr(self.spp[prv]), capi.ptr(self.acc[prv]),
+                                   capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]))
+        capi.check(capi.lib().pbrt_accumulate(C.byref(p), C.byref(im), self.stream()))
+        capi.check(capi.lib().pbrt_copy_back(self.w, self.h, capi.ptr(self.img["depth"]), None, None,
+                                             capi.ptr(self.depth_hist[cur]), None, None, self.stream()))
+
+    def bmfr(self, frame):
+        capi = self.capi
+        cur = frame & 1
+        p = capi.BmfrParams(self.w, self.h, 32, frame, self.fit)
+        im = capi.BmfrImages(capi.ptr(self.img["depth"]), capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]),
+                             capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]),
+                             capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
+        capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), self.stream()))
+
+    def taa(self, frame):
+        capi = self.capi
+        im = capi.TaaImages(capi.ptr(self.motion), capi.ptr(self.bmfr_final), capi.ptr(self.taa_hist),
+                            capi.ptr(self.final))
+        capi.check(capi.lib().pbrt_taa(self.w, self.h, frame, C.byref(im), self.stream()))
+
+    def run(self, frame, events=None):
+        pc, view, proj = self.push_constants(frame)
+        if events:
+            events[0].record()
+        self.trace(pc)
+        if events:
+            events[1].record()
+        self.accumulate(frame, view, proj)
+        if events:
+            events[2].record()
+        self.bmfr(frame)
+        if events:
+            events[3].record()
+        self.taa(frame)
+        if events:
+            events[4].record()
+        self.prev_view = view
+
+
+def cpu_baseline(gen, cfg, budget_s):
+    """The oracle's FP32 restatement on the host cores, over a bounded row sample."""
+    from oracle import oracle
+    from vulkanpbrt_amd import camera
+    osc = oracle.OracleScene(gen.desc)
+    w, h = cfg["width"], cfg["height"]
+    view, proj = camera.pose_matrices(gen.camera(0), w, h)
+    pc = camera.push_constants(view, proj, view, 0)
+    iv = np.array(pc.view_inverse[:], np.float32)
+    ip = np.array(pc.proj_inverse[:], np.float32)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(threads, 16))
+    rows, y0, total_rays, total_t = 4, h // 2 - 2, 0, 0.0
+    while total_t < budget_s and y0 + rows <= h:
+        t0 = time.perf_counter()
+        r = osc.trace(w, h, iv, ip, frame=0, max_depth=cfg["depth"], rows=(y0, y0 + rows), threads=threads)
+        dt = time.perf_counter() - t0
+        total_rays += r["rays"] + r["shadow_rays"]
+        total_t += dt
+        y0 += rows
+        if dt < budget_s / 8:
+            rows *= 2
+    sampled_rows = y0 - (h // 2 - 2)
+    return dict(value=round(total_rays / total_t / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
+                sample=f"frame 0, rows {h // 2 - 2}..{y0 - 1} ({sampled_rows} of {h} rows x {w} px), "
+                       f"FP32 oracle restatement with FP64 BVH queries, {total_t:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is design-only).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from vulkanpbrt_amd import camera, capi
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    cfg = WORKLOADS[args.config]
+    w, h = cfg["width"], cfg["height"]
+    t_setup = time.perf_counter()
+    gen = capi.GeneratedScene(args.config, cfg["seed"])
+    scene = capi.Scene(gen.desc)
+    info = scene.info()
+    fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit)
+    setup_s = time.perf_counter() - t_setup
+
+    # counted rays / traversal work of one frame (counting variant, outside the timed region)
+    counters = torch.zeros(4, dtype=torch.int64, device="cuda")
+    pc0, _, _ = fr.push_constants(0)
+    fr.trace(pc0, counters)
+    torch.cuda.synchronize()
+    cnt = counters.cpu().numpy().astype(np.float64)
+    rays_per_frame = cnt[0] + cnt[1]
+
+    for f in range(args.warmup):
+        fr.run(f)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(K)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for k in range(K):
+        fr.run(args.warmup + k, ev[k])
+    end.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    stage = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)] for k in range(K)])
+    stage_ms = stage.mean(axis=0)
+    ms_per_step = total_ms / K
+    mrays = rays_per_frame * K / (total_ms / 1e3) / 1e6
+    nominal = 1000.0 / ms_per_step * (2 * cfg["depth"]) * w * h / 1e6  # Gui.hpp:41-43 formula
+
+    # roofline of the dominant kernel (the tracer): algorithmic bytes per launch =
+    # 48 B ray I/O per traced ray + 64 B per BVH node visit + 48 B per triangle test
+    # + 36 B/px G-buffer/illumination output
+    trace_bytes = 48.0 * rays_per_frame + 64.0 * cnt[2] + 48.0 * cnt[3] + 36.0 * w * h
+    trace_ms = stage_ms[0]
+    achieved = trace_bytes / (trace_ms / 1e3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="pt_megakernel"
+                    if args.tracer == "megakernel" else "wavefront trace kernels")
+
+    out = {
+        "metric": "1080p trace+denoise ms & Mrays/s; learned vs BMFR denoise ms at equal relMSE",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seed-generated scene, random-free camera spline)",
+        "config": {"workload": cfg["name"], "width": w, "height": h, "spp": 1, "max_depth": cfg["depth"],
+                   "triangles": info.triangle_count, "lights": info.light_count, "tracer": args.tracer,
+                   "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"]},
+        "stage_ms": {"trace": round(stage_ms[0], 3), "accumulate": round(stage_ms[1], 3),
+                     "bmfr": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
+        "rays_per_frame": int(rays_per_frame),
+        "nominal_mrays_reference_formula": round(nominal, 2),
+        "bvh": {"nodes": info.bvh_node_count, "depth": info.bvh_depth,
+                "node_visits_per_ray": round(cnt[2] / max(1.0, rays_per_frame), 2),
+                "tri_tests_per_ray": round(cnt[3] / max(1.0, rays_per_frame), 2)},
+        "setup_s": round(setup_s, 2),
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -204,7 +204,13 @@ typedef struct pbrt_trace_images {
     uint8_t* material;     /* RGBA8    category / 255 */
     uint8_t* albedo;       /* RGBA8    diffuse + specular colour */
     float* illumination;   /* RGBA32F  demodulated radiance, or the progressive final image */
+    void* workspace;       /* wavefront tracer: device scratch of pbrt_trace_workspace_size() bytes
+                              (path state and ray queues); unused by the megakernel */
+    size_t workspace_bytes;
 } pbrt_trace_images;
+
+/* Device scratch the wavefront tracer needs for a width x height launch. */
+int32_t pbrt_trace_workspace_size(uint32_t width, uint32_t height, size_t* out_bytes);
 
 /* Optional per-launch counters (device pointer to 4 x uint64, accumulated with
  * atomics; NULL disables counting): [0] closest-hit rays, [1] shadow rays,
@@ -298,10 +304,17 @@ enum pbrt_bmfr_fit {
     /* the reference algorithm: FP32 Householder QR of the fp16 feature block
      * (bmfrFit.comp:27-81) with wave64 reductions */
     PBRT_BMFR_FIT_HOUSEHOLDER = 0,
-    /* per-block projection of the features against the noised constant column,
-     * Gram matrix on FP16 MFMA (hi/lo split, FP32 accumulation), FP64 Cholesky
-     * solve; pre and fit fused (no feature buffer round trip) */
-    PBRT_BMFR_FIT_GRAM_MFMA = 1
+    /* pre and fit fused (no feature buffer round trip): the columns are projected
+     * against the noised constant column in FP64 and scaled by powers of two per
+     * block, their Gram matrix is built on FP16 MFMA (hi/lo split, FP32
+     * accumulation inside each 32-row chunk, FP64 across chunks) and solved by
+     * FP64 Cholesky.  Accurate on well-conditioned blocks only: on planar or
+     * sky blocks cond(A^T A) reaches 1e10 and the fit error grows to ~1e-1. */
+    PBRT_BMFR_FIT_GRAM_F16 = 1,
+    /* as PBRT_BMFR_FIT_GRAM_F16 but the Gram matrix is accumulated by FP64 MFMA
+     * (v_mfma_f64_16x16x4f64) from the fp32 projected columns; fitted values
+     * stay within ~1e-5 of the FP64 least-squares solution on every block. */
+    PBRT_BMFR_FIT_GRAM_F64 = 2
 };
 
 typedef struct pbrt_bmfr_params {

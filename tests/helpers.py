@@ -29,10 +29,24 @@ def gbuffer(torch, width, height):
     )
 
 
+_workspaces = {}
+
+
+def trace_workspace(torch, width, height):
+    key = (width, height)
+    if key not in _workspaces:
+        n = C.c_size_t()
+        capi.check(capi.lib().pbrt_trace_workspace_size(width, height, C.byref(n)))
+        _workspaces[key] = torch.zeros(n.value, dtype=torch.uint8, device="cuda")
+    return _workspaces[key]
+
+
 def trace(torch, scene, pc, img, width, height, max_depth=2, tracer=capi.TRACER_MEGAKERNEL,
           output=capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, light_sampling=capi.LIGHT_SAMPLE_AUTO, counters=None):
     p = capi.TraceParams(width, height, max_depth, 0, light_sampling, output, 1, tracer)
-    ti = capi.TraceImages(*(capi.ptr(img[k]) for k in ("depth", "normal", "material", "albedo", "illumination")))
+    ws = trace_workspace(torch, width, height) if tracer == capi.TRACER_WAVEFRONT else None
+    ti = capi.TraceImages(*(capi.ptr(img[k]) for k in ("depth", "normal", "material", "albedo", "illumination")),
+                          capi.ptr(ws), ws.numel() if ws is not None else 0)
     capi.check(capi.lib().pbrt_trace(scene.handle, C.byref(p), C.byref(pc), C.byref(ti),
                                      capi.ptr(counters) or None, stream(torch)))
 

@@ -122,6 +122,8 @@ class TraceImages(C.Structure):
         ("material", C.c_void_p),
         ("albedo", C.c_void_p),
         ("illumination", C.c_void_p),
+        ("workspace", C.c_void_p),
+        ("workspace_bytes", C.c_size_t),
     ]
 
 
@@ -136,6 +138,39 @@ class CameraPose(C.Structure):
     ]
 
 
+class AccumulateParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("frame_number", C.c_uint32), ("reserved", C.c_uint32),
+        ("inv_proj", C.c_float * 16), ("inv_view", C.c_float * 16), ("prev_view", C.c_float * 16),
+        ("prev_origin", C.c_float * 4),
+    ]
+
+
+class AccumulateImages(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "illumination", "depth", "prev_depth", "prev_spp", "prev_accumulated", "motion", "spp", "accumulated")]
+
+
+class TaaImages(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("motion", "denoised", "history", "final_image")]
+
+
+BMFR_FIT_HOUSEHOLDER = 0
+BMFR_FIT_GRAM_F16 = 1
+BMFR_FIT_GRAM_F64 = 2
+
+
+class BmfrParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("block_size", C.c_uint32),
+                ("frame_number", C.c_uint32), ("fit", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class BmfrImages(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "depth", "normal", "albedo", "motion", "spp", "noisy", "denoised", "final_image", "workspace")] + [
+        ("workspace_bytes", C.c_size_t)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "pbrt_abi_version": (C.c_int32, []),
@@ -146,9 +181,16 @@ SIGNATURES = {
     "pbrt_scene_get_lights": (C.c_int32, [C.c_void_p, C.POINTER(Light), C.c_uint32]),
     "pbrt_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
                                C.POINTER(TraceImages), C.c_void_p, C.c_void_p]),
+    "pbrt_trace_workspace_size": (C.c_int32, [C.c_uint32, C.c_uint32, C.POINTER(C.c_size_t)]),
     "pbrt_intersect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_occluded": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pbrt_rng_sequence": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pbrt_accumulate": (C.c_int32, [C.POINTER(AccumulateParams), C.POINTER(AccumulateImages), C.c_void_p]),
+    "pbrt_copy_back": (C.c_int32, [C.c_uint32, C.c_uint32] + [C.c_void_p] * 7),
+    "pbrt_taa": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(TaaImages), C.c_void_p]),
+    "pbrt_convert": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbrt_bmfr_workspace_size": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(C.c_size_t)]),
+    "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
     "pbrt_generate_scene": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pbrt_generated_scene_desc": (C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)]),
     "pbrt_generated_scene_camera": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(CameraPose)]),

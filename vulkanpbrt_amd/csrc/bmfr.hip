@@ -8,16 +8,21 @@
 //   noise     feature j < 10 gets + 2e-4 * (hash(i + j*B^2 + frame*13*B^2) - 0.5),
 //             B = b*b, integer wrap, hash = bmfrGeneral.comp random()
 //
-// Two fits:
+// Fits:
 //   k_bmfr_fit_householder   the reference's FP32 Householder QR over the fp16
 //                            feature buffer, subgroup reductions on wave64
-//   k_bmfr_fit_gram          pre + fit fused: features stay in registers; the
+//   k_bmfr_fit_gram<F64>     pre + fit fused: features stay in registers; the
 //                            columns are projected against the noised constant
-//                            column a0 in FP64, scaled by powers of two, split into
-//                            fp16 hi + lo and their Gram matrix is accumulated with
-//                            v_mfma_f32_16x16x32_f16 (FP32 inside a 32-row chunk,
-//                            FP64 across chunks); FP64 Cholesky solves for w1..w9,
-//                            w0 = a0.(y - A'w') / a0.a0.
+//                            column a0 in FP64 and scaled by powers of two; their
+//                            Gram matrix is accumulated on the matrix cores, by
+//                            v_mfma_f64_16x16x4f64 (F64) or by fp16 hi + lo splits
+//                            through v_mfma_f32_16x16x32_f16 (F16, FP32 inside a
+//                            32-row chunk, FP64 across chunks); FP64 Cholesky solves
+//                            for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+//   The BMFR features are nearly collinear on planar blocks (depth is almost a
+//   linear function of the block coordinates; only the 1e-4 noise separates
+//   them), so cond(A^T A) reaches ~1e10: an FP32-accumulated Gram loses the fit
+//   there, the FP64-accumulated one does not.
 #include <hip/hip_runtime.h>
 
 #include "capi_common.hpp"
@@ -219,23 +224,36 @@ __global__ void __launch_bounds__(256) k_bmfr_fit_householder(BmfrGeom g) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused pre + fit: projected features -> fp16 hi/lo -> MFMA Gram -> FP64 solve
+// Fused pre + fit: projected features -> MFMA Gram -> FP64 Cholesky solve
 // ---------------------------------------------------------------------------
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef double double4v __attribute__((ext_vector_type(4)));
 
 constexpr int kGramCols = 16;  // 9 projected features + 3 projected rgb + 4 zero columns
 constexpr int kFitThreads = 256;
 
-template <int kRows>
+template <int kRows, bool kF64>
+struct GramLds {
+    // F16: column-major fp16 hi / lo planes; F64: row-major fp32 rows
+    union {
+        struct {
+            _Float16 hi[kGramCols][kRows];
+            _Float16 lo[kGramCols][kRows];
+        } h;
+        float f[kRows][kGramCols];
+    } m;
+    double gram[4][kGramCols * kGramCols];
+};
+
+template <int kRows, bool kF64>
 __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
     constexpr int kSub = (kRows + kFitThreads - 1) / kFitThreads;
     __shared__ float fscratch[16];
     __shared__ double dscratch[16];
-    __shared__ _Float16 lds_hi[kGramCols][kRows];
-    __shared__ _Float16 lds_lo[kGramCols][kRows];
-    __shared__ double gram[4][kGramCols * kGramCols];
+    __shared__ GramLds<kRows, kF64> L;
     __shared__ double dots[kAlphaSize];
+    __shared__ float scale[12];
     const int tid = threadIdx.x;
     const int bs = g.bs;
     const uint32_t pb = (uint32_t)(bs * bs);
@@ -286,7 +304,8 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
     }
     __syncthreads();
     // 4. project against a0 (FP64), keep fp32; block max-abs per column
-    const double inv00 = 1.0 / dots[0];
+    double coef[12];
+    for (int c = 0; c < 12; ++c) coef[c] = dots[c + 1] / dots[0];
     float b[kSub][12];
     float amax[12];
     for (int c = 0; c < 12; ++c) amax[c] = 0.0f;
@@ -295,58 +314,71 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
         if (r >= kRows) continue;
         const double a0 = a[s][0];
         for (int c = 0; c < 12; ++c) {
-            double coef = dots[c + 1] * inv00;
-            b[s][c] = (float)((double)a[s][c + 1] - coef * a0);
+            b[s][c] = (float)((double)a[s][c + 1] - coef[c] * a0);
             amax[c] = fmaxf(amax[c], fabsf(b[s][c]));
         }
     }
-    __shared__ float scale[12];
     for (int c = 0; c < 12; ++c) {
         float m = block_reduce(amax[c], OpMax(), fscratch);
         if (tid == 0) {
             int e = 0;
             frexpf(m, &e);
-            scale[c] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;  // |scaled| < 1
+            scale[c] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;  // |scaled| < 1, exact power of two
         }
     }
     __syncthreads();
-    // 5. fp16 hi/lo split into LDS, column-major [col][row]
+    // 5. stage the scaled columns in LDS
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r >= kRows) continue;
         for (int c = 0; c < kGramCols; ++c) {
             float v = c < 12 ? b[s][c] * scale[c] : 0.0f;
-            _Float16 hi = (_Float16)v;
-            _Float16 lo = (_Float16)(v - (float)hi);
-            lds_hi[c][r] = hi;
-            lds_lo[c][r] = lo;
+            if (kF64) {
+                L.m.f[r][c] = v;
+            } else {
+                _Float16 hi = (_Float16)v;
+                L.m.h.hi[c][r] = hi;
+                L.m.h.lo[c][r] = (_Float16)(v - (float)hi);
+            }
         }
     }
     __syncthreads();
-    // 6. Gram on MFMA: each wave takes 32-row chunks; per chunk hi*hi + hi*lo + lo*hi
+    // 6. Gram on the matrix cores.  For both instructions the A and B operands of a
+    //    Gram product hold the same values (lane l: column l % 16 of a row group).
     {
         const int wave = tid >> 6, lane = tid & 63;
         const int col = lane & 15, grp = lane >> 4;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        constexpr int kChunks = kRows / 32;
-        for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
-            const int row0 = ch * 32 + grp * 8;
-            half8 hi, lo;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                hi[j] = lds_hi[col][row0 + j];
-                lo[j] = lds_lo[col][row0 + j];
+        if (kF64) {
+            constexpr int kChunks = kRows / 4;
+            double4v d = {0.0, 0.0, 0.0, 0.0};
+            for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
+                double x = (double)L.m.f[ch * 4 + grp][col];
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, d, 0, 0, 0);
             }
-            float4v d = {0.0f, 0.0f, 0.0f, 0.0f};
-            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, d, 0, 0, 0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] += (double)d[i];
+            for (int i = 0; i < 4; ++i) acc[i] = d[i];
+        } else {
+            constexpr int kChunks = kRows / 32;
+            for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
+                const int row0 = ch * 32 + grp * 8;
+                half8 hi, lo;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    hi[j] = L.m.h.hi[col][row0 + j];
+                    lo[j] = L.m.h.lo[col][row0 + j];
+                }
+                float4v d = {0.0f, 0.0f, 0.0f, 0.0f};
+                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, d, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i] += (double)d[i];
+            }
         }
-        // D layout: lane holds rows 4*grp + i of column `col`
+        // D layout: lane holds rows 4 * grp + i of column `col`
 #pragma unroll
-        for (int i = 0; i < 4; ++i) gram[wave][(4 * grp + i) * kGramCols + col] = acc[i];
+        for (int i = 0; i < 4; ++i) L.gram[wave][(4 * grp + i) * kGramCols + col] = acc[i];
     }
     __syncthreads();
     // 7. FP64 solve on one lane
@@ -354,28 +386,27 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
         double G[12][12];
         for (int i = 0; i < 12; ++i)
             for (int j = 0; j < 12; ++j) {
-                double v = gram[0][i * kGramCols + j] + gram[1][i * kGramCols + j] + gram[2][i * kGramCols + j] +
-                           gram[3][i * kGramCols + j];
+                double v = L.gram[0][i * kGramCols + j] + L.gram[1][i * kGramCols + j] + L.gram[2][i * kGramCols + j] +
+                           L.gram[3][i * kGramCols + j];
                 G[i][j] = v / ((double)scale[i] * (double)scale[j]);
             }
-        // symmetrise (the D layout orientation is irrelevant for a Gram matrix)
         for (int i = 0; i < 12; ++i)
             for (int j = 0; j < i; ++j) G[i][j] = G[j][i] = 0.5 * (G[i][j] + G[j][i]);
         constexpr int n = kFeatures - 1;  // projected features A1..A9
-        double L[n][n];
+        double Lc[n][n];
         bool keep[n];
         double maxdiag = 0.0;
         for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, G[i][i]);
         for (int j = 0; j < n; ++j) {
             double d = G[j][j];
-            for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
-            keep[j] = d > 1e-14 * maxdiag && d > 0.0;
+            for (int k = 0; k < j; ++k) d -= Lc[j][k] * Lc[j][k];
+            keep[j] = d > 1e-15 * maxdiag && d > 0.0;
             double ljj = keep[j] ? sqrt(d) : 1.0;
-            L[j][j] = ljj;
+            Lc[j][j] = ljj;
             for (int i = j + 1; i < n; ++i) {
                 double v = G[i][j];
-                for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
-                L[i][j] = keep[j] ? v / ljj : 0.0;
+                for (int k = 0; k < j; ++k) v -= Lc[i][k] * Lc[j][k];
+                Lc[i][j] = keep[j] ? v / ljj : 0.0;
             }
         }
         float* out = g.weights + ((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30;
@@ -383,13 +414,13 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
             double z[n], w[n];
             for (int i = 0; i < n; ++i) {
                 double v = keep[i] ? G[i][9 + c] : 0.0;
-                for (int k = 0; k < i; ++k) v -= L[i][k] * z[k];
-                z[i] = keep[i] ? v / L[i][i] : 0.0;
+                for (int k = 0; k < i; ++k) v -= Lc[i][k] * z[k];
+                z[i] = keep[i] ? v / Lc[i][i] : 0.0;
             }
             for (int i = n - 1; i >= 0; --i) {
                 double v = z[i];
-                for (int k = i + 1; k < n; ++k) v -= L[k][i] * w[k];
-                w[i] = keep[i] ? v / L[i][i] : 0.0;
+                for (int k = i + 1; k < n; ++k) v -= Lc[k][i] * w[k];
+                w[i] = keep[i] ? v / Lc[i][i] : 0.0;
             }
             double w0 = dots[10 + c];
             for (int i = 0; i < n; ++i) w0 -= w[i] * dots[i + 1];
@@ -458,7 +489,9 @@ static BmfrGeom make_geom(const pbrt_bmfr_params& p) {
     PBRT_REQUIRE(p.block_size == 8 || p.block_size == 16 || p.block_size == 32, "pbrt_bmfr: block_size must be 8, 16 or 32");
     PBRT_REQUIRE(p.width >= 2 * p.block_size && p.height >= 2 * p.block_size,
                  "pbrt_bmfr: image must be at least two blocks wide and high (mirror addressing)");
-    PBRT_REQUIRE(p.fit == PBRT_BMFR_FIT_HOUSEHOLDER || p.fit == PBRT_BMFR_FIT_GRAM_MFMA, "pbrt_bmfr: unknown fit");
+    PBRT_REQUIRE(p.fit == PBRT_BMFR_FIT_HOUSEHOLDER || p.fit == PBRT_BMFR_FIT_GRAM_F16 ||
+                     p.fit == PBRT_BMFR_FIT_GRAM_F64,
+                 "pbrt_bmfr: unknown fit");
     g.width = (int)p.width;
     g.height = (int)p.height;
     g.bs = (int)p.block_size;
@@ -512,13 +545,20 @@ extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* 
                 hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(256), 0, st, g);
             else
                 hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
+        } else if (p->fit == PBRT_BMFR_FIT_GRAM_F64) {
+            if (g.bs == 32)
+                hipLaunchKernelGGL((k_bmfr_fit_gram<1024, true>), grid, dim3(kFitThreads), 0, st, g, *im);
+            else if (g.bs == 16)
+                hipLaunchKernelGGL((k_bmfr_fit_gram<256, true>), grid, dim3(kFitThreads), 0, st, g, *im);
+            else
+                hipLaunchKernelGGL((k_bmfr_fit_gram<64, true>), grid, dim3(kFitThreads), 0, st, g, *im);
         } else {
             if (g.bs == 32)
-                hipLaunchKernelGGL(k_bmfr_fit_gram<1024>, grid, dim3(kFitThreads), 0, st, g, *im);
+                hipLaunchKernelGGL((k_bmfr_fit_gram<1024, false>), grid, dim3(kFitThreads), 0, st, g, *im);
             else if (g.bs == 16)
-                hipLaunchKernelGGL(k_bmfr_fit_gram<256>, grid, dim3(kFitThreads), 0, st, g, *im);
+                hipLaunchKernelGGL((k_bmfr_fit_gram<256, false>), grid, dim3(kFitThreads), 0, st, g, *im);
             else
-                hipLaunchKernelGGL(k_bmfr_fit_gram<64>, grid, dim3(kFitThreads), 0, st, g, *im);
+                hipLaunchKernelGGL((k_bmfr_fit_gram<64, false>), grid, dim3(kFitThreads), 0, st, g, *im);
         }
         hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, *im);
         PBRT_HIP(hipGetLastError());

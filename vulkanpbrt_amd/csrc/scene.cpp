@@ -352,6 +352,9 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
         dv.light_strength_sum = ps.light_strength_sum;
         dv.triangle_count = ntri;
         dv.node_count = (uint32_t)(s->bvh.nodes.size() / 16);
+        dv.has_transmissive = 0;
+        for (const DeviceMaterial& m : ps.materials)
+            if ((int)m.transmittance_illum.w == 7) dv.has_transmissive = 1;
 
         pbrt_scene_info& info = s->info;
         info.triangle_count = ntri;

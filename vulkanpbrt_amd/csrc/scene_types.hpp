@@ -86,6 +86,8 @@ struct DeviceScene {
     float light_strength_sum;
     uint32_t triangle_count;
     uint32_t node_count;
+    uint32_t has_transmissive;  // any material with illum 7 (bounces that do not count toward depth)
+    uint32_t pad[3];
 };
 
 constexpr int kLeafCountBits = 4;

@@ -1,0 +1,372 @@
+// Wavefront path tracer: the per-pixel program of ptRaygen.rgen split into
+// queue-driven launches per bounce so that traversal runs in lean kernels and
+// shading in its own:
+//
+//   k_wf_generate   camera rays (createRay) for every pixel, 8x8-tile ordered queue
+//   k_wf_extend     closest-hit traversal of the current ray queue -> hit records
+//   k_wf_shade      hit shading (ptClosesthit / ptMiss), G-buffer at the primary
+//                   vertex, next-event estimation (light sample -> shadow queue with
+//                   the unshadowed contribution), emission, BRDF sample + Russian
+//                   roulette -> compacted ray queue of the next bounce
+//   k_wf_shadow     any-hit traversal of the shadow queue; unoccluded samples add
+//                   their contribution
+//   k_wf_finalize   clamp, demodulation / progressive final image
+//
+// Queue appends are wave-aggregated (one atomic per wave).  Queue sizes stay on
+// the device; every launch is sized for the worst case and threads past the
+// live count exit, so a frame needs no host synchronisation.  The random stream
+// of each path is consumed in the same order as the per-pixel program:
+// light sample, then BRDF sample (on a copy) and the roulette draw.
+#include <hip/hip_runtime.h>
+
+#include "capi_common.hpp"
+#include "scene_internal.hpp"
+#include "trace_common.hpp"
+
+namespace pbrt {
+
+constexpr int kWfThreads = 64;
+constexpr int kMaxWfBounces = 32;
+constexpr uint32_t kDeadPath = 0xFFFFFFFFu;
+
+struct WfBuffers {
+    uint4* rng;
+    float4* tp;      // throughput
+    float4* col;     // accumulated radiance
+    float4* alb;     // primary albedo (diffuse + specular)
+    int4* state;     // x: loop index i, y: transmissive depth, z: primary hit flag
+    float4* ray_o[2];
+    float4* ray_d[2];
+    uint32_t* ray_path[2];
+    float4* hit;
+    float4* sh_o;
+    float4* sh_d;
+    float4* sh_pend;
+    uint32_t* sh_path;
+    uint32_t* counters;  // [0, kMaxWfBounces): ray queue sizes; [kMaxWfBounces, 2k): shadow queue sizes
+    uint32_t capacity;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t wf_layout(uint32_t w, uint32_t h, char* base, WfBuffers* b) {
+    const size_t tiles = (size_t)((w + kTileX - 1) / kTileX) * ((h + kTileY - 1) / kTileY);
+    const size_t n = tiles * kTileThreads;  // queue capacity: the tile-padded pixel count
+    const size_t np = (size_t)w * h;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off = align256(off + bytes);
+        return p;
+    };
+    WfBuffers t{};
+    t.rng = (uint4*)take(np * 16);
+    t.tp = (float4*)take(np * 16);
+    t.col = (float4*)take(np * 16);
+    t.alb = (float4*)take(np * 16);
+    t.state = (int4*)take(np * 16);
+    for (int q = 0; q < 2; ++q) {
+        t.ray_o[q] = (float4*)take(n * 16);
+        t.ray_d[q] = (float4*)take(n * 16);
+        t.ray_path[q] = (uint32_t*)take(n * 4);
+    }
+    t.hit = (float4*)take(n * 16);
+    t.sh_o = (float4*)take(n * 16);
+    t.sh_d = (float4*)take(n * 16);
+    t.sh_pend = (float4*)take(n * 16);
+    t.sh_path = (uint32_t*)take(n * 4);
+    t.counters = (uint32_t*)take(2 * kMaxWfBounces * 4);
+    t.capacity = (uint32_t)n;
+    if (b) *b = t;
+    return off;
+}
+
+// Append `want` from each lane to a queue with one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    return base + (uint32_t)__popcll(below);
+}
+
+__global__ void k_wf_init(WfBuffers b) {
+    const int i = threadIdx.x;
+    if (i < 2 * kMaxWfBounces) b.counters[i] = i == 0 ? b.capacity : 0u;
+}
+
+__global__ void __launch_bounds__(kWfThreads) k_wf_generate(TraceConsts c, WfBuffers b) {
+    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+    if (q >= b.capacity) return;
+    const uint32_t tiles_x = (c.width + kTileX - 1) / kTileX;
+    const uint32_t tile = q / kTileThreads, lid = q % kTileThreads;
+    const uint32_t px = (tile % tiles_x) * kTileX + lid % kTileX;
+    const uint32_t py = (tile / tiles_x) * kTileY + lid / kTileX;
+    if (px >= c.width || py >= c.height) {
+        b.ray_path[0][q] = kDeadPath;
+        return;
+    }
+    const uint32_t path = py * c.width + px;
+    Rng re = rng_init(px, py, c.frame_number);
+    bool anti_alias = c.output == PBRT_OUTPUT_FINAL_IMAGE && c.sample_number > 0;
+    vec3 pos, dir;
+    create_ray(c, px, py, anti_alias, re, pos, dir);
+    b.ray_o[0][q] = make_float4(pos.x, pos.y, pos.z, 0.001f);
+    b.ray_d[0][q] = make_float4(dir.x, dir.y, dir.z, 10000.0f);
+    b.ray_path[0][q] = path;
+    b.rng[path] = make_uint4(re.x, re.y, re.z, re.w);
+    b.tp[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.col[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.state[path] = make_int4(0, 0, 0, 0);
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kWfThreads) k_wf_extend(DeviceScene sc, WfBuffers b, int bounce,
+                                                          unsigned long long* counters) {
+    __shared__ int stack_mem[kTraversalStackSize * kWfThreads];
+    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+    const int qi = bounce & 1;
+    TraversalCounters cnt{0, 0};
+    if (q < b.counters[bounce] && b.ray_path[qi][q] != kDeadPath) {
+        const float4 o = b.ray_o[qi][q];
+        const float4 d = b.ray_d[qi][q];
+        Hit h;
+        traverse<false, kCount>(sc, make_vec3(o.x, o.y, o.z), make_vec3(d.x, d.y, d.z), o.w, d.w, h,
+                                stack_mem + threadIdx.x, kWfThreads, cnt);
+        b.hit[q] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
+    }
+    if (kCount) {
+        atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
+        atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+    }
+}
+
+__global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceConsts c, WfBuffers b, int bounce,
+                                                         float* depth_img, float* normal_img, uint8_t* material_img,
+                                                         uint8_t* albedo_img) {
+    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+    const int qi = bounce & 1, qn = qi ^ 1;
+    bool live = q < b.counters[bounce];
+    uint32_t path = live ? b.ray_path[qi][q] : kDeadPath;
+    live = live && path != kDeadPath;
+    bool want_shadow = false, want_ray = false;
+    float4 sh_o{}, sh_d{}, sh_p{}, nr_o{}, nr_d{};
+    if (live) {
+        const float4 ro = b.ray_o[qi][q];
+        const float4 rd = b.ray_d[qi][q];
+        const float4 hr = b.hit[q];
+        const vec3 origin = make_vec3(ro.x, ro.y, ro.z);
+        const vec3 dir = make_vec3(rd.x, rd.y, rd.z);
+        Hit h;
+        h.t = hr.x;
+        h.tri = __float_as_uint(hr.y);
+        h.u = hr.z;
+        h.v = hr.w;
+        SurfaceInfo s;
+        vec3 pos;
+        uint32_t category;
+        if (h.tri != kNoHit)
+            shade_hit(sc, h, dir, s, pos, category);
+        else
+            miss_surface(s, pos, category);
+        const uint4 r4 = b.rng[path];
+        Rng re{r4.x, r4.y, r4.z, r4.w};
+        float4 tp4 = b.tp[path];
+        vec3 tp = make_vec3(tp4.x, tp4.y, tp4.z);
+        float4 c4 = b.col[path];
+        vec3 col = make_vec3(c4.x, c4.y, c4.z);
+        int4 st = b.state[path];
+        const vec3 o = bounce == 0 ? -normalize(dir) : -dir;
+        if (bounce == 0) {
+            vec3 albedo = s.diffuse_color + s.specular_color;
+            b.alb[path] = make_float4(albedo.x, albedo.y, albedo.z, 0.0f);
+            if (c.write_gbuffer) {
+                depth_img[path] = distance(pos, origin);
+                normal_img[2 * (size_t)path + 0] = acosf(s.normal.z);
+                normal_img[2 * (size_t)path + 1] = atan2f(s.normal.y, s.normal.x);
+                uchar4 m;
+                m.x = unorm8((float)category / 255.0f);
+                m.y = m.z = m.w = 0;
+                reinterpret_cast<uchar4*>(material_img)[path] = m;
+                uchar4 a;
+                a.x = unorm8(albedo.x);
+                a.y = unorm8(albedo.y);
+                a.z = unorm8(albedo.z);
+                a.w = 255;
+                reinterpret_cast<uchar4*>(albedo_img)[path] = a;
+            }
+        }
+        // next-event estimation
+        if (is_miss(s)) {
+            col += sky_color(-o) * tp;
+        } else {
+            LightSample ls = sample_light(sc, c.light_sampling, pos, s.normal, re);
+            if (ls.valid) {
+                vec3 pend = nee_unshadowed(ls, o, s, tp);
+                if (pend != make_vec3(0.0f)) {
+                    want_shadow = true;
+                    sh_o = make_float4(pos.x, pos.y, pos.z, 0.001f);
+                    sh_d = make_float4(ls.l.x, ls.l.y, ls.l.z, ls.tmax);
+                    sh_p = make_float4(pend.x, pend.y, pend.z, 0.0f);
+                }
+            }
+        }
+        col += bounce == 0 ? s.emissive_color : s.emissive_color * tp;
+        // the bounce loop of ptRaygen.rgen:66-75
+        if (!is_miss(s)) {
+            int i = bounce == 0 ? 0 : st.x + 1;
+            int trans = st.y;
+            if (i < (int)c.max_depth && trans < 10) {
+                if (s.illumination_type == 7) {
+                    --i;
+                    ++trans;
+                }
+                vec3 v = normalize(origin - pos);
+                vec3 l;
+                float pdf;
+                vec3 brdf = sample_brdf(s, re, v, l, pdf);
+                if (!(brdf == make_vec3(0.0f) || pdf < kEpsilon)) {
+                    float tc = dot(l, s.normal);
+                    if (s.illumination_type == 7) tc = 1.0f;
+                    vec3 ptp = tp * (brdf * tc) / pdf;
+                    bool alive = true;
+                    if ((uint32_t)i > c.min_depth) {
+                        if (rng_float(re) < kMinTermination)
+                            alive = false;
+                        else
+                            ptp /= 1.0f - kMinTermination;
+                    }
+                    if (alive) {
+                        want_ray = true;
+                        tp = ptp;
+                        nr_o = make_float4(pos.x, pos.y, pos.z, 0.001f);
+                        nr_d = make_float4(l.x, l.y, l.z, 10000.0f);
+                        st.x = i;
+                        st.y = trans;
+                    }
+                }
+            }
+        }
+        b.rng[path] = make_uint4(re.x, re.y, re.z, re.w);
+        b.tp[path] = make_float4(tp.x, tp.y, tp.z, 0.0f);
+        b.col[path] = make_float4(col.x, col.y, col.z, 0.0f);
+        if (bounce == 0) st.z = isinf(pos.x) ? 0 : 1;
+        b.state[path] = st;
+    }
+    const uint32_t rs = wave_append(&b.counters[bounce + 1], want_ray);
+    if (want_ray) {
+        b.ray_o[qn][rs] = nr_o;
+        b.ray_d[qn][rs] = nr_d;
+        b.ray_path[qn][rs] = path;
+    }
+    const uint32_t ss = wave_append(&b.counters[kMaxWfBounces + bounce], want_shadow);
+    if (want_shadow) {
+        b.sh_o[ss] = sh_o;
+        b.sh_d[ss] = sh_d;
+        b.sh_pend[ss] = sh_p;
+        b.sh_path[ss] = path;
+    }
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kWfThreads) k_wf_shadow(DeviceScene sc, WfBuffers b, int bounce,
+                                                          unsigned long long* counters) {
+    __shared__ int stack_mem[kTraversalStackSize * kWfThreads];
+    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+    TraversalCounters cnt{0, 0};
+    if (q < b.counters[kMaxWfBounces + bounce]) {
+        const float4 o = b.sh_o[q];
+        const float4 d = b.sh_d[q];
+        Hit h;
+        bool occ = traverse<true, kCount>(sc, make_vec3(o.x, o.y, o.z), make_vec3(d.x, d.y, d.z), o.w, d.w, h,
+                                          stack_mem + threadIdx.x, kWfThreads, cnt);
+        if (!occ) {
+            const uint32_t path = b.sh_path[q];
+            const float4 p = b.sh_pend[q];
+            float4 c = b.col[path];
+            b.col[path] = make_float4(c.x + p.x, c.y + p.y, c.z + p.z, 0.0f);
+        }
+    }
+    if (kCount) {
+        atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
+        atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+    }
+}
+
+__global__ void k_wf_finalize(TraceConsts c, WfBuffers b, float4* illum_img) {
+    const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
+    if (path >= c.width * c.height) return;
+    float4 c4 = b.col[path];
+    vec3 color = vclamp(make_vec3(c4.x, c4.y, c4.z), 0.0f, kMaxRadiance);
+    if (c.output == PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT) {
+        // the miss position (1e10) is finite, so demodulation always applies (ptRaygen.rgen:85)
+        float4 a = b.alb[path];
+        vec3 demod = vmin(color / (make_vec3(a.x, a.y, a.z) + make_vec3(kEpsilon)), 1e3f);
+        illum_img[path] = make_float4(demod.x, demod.y, demod.z, 1.0f);
+    } else {
+        vec3 fc = color;
+        if (c.sample_number > 0) {
+            float4 prev = illum_img[path];
+            float alpha = 1.0f / (float)(c.sample_number + 1);
+            fc = vmix(srgb_to_linear(make_vec3(prev.x, prev.y, prev.z)), fc, alpha);
+        }
+        fc = vclamp(vpow(fc, 1.0f / 2.2f), 0.0f, 1.0f);
+        illum_img[path] = make_float4(fc.x, fc.y, fc.z, 1.0f);
+    }
+}
+
+__global__ void k_wf_count(WfBuffers b, int bounces, uint32_t pixels, unsigned long long* counters) {
+    if (threadIdx.x != 0) return;
+    unsigned long long rays = pixels, shadow = 0;
+    for (int i = 1; i < bounces; ++i) rays += b.counters[i];
+    for (int i = 0; i < bounces; ++i) shadow += b.counters[kMaxWfBounces + i];
+    counters[0] += rays;
+    counters[1] += shadow;
+}
+
+size_t wavefront_workspace_size(uint32_t w, uint32_t h) { return wf_layout(w, h, nullptr, nullptr); }
+
+void wavefront_trace(const DeviceScene& sc, const TraceConsts& c, const pbrt_trace_images& img,
+                     unsigned long long* counters, hipStream_t st) {
+    WfBuffers b;
+    size_t need = wf_layout(c.width, c.height, static_cast<char*>(img.workspace), &b);
+    PBRT_REQUIRE(img.workspace != nullptr && img.workspace_bytes >= need,
+                 "pbrt_trace: the wavefront tracer needs a workspace of pbrt_trace_workspace_size() bytes");
+    const int bounces = (int)c.max_depth + 1 + (sc.has_transmissive ? 10 : 0);
+    PBRT_REQUIRE(bounces < kMaxWfBounces, "pbrt_trace: max_recursion_depth too large for the wavefront tracer");
+    const dim3 grid((b.capacity + kWfThreads - 1) / kWfThreads);
+    float4* illum = reinterpret_cast<float4*>(img.illumination);
+    hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(2 * kMaxWfBounces), 0, st, b);
+    hipLaunchKernelGGL(k_wf_generate, grid, dim3(kWfThreads), 0, st, c, b);
+    for (int bounce = 0; bounce < bounces; ++bounce) {
+        if (counters)
+            hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+        else
+            hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+        hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfThreads), 0, st, sc, c, b, bounce, img.depth, img.normal,
+                           img.material, img.albedo);
+        if (counters)
+            hipLaunchKernelGGL(k_wf_shadow<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+        else
+            hipLaunchKernelGGL(k_wf_shadow<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+    }
+    const uint32_t np = c.width * c.height;
+    hipLaunchKernelGGL(k_wf_finalize, dim3((np + 255) / 256), dim3(256), 0, st, c, b, illum);
+    if (counters) hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, st, b, bounces, np, counters);
+}
+
+}  // namespace pbrt
+
+using namespace pbrt;
+
+extern "C" int32_t pbrt_trace_workspace_size(uint32_t width, uint32_t height, size_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_trace_workspace_size: NULL output");
+        PBRT_REQUIRE(width > 0 && height > 0, "pbrt_trace_workspace_size: zero-size image");
+        *out = wavefront_workspace_size(width, height);
+    });
+}
