@@ -36,12 +36,17 @@ COMMON = [
 ]
 
 
+MODULES = PKG / "renderModules"
+APP = PKG / "app"
+APP_BIN = PKG / "pbrt_frame"
+
+
 def _sources() -> list[Path]:
-    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")) + list(MODULES.glob("*.cpp")))
 
 
 def _compile(src: Path) -> Path:
-    obj = BUILD / (src.name + ".o")
+    obj = BUILD / (src.parent.name + "_" + src.name + ".o")
     cmd = [HIPCC, "-x", "hip", *COMMON, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
@@ -66,8 +71,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # example application of the render-module API (VulkanPBRT.cpp main loop, headless)
+    app_obj = _compile(APP / "pbrt_frame.cpp")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-o", str(APP_BIN), str(app_obj), "-L", str(PKG), "-lpbrt",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link of pbrt_frame failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources")
+        print(f"built {LIB} from {len(srcs)} sources and {APP_BIN.name}")
     return LIB
 
 

@@ -376,9 +376,14 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
                 for (int i = 0; i < 4; ++i) acc[i] += (double)d[i];
             }
         }
-        // D layout: lane holds rows 4 * grp + i of column `col`
+        // D layout (both instructions: lane holds column `col`): register i of
+        // v_mfma_f32_16x16x32_f16 is row 4 * grp + i, register i of
+        // v_mfma_f64_16x16x4f64 is row grp + 4 * i
 #pragma unroll
-        for (int i = 0; i < 4; ++i) L.gram[wave][(4 * grp + i) * kGramCols + col] = acc[i];
+        for (int i = 0; i < 4; ++i) {
+            const int row = kF64 ? grp + 4 * i : 4 * grp + i;
+            L.gram[wave][row * kGramCols + col] = acc[i];
+        }
     }
     __syncthreads();
     // 7. FP64 solve on one lane

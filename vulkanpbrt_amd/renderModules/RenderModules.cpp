@@ -1,0 +1,407 @@
+// Render modules and frame buffers on top of the C ABI (see RenderModules.hpp).
+#include "RenderModules.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <iostream>
+
+namespace vkpbrt {
+
+// ---------------------------------------------------------------------------
+// Buffers
+// ---------------------------------------------------------------------------
+GBuffer::GBuffer(uint32_t w, uint32_t h)
+    : depth(std::make_shared<DeviceImage>(w, h, Format::R32F)),
+      normal(std::make_shared<DeviceImage>(w, h, Format::RG32F)),
+      material(std::make_shared<DeviceImage>(w, h, Format::RGBA8)),
+      albedo(std::make_shared<DeviceImage>(w, h, Format::RGBA8)) {}
+
+void GBuffer::compile(Context&) {
+    depth->compile();
+    normal->compile();
+    material->compile();
+    albedo->compile();
+}
+
+IlluminationBuffer::IlluminationBuffer(IlluminationKind k, uint32_t w, uint32_t h) : kind(k) {
+    if (k == IlluminationKind::Demodulated) {
+        illumination_images.push_back(std::make_shared<DeviceImage>(w, h, Format::RGBA16F));
+        illumination_images.push_back(std::make_shared<DeviceImage>(w, h, Format::RGBA16F));
+    } else {
+        illumination_images.push_back(std::make_shared<DeviceImage>(w, h, Format::RGBA32F));
+    }
+}
+
+void IlluminationBuffer::compile(Context&) {
+    for (auto& i : illumination_images) i->compile();
+}
+
+AccumulationBuffer::AccumulationBuffer(uint32_t w, uint32_t h)
+    : spp(std::make_shared<DeviceImage>(w, h, Format::R8)),
+      prev_spp(std::make_shared<DeviceImage>(w, h, Format::R8)),
+      prev_depth(std::make_shared<DeviceImage>(w, h, Format::R32F)),
+      prev_normal(std::make_shared<DeviceImage>(w, h, Format::RG32F)),
+      motion(std::make_shared<DeviceImage>(w, h, Format::RG16F)),
+      prev_illu(std::make_shared<DeviceImage>(w, h, Format::RGBA16F)),
+      prev_illu_squared(std::make_shared<DeviceImage>(w, h, Format::RGBA16F)),
+      width(w),
+      height(h) {}
+
+void AccumulationBuffer::compile(Context&) {
+    for (auto* i : {&spp, &prev_spp, &prev_depth, &prev_normal, &motion, &prev_illu, &prev_illu_squared})
+        (*i)->compile();
+}
+
+void AccumulationBuffer::copy_to_back_images(const CommandsPtr& commands, const GBufferPtr& g_buffer,
+                                             const IlluminationBufferPtr& illumination_buffer) {
+    if (illumination_buffer->kind != IlluminationKind::Demodulated)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Error: AccumulationBuffer::copyToBackImages(...) Illumination buffer not supported.");
+    DeviceImagePtr depth = g_buffer->depth, acc = illumination_buffer->illumination_images[0];
+    commands->addChild([this, depth, acc](hipStream_t s) {
+        check(pbrt_copy_back(width, height, depth->as<float>(), spp->as<uint8_t>(), acc->as<uint16_t>(),
+                             prev_depth->as<float>(), prev_spp->as<uint8_t>(), prev_illu->as<uint16_t>(), s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// PBRTPipeline
+// ---------------------------------------------------------------------------
+PBRTPipeline::PBRTPipeline(pbrt_scene* scene, GBufferPtr g_buffer, IlluminationBufferPtr illumination_buffer,
+                           bool write_g_buffer, RayTracingRayOrigin origin)
+    : _scene(scene), _write_g_buffer(write_g_buffer), _g_buffer(g_buffer), _illumination_buffer(illumination_buffer) {
+    if (!scene) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: scene is NULL");
+    if (!illumination_buffer) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: illumination buffer is NULL");
+    if (write_g_buffer && !g_buffer) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: G-buffer requested but NULL");
+    if (origin == RayTracingRayOrigin::GBUFFER)
+        throw Exception(PBRT_ERR_UNSUPPORTED, "PBRTPipeline: G-buffer ray origin is not implemented");
+    if (illumination_buffer->kind == IlluminationKind::Demodulated)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Error: PBRTPipeline::setupRaygenShader(...) Illumination buffer not supported.");
+    _width = illumination_buffer->illumination_images[0]->width();
+    _height = illumination_buffer->illumination_images[0]->height();
+    pbrt_scene_info info{};
+    check(pbrt_scene_get_info(scene, &info));
+    if (info.light_count > 800) light_sampling_method = LightSamplingMethod::SAMPLE_UNIFORM;  // PBRTPipeline.cpp:92-96
+}
+
+void PBRTPipeline::set_tlas(pbrt_scene* as) {
+    if (as != _scene) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline::set_tlas: BVH of another scene");
+    _tlas = as;
+}
+
+void PBRTPipeline::compile(Context& context) {
+    _illumination_buffer->compile(context);
+    if (_g_buffer) _g_buffer->compile(context);
+    size_t bytes = 0;
+    check(pbrt_trace_workspace_size(_width, _height, &bytes));
+    _workspace = std::make_unique<DeviceBuffer>(bytes);
+    _workspace->compile();
+}
+
+void PBRTPipeline::add_trace_rays_to_command_graph(const CommandsPtr& command_graph,
+                                                   const PushConstantsPtr& push_constants) {
+    if (!_tlas) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: set_tlas() was not called");
+    if (!_workspace) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: compile() was not called");
+    command_graph->addChild([this, push_constants](hipStream_t s) {
+        pbrt_trace_params p{};
+        p.width = _width;
+        p.height = _height;
+        p.max_recursion_depth = max_recursion_depth;
+        p.min_recursion_depth = 0;
+        p.light_sampling = (uint32_t)light_sampling_method;
+        p.output = _illumination_buffer->kind == IlluminationKind::FinalFloat ? PBRT_OUTPUT_FINAL_IMAGE
+                                                                               : PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT;
+        p.write_gbuffer = _write_g_buffer ? 1 : 0;
+        p.tracer = tracer;
+        pbrt_push_constants pc{};
+        const RayTracingPushConstants& v = push_constants->value();
+        std::memcpy(pc.view_inverse, v.view_inverse.data(), 64);
+        std::memcpy(pc.proj_inverse, v.proj_inverse.data(), 64);
+        std::memcpy(pc.prev_view, v.prev_view.data(), 64);
+        pc.frame_number = v.frame_number;
+        pc.sample_number = v.sample_number;
+        pbrt_trace_images im{};
+        if (_g_buffer) {
+            im.depth = _g_buffer->depth->as<float>();
+            im.normal = _g_buffer->normal->as<float>();
+            im.material = _g_buffer->material->as<uint8_t>();
+            im.albedo = _g_buffer->albedo->as<uint8_t>();
+        }
+        im.illumination = _illumination_buffer->illumination_images[0]->as<float>();
+        im.workspace = _workspace->data();
+        im.workspace_bytes = _workspace->bytes();
+        check(pbrt_trace(_tlas, &p, &pc, &im, counters, s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Accumulator
+// ---------------------------------------------------------------------------
+Accumulator::Accumulator(GBufferPtr g_buffer, IlluminationBufferPtr illumination_buffer, bool separate_matrices, int,
+                         int)
+    : _width((int)g_buffer->depth->width()),
+      _height((int)g_buffer->depth->height()),
+      _g_buffer(g_buffer),
+      _original_illumination(illumination_buffer) {
+    accumulated_illumination = IlluminationBufferDemodulated(_width, _height);
+    accumulation_buffer = AccumulationBuffer::create(_width, _height);
+    if (!separate_matrices)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Accumulator: only the separate-matrices form (online tracing) is implemented");
+    if (illumination_buffer->illumination_images[0]->format() != Format::RGBA32F)
+        throw Exception(PBRT_ERR_UNSUPPORTED, "Accumulator: the source illumination must be RGBA32F");
+    _params.width = _width;
+    _params.height = _height;
+}
+
+void Accumulator::compile_images(Context& context) const {
+    accumulation_buffer->compile(context);
+    accumulated_illumination->compile(context);
+}
+
+void Accumulator::add_dispatch_to_command_graph(const CommandsPtr& command_graph) {
+    command_graph->addChild([this](hipStream_t s) {
+        pbrt_accumulate_images im{};
+        im.illumination = _original_illumination->illumination_images[0]->as<float>();
+        im.depth = _g_buffer->depth->as<float>();
+        im.prev_depth = accumulation_buffer->prev_depth->as<float>();
+        im.prev_spp = accumulation_buffer->prev_spp->as<uint8_t>();
+        im.prev_accumulated = accumulation_buffer->prev_illu->as<uint16_t>();
+        im.motion = accumulation_buffer->motion->as<uint16_t>();
+        im.spp = accumulation_buffer->spp->as<uint8_t>();
+        im.accumulated = accumulated_illumination->illumination_images[0]->as<uint16_t>();
+        check(pbrt_accumulate(&_params, &im, s));
+    });
+}
+
+// Accumulator.cpp:85-117 (separate matrices)
+void Accumulator::set_camera_matrices(int frame_index, const CameraMatrices& cur, const CameraMatrices& prev) {
+    if (!cur.proj || !cur.inv_proj)
+        throw Exception(PBRT_ERR_INVALID_ARG,
+                        "Accumulator::setDoubleMatrix(...) Accumulator was created with separateMatrices = true, but "
+                        "DubleMatrix is missing seperate matrices");
+    std::memcpy(_params.inv_proj, cur.inv_proj->data(), 64);
+    std::memcpy(_params.inv_view, cur.inv_view.data(), 64);
+    if (frame_index != 0) {
+        std::memcpy(_params.prev_view, prev.view.data(), 64);
+        std::array<double, 16> pv;
+        for (int i = 0; i < 16; ++i) pv[i] = prev.view[i];
+        std::array<double, 16> inv = inverse(pv);
+        for (int r = 0; r < 3; ++r) _params.prev_origin[r] = (float)inv[12 + r];
+        _params.prev_origin[3] = 1.0f;
+    }
+    _params.frame_number = (uint32_t)frame_index;
+}
+
+// ---------------------------------------------------------------------------
+// BMFR
+// ---------------------------------------------------------------------------
+// fitting_kernel is implied by the block size in the C ABI (64 threads for 8x8
+// blocks, 256 otherwise, as DenoiserUtils.cpp passes it).
+BMFR::BMFR(uint32_t width, uint32_t height, uint32_t work_width, uint32_t work_height, GBufferPtr g_buffer,
+           IlluminationBufferPtr illu_buffer, AccumulationBufferPtr acc_buffer, uint32_t)
+    : _width(width),
+      _height(height),
+      _work_width(work_width),
+      _g_buffer(g_buffer),
+      _illumination(illu_buffer),
+      _acc(acc_buffer) {
+    if (illu_buffer->kind != IlluminationKind::Demodulated)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Illumination Buffer type is required to be IlluminationBufferDemodulated for BMFR");
+    if (work_width != work_height || (work_width != 8 && work_width != 16 && work_width != 32))
+        throw Exception(PBRT_ERR_UNSUPPORTED, "BMFR: square 8, 16 or 32 pixel blocks");
+    _accumulated_illumination = std::make_shared<DeviceImage>(width, height, Format::RGBA16F, 2);
+    _final_illumination = std::make_shared<DeviceImage>(width, height, Format::BGRA8);
+}
+
+void BMFR::compile(Context&) {
+    _accumulated_illumination->compile();
+    _final_illumination->compile();
+    pbrt_bmfr_params p{};
+    p.width = _width;
+    p.height = _height;
+    p.block_size = _work_width;
+    p.fit = fit;
+    size_t bytes = 0;
+    check(pbrt_bmfr_workspace_size(&p, &bytes));
+    _workspace = std::make_unique<DeviceBuffer>(bytes);
+    _workspace->compile();
+}
+
+void BMFR::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
+    if (!_workspace) throw Exception(PBRT_ERR_INVALID_ARG, "BMFR: compile() was not called");
+    command_graph->addChild([this, push_constants](hipStream_t s) {
+        pbrt_bmfr_params p{};
+        p.width = _width;
+        p.height = _height;
+        p.block_size = _work_width;
+        p.frame_number = push_constants->value().frame_number;
+        p.fit = fit;
+        pbrt_bmfr_images im{};
+        im.depth = _g_buffer->depth->as<float>();
+        im.normal = _g_buffer->normal->as<float>();
+        im.albedo = _g_buffer->albedo->as<uint8_t>();
+        im.motion = _acc->motion->as<uint16_t>();
+        im.spp = _acc->spp->as<uint8_t>();
+        im.noisy = _illumination->illumination_images[0]->as<uint16_t>();
+        im.denoised = _accumulated_illumination->as<uint16_t>();
+        im.final_image = _final_illumination->as<uint8_t>();
+        im.workspace = _workspace->data();
+        im.workspace_bytes = _workspace->bytes();
+        check(pbrt_bmfr(&p, &im, s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Taa
+// ---------------------------------------------------------------------------
+Taa::Taa(uint32_t width, uint32_t height, uint32_t, uint32_t, GBufferPtr, AccumulationBufferPtr acc_buffer,
+         DeviceImagePtr denoised)
+    : _width(width), _height(height), _acc(acc_buffer), _denoised(denoised) {
+    _accumulation_image = std::make_shared<DeviceImage>(width, height, Format::RGBA8);
+    _final_image = std::make_shared<DeviceImage>(width, height, Format::BGRA8);
+}
+
+void Taa::compile(Context&) {
+    _final_image->compile();
+    _accumulation_image->compile();
+}
+
+void Taa::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
+    command_graph->addChild([this, push_constants](hipStream_t s) {
+        pbrt_taa_images im{};
+        im.motion = _acc->motion->as<uint16_t>();
+        im.denoised = _denoised->as<uint8_t>();
+        im.history = _accumulation_image->as<uint8_t>();
+        im.final_image = _final_image->as<uint8_t>();
+        check(pbrt_taa(_width, _height, push_constants->value().frame_number, &im, s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// FormatConverter
+// ---------------------------------------------------------------------------
+FormatConverter::FormatConverter(DeviceImagePtr src_image, Format dst_format, int, int) : _src(src_image) {
+    if (src_image->format() != Format::RGBA32F || dst_format != Format::BGRA8)
+        throw Exception(PBRT_ERR_UNSUPPORTED, "FormatConverter: RGBA32F -> B8G8R8A8 only");
+    final_image = std::make_shared<DeviceImage>(src_image->width(), src_image->height(), Format::BGRA8);
+}
+
+void FormatConverter::compile_images(Context&) const { final_image->compile(); }
+
+void FormatConverter::add_dispatch_to_command_graph(const CommandsPtr& command_graph) {
+    command_graph->addChild([this](hipStream_t s) {
+        check(pbrt_convert(_src->width(), _src->height(), _src->as<float>(), final_image->as<uint8_t>(), s));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// DenoiserUtils.cpp:8-130
+// ---------------------------------------------------------------------------
+void add_denoiser_to_commands(DenoisingType denoising_type, DenoisingBlockSize denoising_size,
+                              const CommandsPtr& commands, Context& compile, int width, int height,
+                              const PushConstantsPtr& compute_constants, const GBufferPtr& g_buffer,
+                              const IlluminationBufferPtr& illumination_buffer,
+                              const AccumulationBufferPtr& accumulation_buffer, DeviceImagePtr& final_descriptor_image) {
+    switch (denoising_type) {
+        case DenoisingType::NONE:
+            break;
+        case DenoisingType::BMFR: {
+            uint32_t b = 0, fk = 256;
+            switch (denoising_size) {
+                case DenoisingBlockSize::X8: b = 8; fk = 64; break;
+                case DenoisingBlockSize::X16: b = 16; break;
+                case DenoisingBlockSize::X32: b = 32; break;
+                default:
+                    throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: BMFR block size not supported");
+            }
+            auto bmfr = BMFR::create(width, height, b, b, g_buffer, illumination_buffer, accumulation_buffer, fk);
+            bmfr->compile(compile);
+            bmfr->update_image_layouts(compile);
+            bmfr->add_dispatch_to_command_graph(commands, compute_constants);
+            final_descriptor_image = bmfr->get_final_descriptor_image();
+            commands->keep(bmfr);
+            break;
+        }
+        case DenoisingType::BFR:
+            throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: BFR is not available in this build");
+        case DenoisingType::SVG:
+            std::cout << "Not yet implemented" << std::endl;
+            break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Camera helpers (vsg transform.h:148-183), column-major
+// ---------------------------------------------------------------------------
+std::array<double, 16> perspective(double fovy, double aspect, double n, double f) {
+    double ff = 1.0 / std::tan(fovy * 0.5);
+    double r = 1.0 / (n - f);
+    return {ff / aspect, 0, 0, 0, 0, -ff, 0, 0, 0, 0, f * r, -1, 0, 0, f * n * r, 0};
+}
+
+static void norm3(double* v) {
+    double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= l;
+    v[1] /= l;
+    v[2] /= l;
+}
+
+std::array<double, 16> look_at(const double* eye, const double* center, const double* up) {
+    double fwd[3] = {center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]};
+    norm3(fwd);
+    double upn[3] = {up[0], up[1], up[2]};
+    norm3(upn);
+    double side[3] = {fwd[1] * upn[2] - fwd[2] * upn[1], fwd[2] * upn[0] - fwd[0] * upn[2],
+                      fwd[0] * upn[1] - fwd[1] * upn[0]};
+    norm3(side);
+    double u[3] = {side[1] * fwd[2] - side[2] * fwd[1], side[2] * fwd[0] - side[0] * fwd[2],
+                   side[0] * fwd[1] - side[1] * fwd[0]};
+    norm3(u);
+    std::array<double, 16> m{};
+    // rows: side, u, -forward; then * translate(-eye)
+    for (int c = 0; c < 3; ++c) {
+        m[4 * c + 0] = side[c];
+        m[4 * c + 1] = u[c];
+        m[4 * c + 2] = -fwd[c];
+    }
+    m[15] = 1.0;
+    for (int r = 0; r < 3; ++r) m[12 + r] = -(m[r] * eye[0] + m[4 + r] * eye[1] + m[8 + r] * eye[2]);
+    return m;
+}
+
+std::array<double, 16> inverse(const std::array<double, 16>& m) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            a[r][c] = m[4 * c + r];
+            a[r][4 + c] = r == c ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[p][k]);
+        double piv = a[c][c];
+        for (int k = 0; k < 8; ++k) a[c][k] /= piv;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                double f = a[r][c];
+                for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+            }
+    }
+    std::array<double, 16> out;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[4 * c + r] = a[r][4 + c];
+    return out;
+}
+
+std::array<float, 16> to_float(const std::array<double, 16>& m) {
+    std::array<float, 16> f;
+    for (int i = 0; i < 16; ++i) f[i] = (float)m[i];
+    return f;
+}
+
+}  // namespace vkpbrt
