@@ -107,8 +107,12 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
         if (ext[2] > ext[axis]) axis = 2;
         uint32_t mid = t.begin + count / 2;
         bool make_leaf = false;
-        if (ext[axis] <= 0.0f || t.depth >= max_depth - 1) {
-            // degenerate centroid spread or depth budget reached: median split by index
+        // Levels that median splits still need to bring this range down to leaves;
+        // once SAH splits could overrun the depth budget, split at the median.
+        int need = 0;
+        for (uint32_t c = count; c > (uint32_t)max_leaf; c = (c + 1) / 2) ++need;
+        if (ext[axis] <= 0.0f || t.depth + need >= max_depth) {
+            // degenerate centroid spread or depth budget reached: median split
             if (count <= (uint32_t)max_leaf) make_leaf = true;
             if (!make_leaf && ext[axis] > 0.0f) {
                 std::nth_element(order + t.begin, order + mid, order + t.end, [&](uint32_t a, uint32_t b) {

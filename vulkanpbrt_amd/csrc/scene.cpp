@@ -307,8 +307,10 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
             }
         if (desc->bvh_builder == PBRT_BVH_LBVH_GPU)
             fail(PBRT_ERR_UNSUPPORTED, "pbrt_scene_create: the GPU LBVH builder is not available in this build");
-        s->bvh = build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize - 2);
-        PBRT_REQUIRE(s->bvh.depth < (uint32_t)kTraversalStackSize - 1,
+        // A traversal holds at most depth - 1 deferred children, so a tree of depth
+        // <= kTraversalStackSize fits the per-thread stack.
+        s->bvh = build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
+        PBRT_REQUIRE(s->bvh.depth <= (uint32_t)kTraversalStackSize,
                      "pbrt_scene_create: BVH deeper than the traversal stack");
 
         // BVH-ordered triangle records

@@ -8,9 +8,11 @@
 
 namespace pbrt {
 
-// Per-thread traversal stack depth (LDS); the BVH builder keeps the tree
-// shallower than this.
-constexpr int kTraversalStackSize = 64;
+// Per-thread traversal stack entries (LDS, 4 B each).  The BVH builder bounds
+// the tree depth by this value (median splits once SAH splits could overrun
+// it), and a traversal holds at most depth - 1 deferred children.  32 entries
+// keep the stack at 8 KB per 64-lane wave.
+constexpr int kTraversalStackSize = 32;
 
 struct PackedScene {
     std::vector<DeviceTexture> textures;
