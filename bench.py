@@ -33,11 +33,25 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
-    ap.add_argument("--tracer", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (the reference kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(torch, mib):
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n, "copies": 3}))
 
 
 WORKLOADS = {
@@ -197,6 +211,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(torch, args.copy_calibration)
+        return
     cfg = WORKLOADS[args.config]
     w, h = cfg["width"], cfg["height"]
     t_setup = time.perf_counter()
@@ -206,14 +223,6 @@ def main():
     fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit)
     setup_s = time.perf_counter() - t_setup
 
-    # counted rays / traversal work of one frame (counting variant, outside the timed region)
-    counters = torch.zeros(4, dtype=torch.int64, device="cuda")
-    pc0, _, _ = fr.push_constants(0)
-    fr.trace(pc0, counters)
-    torch.cuda.synchronize()
-    cnt = counters.cpu().numpy().astype(np.float64)
-    rays_per_frame = cnt[0] + cnt[1]
-
     for f in range(args.warmup):
         fr.run(f)
     torch.cuda.synchronize()
@@ -221,28 +230,58 @@ def main():
     K = args.steps
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(K)]
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    lib = capi.lib()
     torch.cuda.synchronize()
+    # per-stage tracer time: HIP events recorded by the library on the launch stream
+    capi.check(lib.pbrt_trace_timing_begin(scene.handle))
     start.record()
     for k in range(K):
         fr.run(args.warmup + k, ev[k])
     end.record()
     torch.cuda.synchronize()
+    timing = capi.TraceTiming()
+    capi.check(lib.pbrt_trace_timing_end(scene.handle, C.byref(timing)))
     total_ms = start.elapsed_time(end)
     stage = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)] for k in range(K)])
     stage_ms = stage.mean(axis=0)
     ms_per_step = total_ms / K
-    mrays = rays_per_frame * K / (total_ms / 1e3) / 1e6
+
+    # rays and traversal work of exactly the timed frames: the counting variant of the
+    # tracer replays each timed frame after the timed region (same camera and frame
+    # index, so the same path decisions)
+    counters = torch.zeros(6, dtype=torch.int64, device="cuda")
+    for k in range(K):
+        pc, _, _ = fr.push_constants(args.warmup + k)
+        fr.trace(pc, counters)
+    torch.cuda.synchronize()
+    cnt = counters.cpu().numpy().astype(np.float64)
+    rays = cnt[0] + cnt[1]
+    rays_per_frame = rays / K
+    mrays = rays / (total_ms / 1e3) / 1e6
     nominal = 1000.0 / ms_per_step * (2 * cfg["depth"]) * w * h / 1e6  # Gui.hpp:41-43 formula
 
-    # roofline of the dominant kernel (the tracer): algorithmic bytes per launch =
-    # 48 B ray I/O per traced ray + 64 B per BVH node visit + 48 B per triangle test
-    # + 36 B/px G-buffer/illumination output
-    trace_bytes = 48.0 * rays_per_frame + 64.0 * cnt[2] + 48.0 * cnt[3] + 36.0 * w * h
-    trace_ms = stage_ms[0]
-    achieved = trace_bytes / (trace_ms / 1e3) / 1e9
+    # roofline of the dominant kernel.  Algorithmic bytes: 48 B ray I/O per traced
+    # ray + 64 B per BVH node visit + 48 B per triangle record tested (+ 36 B/px of
+    # G-buffer and illumination output for the megakernel, which also shades).
+    ext_bytes = 48.0 * cnt[0] + 64.0 * cnt[2] + 48.0 * cnt[3]
+    sh_bytes = 48.0 * cnt[1] + 64.0 * cnt[4] + 48.0 * cnt[5]
+    kernels = {}
+    if args.tracer == "wavefront":
+        for name, b, ms, n in (("k_wf_extend", ext_bytes, timing.extend_ms, timing.extend_launches),
+                               ("k_wf_shadow", sh_bytes, timing.shadow_ms, timing.shadow_launches)):
+            kernels[name] = dict(ms_per_frame=round(ms / K, 3), launches=int(n),
+                                 avg_launch_us=round(ms / max(1, n) * 1e3, 1),
+                                 gbs=round(b / (ms / 1e3) / 1e9, 1))
+        kernels["k_wf_shade"] = dict(ms_per_frame=round(timing.shade_ms / K, 3), launches=int(timing.shade_launches))
+        kernels["other"] = dict(ms_per_frame=round(timing.other_ms / K, 3))
+        dom, dom_bytes, dom_ms = "k_wf_extend", ext_bytes, timing.extend_ms
+    else:
+        dom, dom_bytes, dom_ms = "pt_megakernel", ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
+        kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="pt_megakernel"
-                    if args.tracer == "megakernel" else "wavefront trace kernels")
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
+                    bytes_per_frame=round(dom_bytes / K), kernel_ms_per_frame=round(dom_ms / K, 3))
 
     out = {
         "metric": "1080p trace+denoise ms & Mrays/s; learned vs BMFR denoise ms at equal relMSE",
@@ -263,10 +302,13 @@ def main():
         "stage_ms": {"trace": round(stage_ms[0], 3), "accumulate": round(stage_ms[1], 3),
                      "bmfr": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
         "rays_per_frame": int(rays_per_frame),
+        "trace_kernels": kernels,
         "nominal_mrays_reference_formula": round(nominal, 2),
         "bvh": {"nodes": info.bvh_node_count, "depth": info.bvh_depth,
-                "node_visits_per_ray": round(cnt[2] / max(1.0, rays_per_frame), 2),
-                "tri_tests_per_ray": round(cnt[3] / max(1.0, rays_per_frame), 2)},
+                "node_visits_per_closest_ray": round(cnt[2] / max(1.0, cnt[0]), 2),
+                "tri_tests_per_closest_ray": round(cnt[3] / max(1.0, cnt[0]), 2),
+                "node_visits_per_shadow_ray": round(cnt[4] / max(1.0, cnt[1]), 2),
+                "tri_tests_per_shadow_ray": round(cnt[5] / max(1.0, cnt[1]), 2)},
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
     }

@@ -212,12 +212,36 @@ typedef struct pbrt_trace_images {
 /* Device scratch the wavefront tracer needs for a width x height launch. */
 int32_t pbrt_trace_workspace_size(uint32_t width, uint32_t height, size_t* out_bytes);
 
-/* Optional per-launch counters (device pointer to 4 x uint64, accumulated with
+/* Optional per-launch counters (device pointer to 6 x uint64, accumulated with
  * atomics; NULL disables counting): [0] closest-hit rays, [1] shadow rays,
- * [2] BVH nodes visited, [3] triangles tested. */
+ * [2] BVH nodes visited and [3] triangles tested by closest-hit rays,
+ * [4] BVH nodes visited and [5] triangles tested by shadow rays.  The counts
+ * behind the Mrays/s figure of the reference's GUI stat line (Gui.hpp:39-44). */
 int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* params,
                    const pbrt_push_constants* push_constants, const pbrt_trace_images* images,
                    unsigned long long* counters, void* stream);
+
+/* Per-stage GPU time of pbrt_trace calls (no counterpart in the reference, whose
+ * GUI shows only the frame time).  Between _begin and _end every pbrt_trace on
+ * this scene records HIP events on its stream at each stage boundary: wavefront
+ * extend / shade / shadow launches, the megakernel launch, and the rest
+ * (generate, finalize, bookkeeping).  _end waits for the last recorded event and
+ * returns the summed event times.  Not for use while the stream is captured. */
+typedef struct pbrt_trace_timing {
+    double extend_ms;     /* closest-hit traversal launches (wavefront) */
+    double shade_ms;      /* shading launches (wavefront) */
+    double shadow_ms;     /* any-hit shadow launches (wavefront) */
+    double megakernel_ms; /* megakernel launches */
+    double other_ms;      /* queue set-up, camera rays, finalize */
+    uint64_t extend_launches;
+    uint64_t shade_launches;
+    uint64_t shadow_launches;
+    uint64_t megakernel_launches;
+    uint64_t frames;      /* pbrt_trace calls recorded */
+} pbrt_trace_timing;
+
+int32_t pbrt_trace_timing_begin(const pbrt_scene* scene);
+int32_t pbrt_trace_timing_end(const pbrt_scene* scene, pbrt_trace_timing* out);
 
 /* Closest-hit query over the scene BVH (the traversal traceRayEXT performs).
  * rays: n x 8 floats (origin xyz, tmin, direction xyz, tmax).

@@ -42,6 +42,7 @@ def test_struct_sizes():
     assert C.sizeof(capi.PushConstants) == 200
     assert C.sizeof(capi.Light) == 128
     assert C.sizeof(capi.Material) == 96
+    assert C.sizeof(capi.TraceTiming) == 5 * 8 + 5 * 8
 
 
 def test_null_arguments_are_rejected_without_gpu(pbrt_lib):
@@ -49,6 +50,8 @@ def test_null_arguments_are_rejected_without_gpu(pbrt_lib):
     assert pbrt_lib.pbrt_scene_create(None, None) == -1
     assert b"NULL" in pbrt_lib.pbrt_last_error()
     assert pbrt_lib.pbrt_trace(None, None, None, None, None, None) == -1
+    assert pbrt_lib.pbrt_trace_timing_begin(None) == -1
+    assert pbrt_lib.pbrt_trace_timing_end(None, None) == -1
 
 
 def test_generated_scene_config1_counts(pbrt_lib):

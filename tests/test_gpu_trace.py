@@ -168,7 +168,7 @@ def test_wavefront_counts_rays(cuda, cfg1):
     counts = {}
     for kind in (capi.TRACER_MEGAKERNEL, capi.TRACER_WAVEFRONT):
         img = helpers.gbuffer(cuda, W, H)
-        cnt = cuda.zeros(4, dtype=cuda.int64, device="cuda")
+        cnt = cuda.zeros(6, dtype=cuda.int64, device="cuda")
         helpers.trace(cuda, scene, pc, img, W, H, tracer=kind, counters=cnt)
         cuda.cuda.synchronize()
         counts[kind] = cnt.cpu().numpy()
@@ -181,3 +181,23 @@ def test_wavefront_counts_rays(cuda, cfg1):
     assert abs(int(wf[0]) - ref["rays"]) <= 0.01 * ref["rays"]
     # the wavefront skips shadow rays whose contribution is exactly zero
     assert wf[1] <= mk[1] and abs(int(mk[1]) - ref["shadow_rays"]) <= 0.01 * ref["shadow_rays"]
+
+
+def test_trace_timing_records_every_stage(cuda, cfg1):
+    """pbrt_trace_timing_begin/_end: one segment per wavefront stage launch, positive times."""
+    gen, scene, _ = cfg1
+    pc, _, _ = helpers.frame_push_constants(gen, W, H, 1)
+    lib = capi.lib()
+    capi.check(lib.pbrt_trace_timing_begin(scene.handle))
+    for kind in (capi.TRACER_WAVEFRONT, capi.TRACER_MEGAKERNEL):
+        img = helpers.gbuffer(cuda, W, H)
+        helpers.trace(cuda, scene, pc, img, W, H, tracer=kind)
+    t = capi.TraceTiming()
+    capi.check(lib.pbrt_trace_timing_end(scene.handle, C.byref(t)))
+    bounces = 2 + 1  # max depth 2 + the primary bounce; config 1 has no transmissive material
+    assert t.frames == 2
+    assert t.extend_launches == t.shade_launches == t.shadow_launches == bounces
+    assert t.megakernel_launches == 1
+    assert t.extend_ms > 0 and t.shade_ms > 0 and t.shadow_ms > 0 and t.megakernel_ms > 0
+    # timing is off again: ending twice is an error
+    assert lib.pbrt_trace_timing_end(scene.handle, C.byref(t)) != 0

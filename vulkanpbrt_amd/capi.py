@@ -127,6 +127,12 @@ class TraceImages(C.Structure):
     ]
 
 
+class TraceTiming(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("extend_ms", "shade_ms", "shadow_ms", "megakernel_ms", "other_ms")] + [
+        (n, C.c_uint64) for n in ("extend_launches", "shade_launches", "shadow_launches", "megakernel_launches",
+                                  "frames")]
+
+
 class CameraPose(C.Structure):
     _fields_ = [
         ("eye", C.c_float * 3),
@@ -182,6 +188,8 @@ SIGNATURES = {
     "pbrt_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
                                C.POINTER(TraceImages), C.c_void_p, C.c_void_p]),
     "pbrt_trace_workspace_size": (C.c_int32, [C.c_uint32, C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pbrt_trace_timing_begin": (C.c_int32, [C.c_void_p]),
+    "pbrt_trace_timing_end": (C.c_int32, [C.c_void_p, C.POINTER(TraceTiming)]),
     "pbrt_intersect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_occluded": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pbrt_rng_sequence": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),

@@ -273,11 +273,87 @@ struct pbrt_scene {
         lights;
     DeviceScene dev{};
     pbrt_scene_info info{};
+    mutable TraceTiming timing;
 };
 
 namespace pbrt {
 const DeviceScene& scene_device_view(const pbrt_scene* s) { return s->dev; }
+
+TraceTiming* scene_timing(const pbrt_scene* s) { return s->timing.active() ? &s->timing : nullptr; }
+
+TraceTiming::~TraceTiming() {
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+}
+
+void TraceTiming::start() {
+    used_ = 0;
+    frames_ = 0;
+    segments_.clear();
+    active_ = true;
+}
+
+uint32_t TraceTiming::record(hipStream_t st) {
+    if (used_ == events_.size()) {
+        hipEvent_t e;
+        PBRT_HIP(hipEventCreate(&e));
+        events_.push_back(e);
+    }
+    PBRT_HIP(hipEventRecord(events_[used_], st));
+    return used_++;
+}
+
+void TraceTiming::frame_begin(hipStream_t st) {
+    last_ = record(st);
+    ++frames_;
+}
+
+void TraceTiming::mark(int kind, hipStream_t st) {
+    const uint32_t e = record(st);
+    segments_.push_back({kind, last_, e});
+    last_ = e;
+}
+
+void TraceTiming::finish(pbrt_trace_timing* out) {
+    active_ = false;
+    double ms[kTimeKinds] = {0, 0, 0, 0, 0};
+    uint64_t n[kTimeKinds] = {0, 0, 0, 0, 0};
+    if (used_ > 0) PBRT_HIP(hipEventSynchronize(events_[used_ - 1]));
+    for (const Segment& g : segments_) {
+        float t = 0.0f;
+        PBRT_HIP(hipEventElapsedTime(&t, events_[g.a], events_[g.b]));
+        ms[g.kind] += t;
+        n[g.kind] += 1;
+    }
+    *out = pbrt_trace_timing{};
+    out->extend_ms = ms[kTimeExtend];
+    out->shade_ms = ms[kTimeShade];
+    out->shadow_ms = ms[kTimeShadow];
+    out->megakernel_ms = ms[kTimeMegakernel];
+    out->other_ms = ms[kTimeOther];
+    out->extend_launches = n[kTimeExtend];
+    out->shade_launches = n[kTimeShade];
+    out->shadow_launches = n[kTimeShadow];
+    out->megakernel_launches = n[kTimeMegakernel];
+    out->frames = frames_;
+    segments_.clear();
+    used_ = 0;
+}
 }  // namespace pbrt
+
+extern "C" int32_t pbrt_trace_timing_begin(const pbrt_scene* scene) {
+    return guard([&] {
+        PBRT_REQUIRE(scene, "pbrt_trace_timing_begin: NULL scene");
+        scene->timing.start();
+    });
+}
+
+extern "C" int32_t pbrt_trace_timing_end(const pbrt_scene* scene, pbrt_trace_timing* out) {
+    return guard([&] {
+        PBRT_REQUIRE(scene && out, "pbrt_trace_timing_end: NULL argument");
+        PBRT_REQUIRE(scene->timing.active(), "pbrt_trace_timing_end: timing was not started");
+        scene->timing.finish(out);
+    });
+}
 
 extern "C" int32_t pbrt_abi_version(void) { return PBRT_ABI_VERSION; }
 

@@ -1,6 +1,9 @@
 // Host-side packed scene shared by the scene builder and the kernels' launchers.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
 #include <vector>
 
 #include "../../include/pbrt_c.h"
@@ -31,5 +34,35 @@ struct PackedScene {
 
 void pack_scene(const pbrt_scene_desc& d, PackedScene& ps);
 const DeviceScene& scene_device_view(const pbrt_scene* s);
+
+// Opt-in per-stage timing of pbrt_trace (pbrt_trace_timing_begin/_end): HIP
+// events recorded on the launch stream at every stage boundary; each pair of
+// consecutive events of a frame is one segment charged to a stage.
+enum TimingKind { kTimeExtend = 0, kTimeShade, kTimeShadow, kTimeMegakernel, kTimeOther, kTimeKinds };
+
+class TraceTiming {
+public:
+    ~TraceTiming();
+    void start();                          // clears the log and activates recording
+    bool active() const { return active_; }
+    void frame_begin(hipStream_t st);      // first event of a pbrt_trace call
+    void mark(int kind, hipStream_t st);   // closes the segment since the previous event
+    void finish(pbrt_trace_timing* out);   // waits for the last event, sums, deactivates
+
+private:
+    uint32_t record(hipStream_t st);
+    struct Segment {
+        int kind;
+        uint32_t a, b;
+    };
+    bool active_ = false;
+    std::vector<hipEvent_t> events_;
+    uint32_t used_ = 0, last_ = 0;
+    uint64_t frames_ = 0;
+    std::vector<Segment> segments_;
+};
+
+// The scene's timing log when recording is active, else nullptr.
+TraceTiming* scene_timing(const pbrt_scene* s);
 
 }  // namespace pbrt

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
     const uint32_t py = blockIdx.y * kTileY + (lid / kTileX);
     if (px >= c.width || py >= c.height) return;
     const size_t pix = (size_t)py * c.width + px;
-    TraversalCounters cnt{0, 0};
+    TraversalCounters cnt{0, 0}, cnt_sh{0, 0};
     uint32_t rays = 0, shadow_rays = 0;
 
     Rng re = rng_init(px, py, c.frame_number);
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
     ++rays;
     trace_closest<kCount>(sc, cam_pos, dir, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
     vec3 color = make_vec3(0.0f);
-    color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -normalize(dir), s, throughput, re, stack, cnt,
+    color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -normalize(dir), s, throughput, re, stack, cnt_sh,
                                 shadow_rays);
     color += s.emissive_color;
     vec3 albedo = s.diffuse_color + s.specular_color;
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
             throughput = path_tp;
             ++rays;
             trace_closest<kCount>(sc, wpos, l, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
-            color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -l, s, throughput, re, stack, cnt,
+            color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -l, s, throughput, re, stack, cnt_sh,
                                         shadow_rays) +
                      s.emissive_color * throughput;
             if (is_miss(s)) break;
@@ -142,6 +142,8 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
         atomicAdd(&counters[1], (unsigned long long)shadow_rays);
         atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
         atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+        atomicAdd(&counters[4], (unsigned long long)cnt_sh.nodes);
+        atomicAdd(&counters[5], (unsigned long long)cnt_sh.tris);
     }
 }
 
@@ -233,14 +235,19 @@ extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* 
         dim3 grid((p->width + kTileX - 1) / kTileX, (p->height + kTileY - 1) / kTileY);
         hipStream_t st = (hipStream_t)stream;
         float4* illum = reinterpret_cast<float4*>(img->illumination);
+        TraceTiming* tm = scene_timing(scene);
         if (p->tracer == PBRT_TRACER_WAVEFRONT) {
-            wavefront_trace(sc, c, *img, counters, st);
-        } else if (counters)
-            hipLaunchKernelGGL(pt_megakernel<true>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth, img->normal,
-                               img->material, img->albedo, illum, counters);
-        else
-            hipLaunchKernelGGL(pt_megakernel<false>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
-                               img->normal, img->material, img->albedo, illum, counters);
+            wavefront_trace(sc, c, *img, counters, tm, st);
+        } else {
+            if (tm) tm->frame_begin(st);
+            if (counters)
+                hipLaunchKernelGGL(pt_megakernel<true>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
+                                   img->normal, img->material, img->albedo, illum, counters);
+            else
+                hipLaunchKernelGGL(pt_megakernel<false>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
+                                   img->normal, img->material, img->albedo, illum, counters);
+            if (tm) tm->mark(kTimeMegakernel, st);
+        }
         PBRT_HIP(hipGetLastError());
     });
 }

@@ -292,8 +292,8 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shadow(DeviceScene sc, WfBuff
         }
     }
     if (kCount) {
-        atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
-        atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+        atomicAdd(&counters[4], (unsigned long long)cnt.nodes);
+        atomicAdd(&counters[5], (unsigned long long)cnt.tris);
     }
 }
 
@@ -331,7 +331,7 @@ __global__ void k_wf_count(WfBuffers b, int bounces, uint32_t pixels, unsigned l
 size_t wavefront_workspace_size(uint32_t w, uint32_t h) { return wf_layout(w, h, nullptr, nullptr); }
 
 void wavefront_trace(const DeviceScene& sc, const TraceConsts& c, const pbrt_trace_images& img,
-                     unsigned long long* counters, hipStream_t st) {
+                     unsigned long long* counters, TraceTiming* tm, hipStream_t st) {
     WfBuffers b;
     size_t need = wf_layout(c.width, c.height, static_cast<char*>(img.workspace), &b);
     PBRT_REQUIRE(img.workspace != nullptr && img.workspace_bytes >= need,
@@ -340,23 +340,29 @@ void wavefront_trace(const DeviceScene& sc, const TraceConsts& c, const pbrt_tra
     PBRT_REQUIRE(bounces < kMaxWfBounces, "pbrt_trace: max_recursion_depth too large for the wavefront tracer");
     const dim3 grid((b.capacity + kWfThreads - 1) / kWfThreads);
     float4* illum = reinterpret_cast<float4*>(img.illumination);
+    if (tm) tm->frame_begin(st);
     hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(2 * kMaxWfBounces), 0, st, b);
     hipLaunchKernelGGL(k_wf_generate, grid, dim3(kWfThreads), 0, st, c, b);
+    if (tm) tm->mark(kTimeOther, st);
     for (int bounce = 0; bounce < bounces; ++bounce) {
         if (counters)
             hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
         else
             hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+        if (tm) tm->mark(kTimeExtend, st);
         hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfThreads), 0, st, sc, c, b, bounce, img.depth, img.normal,
                            img.material, img.albedo);
+        if (tm) tm->mark(kTimeShade, st);
         if (counters)
             hipLaunchKernelGGL(k_wf_shadow<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
         else
             hipLaunchKernelGGL(k_wf_shadow<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
+        if (tm) tm->mark(kTimeShadow, st);
     }
     const uint32_t np = c.width * c.height;
     hipLaunchKernelGGL(k_wf_finalize, dim3((np + 255) / 256), dim3(256), 0, st, c, b, illum);
     if (counters) hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, st, b, bounces, np, counters);
+    if (tm) tm->mark(kTimeOther, st);
 }
 
 }  // namespace pbrt
