@@ -43,6 +43,18 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summaries (lines `traffic_bytes_per_launch: <kernel> <bytes>` under profiles/)."""
+    import re
+    for path in sorted((ROOT / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            m = re.match(r"traffic_bytes_per_launch:\s+(\S+)\s+(\d+)", line)
+            if m and m.group(1) == kernel:
+                return int(m.group(2)), path.name
+    return None, None
+
+
 def copy_calibration(torch, mib):
     n = mib << 20
     src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
@@ -172,32 +184,31 @@ class Frame:
         self.prev_view = view
 
 
-def cpu_baseline(gen, cfg, budget_s):
-    """The oracle's FP32 restatement on the host cores, over a bounded row sample."""
+def cpu_baseline(gen, cfg, budget_s, frames):
+    """The oracle's FP32 restatement on the host cores: whole frames of the timed camera
+    positions, one after another, until the time budget is spent (at least one frame)."""
     from oracle import oracle
     from vulkanpbrt_amd import camera
     osc = oracle.OracleScene(gen.desc)
     w, h = cfg["width"], cfg["height"]
-    view, proj = camera.pose_matrices(gen.camera(0), w, h)
-    pc = camera.push_constants(view, proj, view, 0)
-    iv = np.array(pc.view_inverse[:], np.float32)
-    ip = np.array(pc.proj_inverse[:], np.float32)
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 16))
-    rows, y0, total_rays, total_t = 4, h // 2 - 2, 0, 0.0
-    while total_t < budget_s and y0 + rows <= h:
+    total_rays, total_t, done = 0, 0.0, []
+    for f in frames:
+        view, proj = camera.pose_matrices(gen.camera(f), w, h)
+        pc = camera.push_constants(view, proj, view, f)
         t0 = time.perf_counter()
-        r = osc.trace(w, h, iv, ip, frame=0, max_depth=cfg["depth"], rows=(y0, y0 + rows), threads=threads)
-        dt = time.perf_counter() - t0
+        r = osc.trace(w, h, np.array(pc.view_inverse[:], np.float32), np.array(pc.proj_inverse[:], np.float32),
+                      frame=f, max_depth=cfg["depth"], threads=threads)
+        total_t += time.perf_counter() - t0
         total_rays += r["rays"] + r["shadow_rays"]
-        total_t += dt
-        y0 += rows
-        if dt < budget_s / 8:
-            rows *= 2
-    sampled_rows = y0 - (h // 2 - 2)
+        done.append(f)
+        if total_t >= budget_s:
+            break
     return dict(value=round(total_rays / total_t / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-                sample=f"frame 0, rows {h // 2 - 2}..{y0 - 1} ({sampled_rows} of {h} rows x {w} px), "
-                       f"FP32 oracle restatement with FP64 BVH queries, {total_t:.1f} s")
+                sample=f"{len(done)} whole {w}x{h} frames (camera frames {done[0]}..{done[-1]} of the timed run), "
+                       f"FP32 oracle restatement with FP64 BVH queries, {total_rays / 1e6:.1f} M rays in "
+                       f"{total_t:.1f} s")
 
 
 def main():
@@ -279,9 +290,13 @@ def main():
         dom, dom_bytes, dom_ms = "pt_megakernel", ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
         kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(dom)
+    n_launch = timing.extend_launches if args.tracer == "wavefront" else timing.megakernel_launches
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
-                    bytes_per_frame=round(dom_bytes / K), kernel_ms_per_frame=round(dom_ms / K, 3))
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
+                    algorithmic_bytes_per_launch=round(dom_bytes / max(1, n_launch)),
+                    avg_launch_us=round(dom_ms / max(1, n_launch) * 1e3, 1),
+                    traffic_source=f"profiles/{traffic_src}" if traffic_src else None)
 
     out = {
         "metric": "1080p trace+denoise ms & Mrays/s; learned vs BMFR denoise ms at equal relMSE",
@@ -313,7 +328,7 @@ def main():
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds, range(args.warmup, args.warmup + K))
     print(json.dumps(out))
 
 
