@@ -369,6 +369,52 @@ typedef struct pbrt_bmfr_images {
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* BFR (blockwise feature regression by 40 Adam steps): replaces bfr.comp      */
+/* (shaders/bfr.comp:202-308, parallel_reduction_alpha :100-142) dispatched by */
+/* BFR::add_dispatch_to_command_graph (source/renderModules/denoisers/        */
+/* BFR.cpp:130-140) over (W/b + 2) x (H/b + 2) blocks of b x b pixels.         */
+/* ------------------------------------------------------------------------- */
+
+typedef struct pbrt_bfr_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t block_size;   /* 8, 16 or 32 (DenoiserUtils.cpp:25-70) */
+    uint32_t frame_number; /* push constant frameNumber: pixel offset and ping-pong layer */
+} pbrt_bfr_params;
+
+typedef struct pbrt_bfr_images {
+    const float* depth;         /* R32F G-buffer depth */
+    const float* normal;        /* RG32F spherical normal */
+    const uint8_t* albedo;      /* RGBA8 G-buffer albedo */
+    const uint16_t* motion;     /* RG16F accumulator motion (x < 0: not reprojected) */
+    const uint8_t* spp;         /* R8 accumulator sample count / 256 */
+    const uint16_t* noisy;      /* RGBA16F accumulated demodulated illumination */
+    uint16_t* denoised;         /* RGBA16F x 2 layers: reads layer frame&1, writes (frame&1)^1 */
+    uint8_t* final_image;       /* B8G8R8A8 remodulated, gamma 1/2.2 */
+    float* weights;             /* optional (NULL: not written): fitted alpha per block,
+                                   [blocks_y][blocks_x][7 features][rgb] */
+} pbrt_bfr_images;
+
+int32_t pbrt_bfr(const pbrt_bfr_params* params, const pbrt_bfr_images* images, void* stream);
+
+/* BFR blender (shaders/bfrBlender.comp:22-69, BFRBlender.cpp:84-90): per-pixel
+ * standard deviation of the mean illumination over a (2r+1)^2 window mixed with
+ * the pixel's own mean and mean-square, then a three-way blend of the 32x32 (low
+ * deviation), 16x16 and 8x8 (high deviation) denoiser outputs.  Window taps
+ * outside the image read as zero (robust image access). */
+typedef struct pbrt_bfr_blend_images {
+    const uint16_t* average;         /* RGBA16F mean illumination */
+    const uint16_t* average_squared; /* RGBA16F mean squared illumination */
+    const uint8_t* denoised8;        /* B8G8R8A8 output of the 8x8 denoiser */
+    const uint8_t* denoised16;       /* B8G8R8A8 output of the 16x16 denoiser */
+    const uint8_t* denoised32;       /* B8G8R8A8 output of the 32x32 denoiser */
+    uint8_t* final_image;            /* B8G8R8A8 */
+} pbrt_bfr_blend_images;
+
+int32_t pbrt_bfr_blend(uint32_t width, uint32_t height, uint32_t filter_radius,
+                       const pbrt_bfr_blend_images* images, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

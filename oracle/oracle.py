@@ -53,6 +53,8 @@ def lib():
         L.orc_bmfr_fit.argtypes = [C.c_uint32] * 4 + [C.c_int32, vp, vp]
         L.orc_bmfr_lsq64.argtypes = [C.c_uint32] * 4 + [vp, vp]
         L.orc_bmfr_block_matrix.argtypes = [C.c_uint32] * 4 + [vp, C.c_uint32, C.c_uint32, vp]
+        L.orc_bfr.argtypes = [C.c_uint32] * 4 + [C.c_int32] + [vp] * 9
+        L.orc_bfr_blend.argtypes = [C.c_uint32] * 3 + [vp] * 6
         L.orc_bmfr_post.argtypes = [C.c_uint32] * 4 + [vp] * 9
         _lib = L
     return _lib
@@ -220,3 +222,24 @@ def bmfr_post(width, height, bs, frame, depth, normal, albedo, motion, spp, nois
             _c(spp, np.uint8), _c(noisy, np.uint16), _c(weights, np.float32)]
     lib().orc_bmfr_post(width, height, bs, frame, *[_p(a) for a in args], _p(den), _p(final))
     return den, final
+
+
+def bfr(width, height, bs, frame, depth, normal, albedo, motion, spp, noisy, denoised, iterations=40):
+    """bfr.comp: returns (denoised (2, H, W, 4) RGBA16F copy, final BGRA8, alpha [by][bx][21])."""
+    den = _c(denoised, np.uint16).copy()
+    final = np.zeros((height, width, 4), np.uint8)
+    nbx, nby = width // bs + 2, height // bs + 2
+    weights = np.zeros((nby, nbx, 21), np.float32)
+    args = [_c(depth, np.float32), _c(normal, np.float32), _c(albedo, np.uint8), _c(motion, np.uint16),
+            _c(spp, np.uint8), _c(noisy, np.uint16)]
+    lib().orc_bfr(width, height, bs, frame, iterations, *[_p(a) for a in args], _p(den), _p(final), _p(weights))
+    return den, final, weights
+
+
+def bfr_blend(width, height, radius, average, average_sq, den8, den16, den32):
+    """bfrBlender.comp: BGRA8 blend of the 8/16/32 outputs."""
+    final = np.zeros((height, width, 4), np.uint8)
+    args = [_c(average, np.uint16), _c(average_sq, np.uint16), _c(den8, np.uint8), _c(den16, np.uint8),
+            _c(den32, np.uint8)]
+    lib().orc_bfr_blend(width, height, radius, *[_p(a) for a in args], _p(final))
+    return final

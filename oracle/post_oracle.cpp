@@ -542,4 +542,187 @@ void orc_bmfr_post(uint32_t w, uint32_t h, uint32_t bs, uint32_t frame, const fl
         }
 }
 
+
+// bfr.comp:202-308 with parallel_reduction_alpha (bfr.comp:100-142): per block, 7
+// features, `iterations` Adam steps (40 in the reference) on the block-summed
+// gradient (sums in double, in pixel order), reconstruction, temporal blend,
+// remodulation.  The stop test reduces |gradient| over all 7 alpha components
+// (only a NaN gradient stops early).  weights (optional): [blocks_y][blocks_x][21].
+void orc_bfr(uint32_t w, uint32_t h, uint32_t bs, uint32_t frame, int iterations, const float* depth,
+             const float* normal, const uint8_t* albedo, const uint16_t* motion, const uint8_t* spp,
+             const uint16_t* noisy, uint16_t* denoised, uint8_t* final_image, float* weights) {
+    static const int kOff[16][2] = {{-7, -11}, {-14, -8}, {-5, -12}, {-15, -1}, {-5, -9},  {-1, -4},
+                                    {-14, -7}, {0, -13},  {-5, -1},  {-1, 0},   {-15, -2}, {-14, -10},
+                                    {-1, -1},  {-6, -3},  {0, -8},   {-10, -4}};
+    const float ALPHA = .863f, K = .116f, BETA1 = .3f, BETA2 = .7314f, EPS = 1e-8f;
+    const float ALPHA_L = 1.1f, BETA1_L = .45f, BETA2_L = .75f;
+    const int W = (int)w, H = (int)h, b = (int)bs, n = b * b;
+    const int nbx = W / b + 2, nby = H / b + 2;
+    const int offx = kOff[frame % 16][0], offy = kOff[frame % 16][1];
+    const size_t plane = (size_t)W * H;
+    struct Px {
+        int ix, iy;
+        bool inside;
+        float f[7];
+        V3f c;
+        float spp;
+        bool l1;
+    };
+    std::vector<Px> px(n);
+    for (int by = 0; by < nby; ++by)
+        for (int bx = 0; bx < nbx; ++bx) {
+            float mn = 3.4e38f, mx = -3.4e38f;
+            int l1 = 0;
+            for (int r = 0; r < n; ++r) {
+                Px& p = px[r];
+                const int lx = r / b, ly = r % b;
+                const int ax = bx * b + lx + offx, ay = by * b + ly + offy;
+                p.ix = mirror(ax, W);
+                p.iy = mirror(ay, H);
+                p.inside = ax == p.ix && ay == p.iy;
+                const size_t pix = (size_t)p.iy * W + p.ix;
+                p.c = V3f(half_to_float(noisy[4 * pix]), half_to_float(noisy[4 * pix + 1]),
+                          half_to_float(noisy[4 * pix + 2]));
+                const float th = normal[2 * pix], ph = normal[2 * pix + 1];
+                p.f[0] = 1.0f;
+                p.f[1] = (float)lx / (float)(b - 1) - 0.5f;
+                p.f[2] = (float)ly / (float)(b - 1) - 0.5f;
+                p.f[3] = depth[pix];
+                p.f[4] = std::cos(ph) * std::sin(th);
+                p.f[5] = std::sin(ph) * std::sin(th);
+                p.f[6] = std::cos(th);
+                p.spp = ((float)spp[pix] / 255.0f) * 256.0f;
+                p.l1 = p.spp >= 10.0f;
+                l1 += p.l1 ? 1 : 0;
+                mn = std::fmin(mn, p.f[3]);
+                mx = std::fmax(mx, p.f[3]);
+            }
+            for (int r = 0; r < n; ++r) {
+                float z = px[r].f[3] - mn;
+                z /= mx - mn + EPS;
+                px[r].f[3] = z * 2.0f - 1.0f;
+            }
+            const float l1r = (float)l1 * 1.0f / (float)n;
+            const float am = l1r * ALPHA_L + (1.0f - l1r) * ALPHA;
+            const float b1 = l1r * BETA1_L + (1.0f - l1r) * BETA1;
+            const float b2 = l1r * BETA2_L + (1.0f - l1r) * BETA2;
+            float alpha[21] = {0}, m[21] = {0}, v[21] = {0};
+            for (int it = 0; it < iterations; ++it) {
+                double g[21] = {0};
+                for (int r = 0; r < n; ++r) {
+                    const Px& p = px[r];
+                    float pr[3] = {0.0f, 0.0f, 0.0f};
+                    for (int j = 0; j < 7; ++j)
+                        for (int c = 0; c < 3; ++c) pr[c] += p.f[j] * alpha[3 * j + c];
+                    float d[3] = {p.c.x - pr[0], p.c.y - pr[1], p.c.z - pr[2]};
+                    if (p.l1)
+                        for (int c = 0; c < 3; ++c) d[c] = d[c] > 0.0f ? 1.0f : (d[c] < 0.0f ? -1.0f : 0.0f);
+                    for (int j = 0; j < 7; ++j)
+                        for (int c = 0; c < 3; ++c) g[3 * j + c] += (double)(p.f[j] * d[c]);
+                }
+                const float t = (float)(it + 1);
+                const float step =
+                    am * std::exp(-K * t) * std::sqrt(1.0f - std::pow(BETA2, t)) / (1.0f - std::pow(BETA1, t));
+                float rest = 0.0f;
+                for (int k = 0; k < 21; ++k) {
+                    const float dl = (float)g[k];
+                    m[k] = b1 * m[k] + (1.0f - BETA1) * dl;
+                    v[k] = b2 * v[k] + (1.0f - BETA2) * (dl * dl);
+                    alpha[k] += step * (m[k] / (std::sqrt(v[k]) + EPS));
+                    rest += std::fabs(dl);
+                }
+                if (!(rest >= 0.0f)) break;
+            }
+            if (weights) std::memcpy(weights + ((size_t)by * nbx + bx) * 21, alpha, sizeof(alpha));
+            for (int r = 0; r < n; ++r) {
+                const Px& p = px[r];
+                if (!p.inside) continue;
+                const size_t pix = (size_t)p.iy * W + p.ix;
+                V3f den(0.0f);
+                for (int j = 0; j < 7; ++j) den = den + V3f(alpha[3 * j], alpha[3 * j + 1], alpha[3 * j + 2]) * p.f[j];
+                den = vclamp(den, 0.0f, 10.0f);
+                const float mux = half_to_float(motion[2 * pix]), muy = half_to_float(motion[2 * pix + 1]);
+                V3f prev(0.0f);
+                float blend = 1.0f;
+                if (frame > 0 && mux >= 0.0f) {
+                    const uint16_t* layer = denoised + 4 * plane * (frame & 1u);
+                    float pv[4];
+                    bilinear(
+                        [&](int sx, int sy, float* o) {
+                            for (int k = 0; k < 4; ++k) o[k] = half_to_float(layer[4 * ((size_t)sy * W + sx) + k]);
+                        },
+                        W, H, mux, muy, pv, 4);
+                    prev = V3f(pv[0], pv[1], pv[2]);
+                    blend = std::fmax(1.0f / p.spp, 0.1f);
+                }
+                den = blend * den + (1.0f - blend) * prev;
+                uint16_t* out = denoised + 4 * plane * ((frame & 1u) ^ 1u) + 4 * pix;
+                out[0] = float_to_half(den.x);
+                out[1] = float_to_half(den.y);
+                out[2] = float_to_half(den.z);
+                out[3] = float_to_half(1.0f);
+                V3f alb((float)albedo[4 * pix] / 255.0f + 1e-6f, (float)albedo[4 * pix + 1] / 255.0f + 1e-6f,
+                        (float)albedo[4 * pix + 2] / 255.0f + 1e-6f);
+                V3f t = alb * den;
+                t = V3f(std::fmax(t.x, 0.0f), std::fmax(t.y, 0.0f), std::fmax(t.z, 0.0f));
+                t = vclamp(vpow(t, 0.454545f), 0.0f, 1.0f);
+                uint8_t* o = final_image + 4 * pix;
+                o[0] = unorm8(t.z);
+                o[1] = unorm8(t.y);
+                o[2] = unorm8(t.x);
+                o[3] = 255;
+            }
+        }
+}
+
+// bfrBlender.comp:22-69.  Window taps outside the image read as zero; a NaN
+// deviation takes t = 1 (IEEE minNum in min(t, 1)).
+static float mix3(float a, float b, float c, float t, float mid, float maxdev) {
+    t /= maxdev;
+    t = std::fmin(t, 1.0f);
+    float fa = std::fmax(1.0f - (t / mid), 0.0f);
+    float fb = (t < mid) ? t / mid : 1.0f - (t - mid) / (1.0f - mid);
+    float fc = 1.0f - fa - fb;
+    return fa * a + fb * b + fc * c;
+}
+
+void orc_bfr_blend(uint32_t w, uint32_t h, uint32_t radius, const uint16_t* average, const uint16_t* average_sq,
+                   const uint8_t* den8, const uint8_t* den16, const uint8_t* den32, uint8_t* final_image) {
+    const int W = (int)w, H = (int)h, R = (int)radius;
+    auto mean3 = [](const uint16_t* img, size_t pix) {
+        float s = half_to_float(img[4 * pix]) * (1.0f / 3.0f);
+        s += half_to_float(img[4 * pix + 1]) * (1.0f / 3.0f);
+        s += half_to_float(img[4 * pix + 2]) * (1.0f / 3.0f);
+        return s;
+    };
+    auto mix = [](float x, float y, float a) { return x * (1.0f - a) + y * a; };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float sq = 0.0f, av = 0.0f;
+            int count = 0;
+            for (int dy = -R; dy <= R; ++dy)
+                for (int dx = -R; dx <= R; ++dx) {
+                    const int sx = x + dx, sy = y + dy;
+                    float cur = 0.0f;
+                    if (sx >= 0 && sy >= 0 && sx < W && sy < H) cur = mean3(average, (size_t)sy * W + sx);
+                    ++count;
+                    sq = mix(sq, cur * cur, 1.0f / (float)count);
+                    av = mix(av, cur, 1.0f / (float)count);
+                }
+            const size_t pix = (size_t)y * W + x;
+            av = mix(av, mean3(average, pix), 0.5f);
+            sq = mix(sq, mean3(average_sq, pix), 0.5f);
+            const float sd = std::sqrt(sq - av * av);
+            uint8_t* o = final_image + 4 * pix;
+            for (int c = 0; c < 3; ++c) {
+                // B8G8R8A8: colour c (r, g, b) sits at byte 2 - c
+                const int bi = 2 - c;
+                float d2 = (float)den8[4 * pix + bi] / 255.0f, d1 = (float)den16[4 * pix + bi] / 255.0f,
+                      d0 = (float)den32[4 * pix + bi] / 255.0f;
+                o[bi] = unorm8(mix3(d0, d1, d2, sd, 0.5f, 1.0f));
+            }
+            o[3] = 255;
+        }
+}
+
 }  // extern "C"

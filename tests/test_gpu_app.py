@@ -1,0 +1,37 @@
+"""The render-module drop-in (vulkanpbrt_amd/app/pbrt_frame.cpp over renderModules/):
+every denoiser wiring of add_denoiser_to_commands runs, and replaying the recorded
+frame as a hipGraph gives the same bytes as submitting it eagerly."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+APP = Path(__file__).resolve().parents[1] / "vulkanpbrt_amd" / "pbrt_frame"
+
+
+def run_app(tmp_path, tag, *args):
+    out = tmp_path / f"{tag}.bin"
+    cmd = [str(APP), "--config", "1", "--frames", "4", "--raw", str(out), *args]
+    print("running", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    return np.fromfile(out, np.uint8).reshape(info["height"], info["width"], 4), info
+
+
+@pytest.mark.parametrize("denoiser,block", [("bmfr", "32"), ("bmfr", "8"), ("bfr", "8"), ("bfr", "16"),
+                                            ("bfr", "32"), ("bfr", "blend"), ("bmfr", "blend"), ("none", "32")])
+def test_app_eager_equals_graph(tmp_path, denoiser, block):
+    assert APP.exists(), "build the library first (python -m vulkanpbrt_amd.build)"
+    args = ["--denoiser", denoiser, "--block", block, "--taa"] if denoiser != "none" else ["--denoiser", "none"]
+    eager, info = run_app(tmp_path, "eager", *args)
+    graph, _ = run_app(tmp_path, "graph", *args, "--graph")
+    print(denoiser, block, "ms per frame", info["ms"])
+    assert np.array_equal(eager, graph)
+    assert np.all(eager[..., 3] == 255)
+    # a lit scene: the final image is not black
+    assert eager[..., :3].mean() > 5

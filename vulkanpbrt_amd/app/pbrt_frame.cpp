@@ -4,8 +4,8 @@
 // runs it per frame with updated push constants and camera matrices.
 //
 //   pbrt_frame [--config 1|2] [--seed S] [--width W --height H] [--frames F]
-//              [--depth D] [--denoiser bmfr|none] [--taa] [--graph] [--out final.ppm]
-//              [--raw image.bin]
+//              [--depth D] [--denoiser bmfr|bfr|none] [--block 8|16|32|blend] [--taa]
+//              [--graph] [--out final.ppm] [--raw image.bin]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -24,7 +24,7 @@ int main(int argc, char** argv) {
     uint32_t config = 1, frames = 4, depth = 2;
     uint64_t seed = 1;
     int width = 0, height = 0;
-    std::string denoiser = "bmfr", out, raw;
+    std::string denoiser = "bmfr", block = "32", out, raw;
     bool taa = false, graph = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -36,6 +36,7 @@ int main(int argc, char** argv) {
         else if (a == "--frames") frames = std::stoul(next());
         else if (a == "--depth") depth = std::stoul(next());
         else if (a == "--denoiser") denoiser = next();
+        else if (a == "--block") block = next();
         else if (a == "--taa") taa = true;
         else if (a == "--graph") graph = true;
         else if (a == "--out") out = next();
@@ -61,7 +62,12 @@ int main(int argc, char** argv) {
         auto push_constants = std::make_shared<PushConstants<RayTracingPushConstants>>();
         GBufferPtr g_buffer;
         IlluminationBufferPtr illumination_buffer;
-        bool denoise = denoiser == "bmfr";
+        bool denoise = denoiser == "bmfr" || denoiser == "bfr";
+        const DenoisingType dtype = denoiser == "bfr" ? DenoisingType::BFR : DenoisingType::BMFR;
+        const DenoisingBlockSize dsize = block == "8"    ? DenoisingBlockSize::X8
+                                         : block == "16" ? DenoisingBlockSize::X16
+                                         : block == "blend" ? DenoisingBlockSize::X8X16X32
+                                                            : DenoisingBlockSize::X32;
         if (denoise) {
             g_buffer = GBuffer::create(width, height);
             illumination_buffer = IlluminationBufferDemodulatedFloat(width, height);
@@ -84,7 +90,7 @@ int main(int argc, char** argv) {
             accumulator = Accumulator::create(g_buffer, pbrt_pipeline->get_illumination_buffer(), true);
             accumulator->compile_images(ctx);
             accumulator->add_dispatch_to_command_graph(commands);
-            add_denoiser_to_commands(DenoisingType::BMFR, DenoisingBlockSize::X32, commands, ctx, width, height,
+            add_denoiser_to_commands(dtype, dsize, commands, ctx, width, height,
                                      push_constants, g_buffer, accumulator->accumulated_illumination,
                                      accumulator->accumulation_buffer, final_image);
             if (taa) {

@@ -177,6 +177,20 @@ class BmfrImages(C.Structure):
         ("workspace_bytes", C.c_size_t)]
 
 
+class BfrParams(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "block_size", "frame_number")]
+
+
+class BfrImages(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "depth", "normal", "albedo", "motion", "spp", "noisy", "denoised", "final_image", "weights")]
+
+
+class BfrBlendImages(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "average", "average_squared", "denoised8", "denoised16", "denoised32", "final_image")]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "pbrt_abi_version": (C.c_int32, []),
@@ -199,6 +213,8 @@ SIGNATURES = {
     "pbrt_convert": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_bmfr_workspace_size": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(C.c_size_t)]),
     "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
+    "pbrt_bfr": (C.c_int32, [C.POINTER(BfrParams), C.POINTER(BfrImages), C.c_void_p]),
+    "pbrt_bfr_blend": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(BfrBlendImages), C.c_void_p]),
     "pbrt_generate_scene": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pbrt_generated_scene_desc": (C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)]),
     "pbrt_generated_scene_camera": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(CameraPose)]),

@@ -42,12 +42,6 @@ struct BmfrGeom {
     float* weights;      // [blocks_y][blocks_x][30]
 };
 
-__device__ __forceinline__ int mirror_coord(int x, int s) {
-    if (x < 0) return -x - 1;
-    if (x >= s) return 2 * s - x - 1;
-    return x;
-}
-
 // random(uint) of bmfrGeneral.comp:83-93 (Jenkins' 6-step integer hash)
 __device__ __forceinline__ float bmfr_random(uint32_t a) {
     a = (a + 0x7ed55d16u) + (a << 12);

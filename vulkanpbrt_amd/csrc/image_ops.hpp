@@ -25,6 +25,14 @@ __device__ __forceinline__ uint8_t unorm8_rne(float x) {
 }
 __device__ __forceinline__ float unorm8_to_float(uint32_t c) { return (float)c / 255.0f; }
 
+// mirror() of bmfrGeneral.comp and bfr.comp: reflect an index that lies at most
+// one image size outside [0, s)
+__device__ __forceinline__ int mirror_coord(int x, int s) {
+    if (x < 0) return -x - 1;
+    if (x >= s) return 2 * s - x - 1;
+    return x;
+}
+
 __device__ __forceinline__ int wrap_coord(int i, int n) {
     int r = i % n;
     return r < 0 ? r + n : r;

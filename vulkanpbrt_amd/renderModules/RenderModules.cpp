@@ -255,6 +255,70 @@ void BMFR::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const
 }
 
 // ---------------------------------------------------------------------------
+// BFR / BFRBlender (BFR.cpp, BFRBlender.cpp)
+// ---------------------------------------------------------------------------
+BFR::BFR(uint32_t width, uint32_t height, uint32_t work_width, uint32_t work_height, GBufferPtr g_buffer,
+         IlluminationBufferPtr illu_buffer, AccumulationBufferPtr acc_buffer)
+    : _width(width), _height(height), _work_width(work_width), _g_buffer(g_buffer), _illumination(illu_buffer),
+      _acc(acc_buffer) {
+    if (illu_buffer->kind != IlluminationKind::Demodulated)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Illumination Buffer type is required to be IlluminationBufferDemodulated for BFR");
+    if (work_width != work_height || (work_width != 8 && work_width != 16 && work_width != 32))
+        throw Exception(PBRT_ERR_UNSUPPORTED, "BFR: square 8, 16 or 32 pixel blocks");
+    _accumulated_illumination = std::make_shared<DeviceImage>(width, height, Format::RGBA16F, 2);
+    _final_illumination = std::make_shared<DeviceImage>(width, height, Format::BGRA8);
+}
+
+void BFR::compile(Context&) {
+    _accumulated_illumination->compile();
+    _final_illumination->compile();
+}
+
+void BFR::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
+    command_graph->addChild([this, push_constants](hipStream_t s) {
+        pbrt_bfr_params p{};
+        p.width = _width;
+        p.height = _height;
+        p.block_size = _work_width;
+        p.frame_number = push_constants->value().frame_number;
+        pbrt_bfr_images im{};
+        im.depth = _g_buffer->depth->as<float>();
+        im.normal = _g_buffer->normal->as<float>();
+        im.albedo = _g_buffer->albedo->as<uint8_t>();
+        im.motion = _acc->motion->as<uint16_t>();
+        im.spp = _acc->spp->as<uint8_t>();
+        im.noisy = _illumination->illumination_images[0]->as<uint16_t>();
+        im.denoised = _accumulated_illumination->as<uint16_t>();
+        im.final_image = _final_illumination->as<uint8_t>();
+        check(pbrt_bfr(&p, &im, s));
+    });
+}
+
+BFRBlender::BFRBlender(uint32_t width, uint32_t height, DeviceImagePtr average_image,
+                       DeviceImagePtr average_squared_image, DeviceImagePtr denoised0, DeviceImagePtr denoised1,
+                       DeviceImagePtr denoised2, uint32_t, uint32_t, uint32_t filter_radius)
+    : _width(width), _height(height), _filter_radius(filter_radius), _average(average_image),
+      _average_squared(average_squared_image), _den8(denoised0), _den16(denoised1), _den32(denoised2) {
+    _final = std::make_shared<DeviceImage>(width, height, Format::BGRA8);
+}
+
+void BFRBlender::compile(Context&) { _final->compile(); }
+
+void BFRBlender::add_dispatch_to_command_graph(const CommandsPtr& command_graph) {
+    command_graph->addChild([this](hipStream_t s) {
+        pbrt_bfr_blend_images im{};
+        im.average = _average->as<uint16_t>();
+        im.average_squared = _average_squared->as<uint16_t>();
+        im.denoised8 = _den8->as<uint8_t>();
+        im.denoised16 = _den16->as<uint8_t>();
+        im.denoised32 = _den32->as<uint8_t>();
+        im.final_image = _final->as<uint8_t>();
+        check(pbrt_bfr_blend(_width, _height, _filter_radius, &im, s));
+    });
+}
+
+// ---------------------------------------------------------------------------
 // Taa
 // ---------------------------------------------------------------------------
 Taa::Taa(uint32_t width, uint32_t height, uint32_t, uint32_t, GBufferPtr, AccumulationBufferPtr acc_buffer,
@@ -308,25 +372,50 @@ void add_denoiser_to_commands(DenoisingType denoising_type, DenoisingBlockSize d
     switch (denoising_type) {
         case DenoisingType::NONE:
             break;
-        case DenoisingType::BMFR: {
-            uint32_t b = 0, fk = 256;
+        case DenoisingType::BMFR:
+        case DenoisingType::BFR: {
+            const bool bmfr = denoising_type == DenoisingType::BMFR;
+            // one denoiser of block size b; returns its final image
+            auto add = [&](uint32_t b) -> DeviceImagePtr {
+                if (bmfr) {
+                    auto d = BMFR::create(width, height, b, b, g_buffer, illumination_buffer, accumulation_buffer,
+                                          b == 8 ? 64 : 256);
+                    d->compile(compile);
+                    d->update_image_layouts(compile);
+                    d->add_dispatch_to_command_graph(commands, compute_constants);
+                    commands->keep(d);
+                    return d->get_final_descriptor_image();
+                }
+                auto d = BFR::create(width, height, b, b, g_buffer, illumination_buffer, accumulation_buffer);
+                d->compile(compile);
+                d->update_image_layouts(compile);
+                d->add_dispatch_to_command_graph(commands, compute_constants);
+                commands->keep(d);
+                return d->get_final_descriptor_image();
+            };
             switch (denoising_size) {
-                case DenoisingBlockSize::X8: b = 8; fk = 64; break;
-                case DenoisingBlockSize::X16: b = 16; break;
-                case DenoisingBlockSize::X32: b = 32; break;
+                case DenoisingBlockSize::X8: final_descriptor_image = add(8); break;
+                case DenoisingBlockSize::X16: final_descriptor_image = add(16); break;
+                case DenoisingBlockSize::X32: final_descriptor_image = add(32); break;
+                case DenoisingBlockSize::X8X16X32: {
+                    // DenoiserUtils.cpp:48-70 / 103-123.  The BMFR branch of the reference
+                    // passes illumination_images[1] and [2] of a two-image buffer; both
+                    // branches here blend on the accumulated illumination and its square.
+                    auto d8 = add(8), d16 = add(16), d32 = add(32);
+                    auto blender = BFRBlender::create(width, height, illumination_buffer->illumination_images[0],
+                                                      illumination_buffer->illumination_images[1], d8, d16, d32);
+                    blender->compile(compile);
+                    blender->update_image_layouts(compile);
+                    blender->add_dispatch_to_command_graph(commands);
+                    commands->keep(blender);
+                    final_descriptor_image = blender->get_final_descriptor_image();
+                    break;
+                }
                 default:
-                    throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: BMFR block size not supported");
+                    throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: block size not supported");
             }
-            auto bmfr = BMFR::create(width, height, b, b, g_buffer, illumination_buffer, accumulation_buffer, fk);
-            bmfr->compile(compile);
-            bmfr->update_image_layouts(compile);
-            bmfr->add_dispatch_to_command_graph(commands, compute_constants);
-            final_descriptor_image = bmfr->get_final_descriptor_image();
-            commands->keep(bmfr);
             break;
         }
-        case DenoisingType::BFR:
-            throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: BFR is not available in this build");
         case DenoisingType::SVG:
             std::cout << "Not yet implemented" << std::endl;
             break;

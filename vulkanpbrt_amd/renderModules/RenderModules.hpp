@@ -121,6 +121,49 @@ private:
     std::unique_ptr<DeviceBuffer> _workspace;
 };
 
+// BFR.hpp: blockwise feature regression by Adam (bfr.comp)
+class BFR {
+public:
+    BFR(uint32_t width, uint32_t height, uint32_t work_width, uint32_t work_height, GBufferPtr g_buffer,
+        IlluminationBufferPtr illu_buffer, AccumulationBufferPtr acc_buffer);
+    static std::shared_ptr<BFR> create(uint32_t w, uint32_t h, uint32_t ww, uint32_t wh, GBufferPtr g,
+                                       IlluminationBufferPtr i, AccumulationBufferPtr a) {
+        return std::make_shared<BFR>(w, h, ww, wh, g, i, a);
+    }
+    void compile(Context& context);
+    void update_image_layouts(Context&) {}
+    void add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants);
+    DeviceImagePtr get_final_descriptor_image() const { return _final_illumination; }
+
+private:
+    uint32_t _width, _height, _work_width;
+    GBufferPtr _g_buffer;
+    IlluminationBufferPtr _illumination;
+    AccumulationBufferPtr _acc;
+    DeviceImagePtr _accumulated_illumination;  // RGBA16F x 2 layers
+    DeviceImagePtr _final_illumination;        // BGRA8
+};
+
+// BFRBlender.hpp: per-pixel blend of the 8x8 / 16x16 / 32x32 denoiser outputs
+class BFRBlender {
+public:
+    BFRBlender(uint32_t width, uint32_t height, DeviceImagePtr average_image, DeviceImagePtr average_squared_image,
+               DeviceImagePtr denoised0, DeviceImagePtr denoised1, DeviceImagePtr denoised2,
+               uint32_t work_width = 16, uint32_t work_height = 16, uint32_t filter_radius = 2);
+    static std::shared_ptr<BFRBlender> create(uint32_t w, uint32_t h, DeviceImagePtr avg, DeviceImagePtr avg_sq,
+                                              DeviceImagePtr d0, DeviceImagePtr d1, DeviceImagePtr d2) {
+        return std::make_shared<BFRBlender>(w, h, avg, avg_sq, d0, d1, d2);
+    }
+    void compile(Context& context);
+    void update_image_layouts(Context&) {}
+    void add_dispatch_to_command_graph(const CommandsPtr& command_graph);
+    DeviceImagePtr get_final_descriptor_image() const { return _final; }
+
+private:
+    uint32_t _width, _height, _filter_radius;
+    DeviceImagePtr _average, _average_squared, _den8, _den16, _den32, _final;
+};
+
 class Taa {
 public:
     Taa(uint32_t width, uint32_t height, uint32_t work_width, uint32_t work_height, GBufferPtr g_buffer,
