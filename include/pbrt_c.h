@@ -369,6 +369,25 @@ typedef struct pbrt_bmfr_images {
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
+/* Batched BMFR: every block of many frames (any mix of resolutions) fitted in
+ * one launch and post-processed in a second, instead of one BMFR dispatch chain
+ * per frame (BMFR::add_dispatch_to_command_graph, BMFR.cpp:206-229).  All frames
+ * share block_size and fit, and the fit is PBRT_BMFR_FIT_GRAM_F16 or _F64 (the
+ * Householder path stages features per frame).  images.workspace of each frame
+ * is ignored: the batch workspace holds the frame table and every frame's
+ * weights.  The frame table is staged through a pinned host buffer owned by the
+ * library; a call waits only for the previous call's table upload. */
+typedef struct pbrt_bmfr_batch_frame {
+    pbrt_bmfr_params params;
+    pbrt_bmfr_images images;
+} pbrt_bmfr_batch_frame;
+
+int32_t pbrt_bmfr_batched_workspace_size(const pbrt_bmfr_batch_frame* frames, uint32_t count, size_t* out_bytes);
+/* Workspace layout: frame table, then per frame (256-byte aligned) float weights
+ * [blocks_y][blocks_x][30] in frame order. */
+int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32_t count, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* BFR (blockwise feature regression by 40 Adam steps): replaces bfr.comp      */
 /* (shaders/bfr.comp:202-308, parallel_reduction_alpha :100-142) dispatched by */

@@ -177,6 +177,10 @@ class BmfrImages(C.Structure):
         ("workspace_bytes", C.c_size_t)]
 
 
+class BmfrBatchFrame(C.Structure):
+    _fields_ = [("params", BmfrParams), ("images", BmfrImages)]
+
+
 class BfrParams(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("width", "height", "block_size", "frame_number")]
 
@@ -213,6 +217,8 @@ SIGNATURES = {
     "pbrt_convert": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_bmfr_workspace_size": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(C.c_size_t)]),
     "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
+    "pbrt_bmfr_batched_workspace_size": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pbrt_bmfr_batched": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbrt_bfr": (C.c_int32, [C.POINTER(BfrParams), C.POINTER(BfrImages), C.c_void_p]),
     "pbrt_bfr_blend": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(BfrBlendImages), C.c_void_p]),
     "pbrt_generate_scene": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),

@@ -25,6 +25,9 @@
 //   there, the FP64-accumulated one does not.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "capi_common.hpp"
 #include "image_ops.hpp"
 
@@ -241,22 +244,26 @@ struct GramLds {
 };
 
 template <int kRows, bool kF64>
-__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
+__device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
     constexpr int kSub = (kRows + kFitThreads - 1) / kFitThreads;
     __shared__ float fscratch[16];
     __shared__ double dscratch[16];
     __shared__ GramLds<kRows, kF64> L;
     __shared__ double dots[kAlphaSize];
     __shared__ float scale[12];
+    __shared__ double G[12][12];
+    __shared__ double Lc[kFeatures - 1][kFeatures - 1];
+    __shared__ int keep[kFeatures - 1];
     const int tid = threadIdx.x;
     const int bs = g.bs;
     const uint32_t pb = (uint32_t)(bs * bs);
-    const int basex = blockIdx.x * bs, basey = blockIdx.y * bs;
+    const int basex = bx * bs, basey = by * bs;
 
     // 1. load rows; block min/max of depth
     float a[kSub][kAlphaSize];
     float dmin = 3.4e38f, dmax = -3.4e38f;
     PixelIn px[kSub];
+    #pragma unroll
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r < kRows) {
@@ -268,6 +275,7 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
     const float mn = block_reduce(dmin, OpMin(), fscratch);
     const float mx = block_reduce(dmax, OpMax(), fscratch);
     // 2. fp16-quantised features (the reference's R16F feature buffer) + noise
+    #pragma unroll
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r >= kRows) continue;
@@ -286,6 +294,7 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
     // 3. FP64 dot products with the constant column: a0.a0, a0.A_j, a0.y_c
     double d_local[kAlphaSize];
     for (int c = 0; c < kAlphaSize; ++c) d_local[c] = 0.0;
+    #pragma unroll
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r >= kRows) continue;
@@ -303,6 +312,7 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
     float b[kSub][12];
     float amax[12];
     for (int c = 0; c < 12; ++c) amax[c] = 0.0f;
+    #pragma unroll
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r >= kRows) continue;
@@ -322,6 +332,7 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
     }
     __syncthreads();
     // 5. stage the scaled columns in LDS
+    #pragma unroll
     for (int s = 0; s < kSub; ++s) {
         const int r = tid + s * kFitThreads;
         if (r >= kRows) continue;
@@ -380,66 +391,84 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_
         }
     }
     __syncthreads();
-    // 7. FP64 solve on one lane
-    if (tid == 0) {
-        double G[12][12];
-        for (int i = 0; i < 12; ++i)
-            for (int j = 0; j < 12; ++j) {
-                double v = L.gram[0][i * kGramCols + j] + L.gram[1][i * kGramCols + j] + L.gram[2][i * kGramCols + j] +
-                           L.gram[3][i * kGramCols + j];
-                G[i][j] = v / ((double)scale[i] * (double)scale[j]);
-            }
-        for (int i = 0; i < 12; ++i)
-            for (int j = 0; j < i; ++j) G[i][j] = G[j][i] = 0.5 * (G[i][j] + G[j][i]);
-        constexpr int n = kFeatures - 1;  // projected features A1..A9
-        double Lc[n][n];
-        bool keep[n];
-        double maxdiag = 0.0;
-        for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, G[i][i]);
-        for (int j = 0; j < n; ++j) {
+    // 7. FP64 solve of the projected normal equations: the symmetrised, unscaled
+    //    Gram matrix in LDS; Cholesky column by column (wave 0: every lane forms
+    //    the pivot, lane i > j its row of the column; a pivot below 1e-15 of the
+    //    largest diagonal drops its feature); then one lane per colour solves for
+    //    w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+    constexpr int n = kFeatures - 1;  // projected features A1..A9
+    constexpr int pw = kGramCols * kGramCols;
+    for (int e = tid; e < 12 * 12; e += kFitThreads) {
+        const int i = e / 12, j = e % 12;
+        const int a = i * kGramCols + j, b = j * kGramCols + i;
+        const double* gr = &L.gram[0][0];
+        const double gij = gr[a] + gr[pw + a] + gr[2 * pw + a] + gr[3 * pw + a];
+        const double gji = gr[b] + gr[pw + b] + gr[2 * pw + b] + gr[3 * pw + b];
+        G[i][j] = 0.5 * (gij + gji) / ((double)scale[i] * (double)scale[j]);
+    }
+    __syncthreads();
+    double maxdiag = 0.0;
+    for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, G[i][i]);
+    for (int j = 0; j < n; ++j) {
+        if (tid < 64) {
             double d = G[j][j];
             for (int k = 0; k < j; ++k) d -= Lc[j][k] * Lc[j][k];
-            keep[j] = d > 1e-15 * maxdiag && d > 0.0;
-            double ljj = keep[j] ? sqrt(d) : 1.0;
-            Lc[j][j] = ljj;
-            for (int i = j + 1; i < n; ++i) {
-                double v = G[i][j];
-                for (int k = 0; k < j; ++k) v -= Lc[i][k] * Lc[j][k];
-                Lc[i][j] = keep[j] ? v / ljj : 0.0;
+            const bool kp = d > 1e-15 * maxdiag && d > 0.0;
+            const double ljj = kp ? sqrt(d) : 1.0;
+            if (tid == j) {
+                Lc[j][j] = ljj;
+                keep[j] = kp ? 1 : 0;
+            } else if (tid > j && tid < n) {
+                double v = G[tid][j];
+                for (int k = 0; k < j; ++k) v -= Lc[tid][k] * Lc[j][k];
+                Lc[tid][j] = kp ? v / ljj : 0.0;
             }
         }
-        float* out = g.weights + ((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30;
-        for (int c = 0; c < 3; ++c) {
-            double z[n], w[n];
-            for (int i = 0; i < n; ++i) {
-                double v = keep[i] ? G[i][9 + c] : 0.0;
-                for (int k = 0; k < i; ++k) v -= Lc[i][k] * z[k];
-                z[i] = keep[i] ? v / Lc[i][i] : 0.0;
-            }
-            for (int i = n - 1; i >= 0; --i) {
-                double v = z[i];
-                for (int k = i + 1; k < n; ++k) v -= Lc[k][i] * w[k];
-                w[i] = keep[i] ? v / Lc[i][i] : 0.0;
-            }
-            double w0 = dots[10 + c];
-            for (int i = 0; i < n; ++i) w0 -= w[i] * dots[i + 1];
-            w0 /= dots[0];
-            out[c] = (float)w0;
-            for (int i = 0; i < n; ++i) out[(i + 1) * 3 + c] = (float)w[i];
-        }
+        __syncthreads();
     }
+    if (tid < 3) {
+        const int c = tid;
+        float* out = g.weights + ((size_t)by * g.blocks_x + bx) * 30;
+        double z[n], w[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            double v = keep[i] ? G[i][9 + c] : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) v -= Lc[i][k] * z[k];
+            z[i] = keep[i] ? v / Lc[i][i] : 0.0;
+        }
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i) {
+            double v = z[i];
+#pragma unroll
+            for (int k = i + 1; k < n; ++k) v -= Lc[k][i] * w[k];
+            w[i] = keep[i] ? v / Lc[i][i] : 0.0;
+        }
+        double w0 = dots[10 + c];
+#pragma unroll
+        for (int i = 0; i < n; ++i) w0 -= w[i] * dots[i + 1];
+        w0 /= dots[0];
+        out[c] = (float)w0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[(i + 1) * 3 + c] = (float)w[i];
+    }
+}
+
+template <int kRows, bool kF64>
+__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
+    bmfr_fit_gram_block<kRows, kF64>(g, im, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
 // bmfrPost: weighted feature sum, temporal blend, remodulation, gamma
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
+__device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
     __shared__ float scratch[16];
     __shared__ float wts[30];
     const int lx = threadIdx.x % g.bs, ly = threadIdx.x / g.bs;
-    const int gx = blockIdx.x * g.bs + lx, gy = blockIdx.y * g.bs + ly;
+    const int gx = bx * g.bs + lx, gy = by * g.bs + ly;
     if (threadIdx.x < 30) {
-        float w = g.weights[((size_t)blockIdx.y * g.blocks_x + blockIdx.x) * 30 + threadIdx.x];
+        float w = g.weights[((size_t)by * g.blocks_x + bx) * 30 + threadIdx.x];
         wts[threadIdx.x] = (isinf(w) || isnan(w)) ? 0.0f : w;
     }
     PixelIn p = load_pixel(g, im, gx, gy);
@@ -475,6 +504,49 @@ __global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images
     t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
     t = vclamp(vpow(t, 0.454545f), 0.0f, 1.0f);
     store_bgra8(im.final_image, pix, make_vec4(t, 1.0f));
+}
+
+__global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
+    bmfr_post_block(g, im, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched BMFR: one fit launch and one post launch over the blocks of many frames.
+// Block b of the launch belongs to the last frame whose first_block <= b.
+// ---------------------------------------------------------------------------
+struct BmfrBatchFrame {
+    BmfrGeom g;
+    pbrt_bmfr_images im;
+    uint32_t first_block;
+};
+
+__device__ __forceinline__ uint32_t batch_frame_of(const BmfrBatchFrame* frames, uint32_t n, uint32_t b) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (frames[mid].first_block <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int kRows, bool kF64>
+__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
+    const uint32_t f = batch_frame_of(frames, n, blockIdx.x);
+    const BmfrGeom g = frames[f].g;
+    const pbrt_bmfr_images im = frames[f].im;
+    const uint32_t local = blockIdx.x - frames[f].first_block;
+    bmfr_fit_gram_block<kRows, kF64>(g, im, (int)(local % (uint32_t)g.blocks_x), (int)(local / (uint32_t)g.blocks_x));
+}
+
+__global__ void __launch_bounds__(1024) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
+    const uint32_t f = batch_frame_of(frames, n, blockIdx.x);
+    const BmfrGeom g = frames[f].g;
+    const pbrt_bmfr_images im = frames[f].im;
+    const uint32_t local = blockIdx.x - frames[f].first_block;
+    bmfr_post_block(g, im, (int)(local % (uint32_t)g.blocks_x), (int)(local / (uint32_t)g.blocks_x));
 }
 
 // pixelOffsets of bmfrGeneral.comp:16
@@ -560,6 +632,133 @@ extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* 
                 hipLaunchKernelGGL((k_bmfr_fit_gram<64, false>), grid, dim3(kFitThreads), 0, st, g, *im);
         }
         hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, *im);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Frame table of a batch and the workspace offsets of every frame's weights.
+struct BatchPlan {
+    std::vector<BmfrBatchFrame> table;
+    size_t table_bytes = 0, total_bytes = 0;
+    uint32_t blocks = 0;
+    int bs = 0;
+    uint32_t fit = 0;
+};
+
+BatchPlan plan_batch(const pbrt_bmfr_batch_frame* frames, uint32_t count, char* workspace) {
+    PBRT_REQUIRE(frames && count > 0, "pbrt_bmfr_batched: no frames");
+    BatchPlan plan;
+    plan.table.resize(count);
+    plan.table_bytes = align256(sizeof(BmfrBatchFrame) * count);
+    size_t off = plan.table_bytes;
+    uint64_t blocks = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const pbrt_bmfr_params& p = frames[i].params;
+        PBRT_REQUIRE(p.fit == PBRT_BMFR_FIT_GRAM_F16 || p.fit == PBRT_BMFR_FIT_GRAM_F64,
+                     "pbrt_bmfr_batched: the batched fit is PBRT_BMFR_FIT_GRAM_F16 or PBRT_BMFR_FIT_GRAM_F64");
+        if (i == 0) {
+            plan.bs = (int)p.block_size;
+            plan.fit = p.fit;
+        }
+        PBRT_REQUIRE((int)p.block_size == plan.bs && p.fit == plan.fit,
+                     "pbrt_bmfr_batched: every frame of a batch needs the same block_size and fit");
+        BmfrBatchFrame& e = plan.table[i];
+        e.g = make_geom(p);
+        e.im = frames[i].images;
+        e.im.workspace = nullptr;
+        e.im.workspace_bytes = 0;
+        e.g.features = nullptr;
+        e.g.weights = workspace ? reinterpret_cast<float*>(workspace + off) : nullptr;
+        e.first_block = (uint32_t)blocks;
+        blocks += (uint64_t)e.g.blocks_x * e.g.blocks_y;
+        off = align256(off + weight_bytes(e.g));
+    }
+    PBRT_REQUIRE(blocks < (1ull << 31), "pbrt_bmfr_batched: too many blocks for one launch");
+    plan.blocks = (uint32_t)blocks;
+    plan.total_bytes = off;
+    return plan;
+}
+
+// Pinned staging buffer for frame tables; reused once the previous upload is done.
+struct TableStaging {
+    std::mutex mu;
+    void* host = nullptr;
+    size_t capacity = 0;
+    hipEvent_t uploaded = nullptr;
+    bool pending = false;
+};
+
+TableStaging& table_staging() {
+    static TableStaging* t = new TableStaging();  // lives until the process ends
+    return *t;
+}
+
+}  // namespace
+
+extern "C" int32_t pbrt_bmfr_batched_workspace_size(const pbrt_bmfr_batch_frame* frames, uint32_t count,
+                                                    size_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_bmfr_batched_workspace_size: NULL output");
+        *out = plan_batch(frames, count, nullptr).total_bytes;
+    });
+}
+
+extern "C" int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32_t count, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(workspace, "pbrt_bmfr_batched: NULL workspace");
+        for (uint32_t i = 0; frames && i < count; ++i) {
+            const pbrt_bmfr_images* im = &frames[i].images;
+            PBRT_REQUIRE(im->depth && im->normal && im->albedo && im->motion && im->spp && im->noisy &&
+                             im->denoised && im->final_image,
+                         "pbrt_bmfr_batched: missing image");
+        }
+        BatchPlan plan = plan_batch(frames, count, static_cast<char*>(workspace));
+        PBRT_REQUIRE(workspace_bytes >= plan.total_bytes, "pbrt_bmfr_batched: workspace too small");
+        hipStream_t st = (hipStream_t)stream;
+        const size_t bytes = sizeof(BmfrBatchFrame) * count;
+        {
+            TableStaging& ts = table_staging();
+            std::lock_guard<std::mutex> lock(ts.mu);
+            if (ts.pending) PBRT_HIP(hipEventSynchronize(ts.uploaded));
+            ts.pending = false;
+            if (ts.capacity < bytes) {
+                if (ts.host) PBRT_HIP(hipHostFree(ts.host));
+                ts.host = nullptr;
+                ts.capacity = 0;
+                PBRT_HIP(hipHostMalloc(&ts.host, bytes, hipHostMallocDefault));
+                ts.capacity = bytes;
+            }
+            if (!ts.uploaded) PBRT_HIP(hipEventCreateWithFlags(&ts.uploaded, hipEventDisableTiming));
+            std::memcpy(ts.host, plan.table.data(), bytes);
+            PBRT_HIP(hipMemcpyAsync(workspace, ts.host, bytes, hipMemcpyHostToDevice, st));
+            PBRT_HIP(hipEventRecord(ts.uploaded, st));
+            ts.pending = true;
+        }
+        const BmfrBatchFrame* table = static_cast<const BmfrBatchFrame*>(workspace);
+        const dim3 grid(plan.blocks);
+        const bool f64 = plan.fit == PBRT_BMFR_FIT_GRAM_F64;
+        if (plan.bs == 32) {
+            if (f64)
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<1024, true>), grid, dim3(kFitThreads), 0, st, table, count);
+            else
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<1024, false>), grid, dim3(kFitThreads), 0, st, table, count);
+        } else if (plan.bs == 16) {
+            if (f64)
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<256, true>), grid, dim3(kFitThreads), 0, st, table, count);
+            else
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<256, false>), grid, dim3(kFitThreads), 0, st, table, count);
+        } else {
+            if (f64)
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<64, true>), grid, dim3(kFitThreads), 0, st, table, count);
+            else
+                hipLaunchKernelGGL((k_bmfr_fit_gram_batched<64, false>), grid, dim3(kFitThreads), 0, st, table, count);
+        }
+        hipLaunchKernelGGL(k_bmfr_post_batched, grid, dim3(plan.bs * plan.bs), 0, st, table, count);
         PBRT_HIP(hipGetLastError());
     });
 }
