@@ -434,6 +434,57 @@ typedef struct pbrt_bfr_blend_images {
 int32_t pbrt_bfr_blend(uint32_t width, uint32_t height, uint32_t filter_radius,
                        const pbrt_bfr_blend_images* images, void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Learned kernel-predicting denoiser.  No counterpart in the reference: it    */
+/* stands in for BMFR's per-block fit (bmfrFit.comp) with a network that reads */
+/* the same accumulator / G-buffer images as bmfrPre.comp and writes the same  */
+/* outputs as bmfrPost.comp (demodulated denoised radiance, remodulated BGRA8).*/
+/*   features (12): log1p(noisy rgb), albedo rgb, normal xyz, depth/(1+depth), */
+/*                  min(spp,16)/16, motion valid                               */
+/*   6 x conv3x3 (12->32->32->32->32->32, ReLU, ->25 logits), BF16 weights and */
+/*   activations, FP32 accumulation on the matrix cores; softmax = per-pixel   */
+/*   5x5 kernel (taps clamped to the image) applied to the noisy illumination. */
+/*   loss: mean (out - target)^2 / (target^2 + 0.01) over pixels and rgb.      */
+/* Parameters in canonical order, per layer W[cout][cin][ky][kx] then b[cout]. */
+/* ------------------------------------------------------------------------- */
+
+typedef struct pbrt_cnn pbrt_cnn;
+
+typedef struct pbrt_cnn_images {
+    uint32_t width;
+    uint32_t height;
+    uint32_t count;           /* images of width x height stored back to back */
+    const uint16_t* noisy;    /* RGBA16F demodulated illumination (accumulator output) */
+    const float* depth;       /* R32F */
+    const float* normal;      /* RG32F spherical */
+    const uint8_t* albedo;    /* RGBA8 */
+    const uint8_t* spp;       /* R8 sample count / 256 */
+    const uint16_t* motion;   /* RG16F */
+    float* denoised;          /* RGBA32F out (denoise): demodulated denoised radiance, may be NULL */
+    uint8_t* final_image;     /* B8G8R8A8 out (denoise): remodulated, gamma 1/2.2, may be NULL */
+    const float* target;      /* RGBA32F reference radiance (training and loss) */
+    void* workspace;          /* device scratch of pbrt_cnn_workspace_size() bytes */
+    size_t workspace_bytes;
+} pbrt_cnn_images;
+
+int32_t pbrt_cnn_create(uint64_t seed, pbrt_cnn** out_net); /* He-normal init from the seed */
+int32_t pbrt_cnn_destroy(pbrt_cnn* net);
+int32_t pbrt_cnn_parameter_count(uint32_t* out_count);
+int32_t pbrt_cnn_get_parameters(const pbrt_cnn* net, float* host_params);
+int32_t pbrt_cnn_set_parameters(pbrt_cnn* net, const float* host_params); /* also resets the optimiser */
+int32_t pbrt_cnn_workspace_size(uint32_t width, uint32_t height, uint32_t count, uint32_t training,
+                                size_t* out_bytes);
+/* Inference: denoised and/or final_image of every image. */
+int32_t pbrt_cnn_denoise(const pbrt_cnn* net, const pbrt_cnn_images* images, void* stream);
+/* Loss and its gradient (device pointers: 1 float, parameter_count floats in
+ * canonical order) without changing the network. */
+int32_t pbrt_cnn_gradients(const pbrt_cnn* net, const pbrt_cnn_images* images, float* loss, float* grad,
+                           void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch; loss (device, 1 float,
+ * may be NULL) receives the batch loss before the step. */
+int32_t pbrt_cnn_train_step(pbrt_cnn* net, const pbrt_cnn_images* images, float learning_rate, float* loss,
+                            void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -195,6 +195,12 @@ class BfrBlendImages(C.Structure):
         "average", "average_squared", "denoised8", "denoised16", "denoised32", "final_image")]
 
 
+class CnnImages(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("count", C.c_uint32)] + [
+        (n, C.c_void_p) for n in ("noisy", "depth", "normal", "albedo", "spp", "motion", "denoised", "final_image",
+                                  "target", "workspace")] + [("workspace_bytes", C.c_size_t)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "pbrt_abi_version": (C.c_int32, []),
@@ -219,6 +225,15 @@ SIGNATURES = {
     "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
     "pbrt_bmfr_batched_workspace_size": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.POINTER(C.c_size_t)]),
     "pbrt_bmfr_batched": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "pbrt_cnn_create": (C.c_int32, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pbrt_cnn_destroy": (C.c_int32, [C.c_void_p]),
+    "pbrt_cnn_parameter_count": (C.c_int32, [C.POINTER(C.c_uint32)]),
+    "pbrt_cnn_get_parameters": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "pbrt_cnn_set_parameters": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "pbrt_cnn_workspace_size": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pbrt_cnn_denoise": (C.c_int32, [C.c_void_p, C.POINTER(CnnImages), C.c_void_p]),
+    "pbrt_cnn_gradients": (C.c_int32, [C.c_void_p, C.POINTER(CnnImages), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbrt_cnn_train_step": (C.c_int32, [C.c_void_p, C.POINTER(CnnImages), C.c_float, C.c_void_p, C.c_void_p]),
     "pbrt_bfr": (C.c_int32, [C.POINTER(BfrParams), C.POINTER(BfrImages), C.c_void_p]),
     "pbrt_bfr_blend": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(BfrBlendImages), C.c_void_p]),
     "pbrt_generate_scene": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),

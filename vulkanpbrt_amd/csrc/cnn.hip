@@ -1,0 +1,722 @@
+// Learned kernel-predicting denoiser on gfx950: BF16 matrix-core convolutions,
+// forward and backward, and an Adam trainer.
+//
+// Data layout (per image, row-major pixels):
+//   features       NHWC bf16, 16 channels (12 used)
+//   activations    NHWC bf16, 32 channels; logits: 32 channels (25 used)
+//   master weights fp32, per layer W[tap 9][cout 32][cin_pad] then b[32]
+//                  (cin_pad 16 for the first layer, 32 after; padded slots stay 0)
+//   forward bf16   same W layout; backward bf16 Wt[tap][cin][cout] with the taps
+//                  flipped (the transposed convolution as a forward convolution)
+//
+// k_conv3x3       implicit GEMM per 16x16 pixel tile: the 18x18 input halo is
+//                 staged in LDS; each wave owns four 16-pixel rows and computes
+//                 D[cout][pixel] = sum_tap W_tap^T X_tap on v_mfma_f32_16x16x32_bf16
+//                 (16x16x16 for the 16-channel input layer), so each lane ends
+//                 with four consecutive output channels of one pixel.  The same
+//                 kernel computes the input gradient from the flipped weights,
+//                 masking the upstream gradient by the ReLU of the activation.
+// k_conv3x3_wgrad per tile: dZ^T (channel-major) and three x-shifted,
+//                 channel-major copies of the input halo in LDS; every tap's
+//                 32 x cin weight gradient is a GEMM over the tile's 256 pixels;
+//                 per-tile partials are summed in a fixed order by k_reduce.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "capi_common.hpp"
+#include "image_ops.hpp"
+
+namespace pbrt {
+namespace {
+
+constexpr int kLayers = 6;
+constexpr int kTaps = 9;
+constexpr int kCh = 32;
+constexpr int kFeatCh = 16;
+constexpr int kFeatUsed = 12;
+constexpr int kKernelTaps = 25;
+constexpr int kTile = 16;
+constexpr float kRelEps = 0.01f;
+
+constexpr int cin_pad(int l) { return l == 0 ? kFeatCh : kCh; }
+constexpr int cin_real(int l) { return l == 0 ? kFeatUsed : kCh; }
+constexpr int cout_real(int l) { return l == kLayers - 1 ? kKernelTaps : kCh; }
+constexpr int layer_weights(int l) { return kTaps * kCh * cin_pad(l); }
+constexpr int layer_size(int l) { return layer_weights(l) + kCh; }
+constexpr int layer_offset(int l) { return l == 0 ? 0 : layer_offset(l - 1) + layer_size(l - 1); }
+constexpr int kMasterSize = layer_offset(kLayers);
+constexpr int weight_offset(int l) { return l == 0 ? 0 : weight_offset(l - 1) + layer_weights(l - 1); }
+constexpr int kWeightSize = weight_offset(kLayers);
+
+int canonical_count() {
+    int n = 0;
+    for (int l = 0; l < kLayers; ++l) n += cout_real(l) * cin_real(l) * kTaps + cout_real(l);
+    return n;
+}
+
+// canonical index -> padded master index
+std::vector<int> canonical_map() {
+    std::vector<int> map;
+    for (int l = 0; l < kLayers; ++l) {
+        for (int co = 0; co < cout_real(l); ++co)
+            for (int ci = 0; ci < cin_real(l); ++ci)
+                for (int t = 0; t < kTaps; ++t) map.push_back(layer_offset(l) + (t * kCh + co) * cin_pad(l) + ci);
+        for (int co = 0; co < cout_real(l); ++co) map.push_back(layer_offset(l) + layer_weights(l) + co);
+    }
+    return map;
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ uint32_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ bool bf_positive(uint32_t h) { return (h & 0x8000u) == 0 && (h & 0x7FFFu) != 0; }
+
+// zero the 16-bit lanes of v whose mask value is not > 0
+__device__ __forceinline__ uint32_t relu_mask2(uint32_t v, uint32_t m) {
+    uint32_t lo = bf_positive(m & 0xFFFFu) ? (v & 0xFFFFu) : 0u;
+    uint32_t hi = bf_positive(m >> 16) ? (v & 0xFFFF0000u) : 0u;
+    return lo | hi;
+}
+__device__ __forceinline__ uint4 relu_mask8(uint4 v, uint4 m) {
+    return make_uint4(relu_mask2(v.x, m.x), relu_mask2(v.y, m.y), relu_mask2(v.z, m.z), relu_mask2(v.w, m.w));
+}
+
+struct CnnGeom {
+    int width, height, count;
+};
+
+// ---------------------------------------------------------------------------
+// Features
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_features(CnnGeom g, pbrt_cnn_images im, uint16_t* feat) {
+    const size_t n = (size_t)g.width * g.height * g.count;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const vec3 nz = xyz(load_half4(im.noisy, i));
+    const vec4 alb = load_rgba8(im.albedo, i);
+    const float2 sn = reinterpret_cast<const float2*>(im.normal)[i];
+    const vec3 nn = decode_normal(sn.x, sn.y);
+    const float d = im.depth[i];
+    const float spp = unorm8_to_float(im.spp[i]) * 256.0f;
+    const float mvx = h2f(reinterpret_cast<const ushort2*>(im.motion)[i].x);
+    float f[kFeatCh] = {log1pf(fmaxf(nz.x, 0.0f)), log1pf(fmaxf(nz.y, 0.0f)), log1pf(fmaxf(nz.z, 0.0f)),
+                        alb.x, alb.y, alb.z, nn.x, nn.y, nn.z, d / (1.0f + d), fminf(spp, 16.0f) / 16.0f,
+                        mvx >= 0.0f ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = f2bf(f[2 * k]) | (f2bf(f[2 * k + 1]) << 16);
+    uint4* o = reinterpret_cast<uint4*>(feat + i * kFeatCh);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution (forward, or input gradient with flipped weights)
+// ---------------------------------------------------------------------------
+enum ConvMode { kConvRelu = 0, kConvLinear = 1, kConvDgrad = 2 };
+
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) k_conv3x3(CnnGeom g, const uint16_t* __restrict__ in,
+                                                 const uint16_t* __restrict__ mask, const uint16_t* __restrict__ wgt,
+                                                 const float* __restrict__ bias, uint16_t* __restrict__ out) {
+    constexpr int kStride = CIN + 8;  // bf16 per LDS pixel: conflict-free 16 / 8 byte lane reads
+    constexpr int kChunks = CIN / 8;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[18 * 18 * kStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile, img = blockIdx.z;
+    const size_t plane = (size_t)g.width * g.height;
+    const uint16_t* src = in + plane * img * CIN;
+    const uint16_t* msk = mask ? mask + plane * img * CIN : nullptr;
+    for (int e = tid; e < 18 * 18 * kChunks; e += 256) {
+        const int p = e / kChunks, c = e % kChunks;
+        const int gy = y0 + p / 18 - 1, gx = x0 + p % 18 - 1;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) {
+            const size_t off = ((size_t)gy * g.width + gx) * CIN + c * 8;
+            v = *reinterpret_cast<const uint4*>(src + off);
+            if (MODE == kConvDgrad && msk) v = relu_mask8(v, *reinterpret_cast<const uint4*>(msk + off));
+        }
+        *reinterpret_cast<uint4*>(&tile[p * kStride + c * 8]) = v;
+    }
+    // weights (A operand): lane holds output channel ct*16 + lane%16
+    const int j = lane & 15, grp = lane >> 4;
+    float4v acc[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    if (CIN == 32) {
+#pragma unroll
+        for (int t = 0; t < kTaps; ++t) {
+            const int dy = t / 3, dx = t % 3;
+            bf16x8 a[2];
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                a[ct] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(wgt + (t * kCh + ct * 16 + j) * CIN + 8 * grp));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ty = wave * 4 + r;
+                const bf16x8 b = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(&tile[((ty + dy) * 18 + j + dx) * kStride + 8 * grp]));
+                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b, acc[r][0], 0, 0, 0);
+                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b, acc[r][1], 0, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < kTaps; ++t) {
+            const int dy = t / 3, dx = t % 3;
+            short4v a[2];
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                a[ct] = __builtin_bit_cast(
+                    short4v, *reinterpret_cast<const uint2*>(wgt + (t * kCh + ct * 16 + j) * CIN + 4 * grp));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ty = wave * 4 + r;
+                const short4v b = __builtin_bit_cast(
+                    short4v, *reinterpret_cast<const uint2*>(&tile[((ty + dy) * 18 + j + dx) * kStride + 4 * grp]));
+                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b, acc[r][0], 0, 0, 0);
+                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b, acc[r][1], 0, 0, 0);
+            }
+        }
+    }
+    // D[cout][pixel]: lane holds pixel j, output channels ct*16 + 4*grp + i
+    const int gx = x0 + j;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int gy = y0 + wave * 4 + r;
+        if (gx >= g.width || gy >= g.height) continue;
+        uint16_t* o = out + (plane * img + (size_t)gy * g.width + gx) * kCh;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[r][ct][i];
+                if (MODE != kConvDgrad) v[i] += bias[ct * 16 + 4 * grp + i];
+                if (MODE == kConvRelu) v[i] = fmaxf(v[i], 0.0f);
+            }
+            *reinterpret_cast<uint2*>(o + ct * 16 + 4 * grp) =
+                make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: per-tile partial dW[tap][cout][cin] and db[cout]
+// ---------------------------------------------------------------------------
+template <int CIN>
+__global__ void __launch_bounds__(256) k_conv3x3_wgrad(CnnGeom g, const uint16_t* __restrict__ dsrc,
+                                                       const uint16_t* __restrict__ mask,
+                                                       const uint16_t* __restrict__ in, float* __restrict__ partial) {
+    constexpr int kPart = kTaps * kCh * CIN + kCh;
+    __shared__ __attribute__((aligned(16))) uint16_t dz[kCh][256];              // [cout][pixel]
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3][CIN][18][kTile];     // [dx][cin][row][col]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile, img = blockIdx.z;
+    const size_t plane = (size_t)g.width * g.height;
+    {
+        // upstream gradient of this tile, masked by the ReLU, transposed to channel-major
+        const int p = tid;  // one pixel per thread
+        const int gy = y0 + p / kTile, gx = x0 + p % kTile;
+        const bool inside = gx < g.width && gy < g.height;
+        const size_t off = (plane * img + (size_t)gy * g.width + gx) * kCh;
+#pragma unroll
+        for (int c = 0; c < kCh / 8; ++c) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (inside) {
+                v = *reinterpret_cast<const uint4*>(dsrc + off + c * 8);
+                if (mask) v = relu_mask8(v, *reinterpret_cast<const uint4*>(mask + off + c * 8));
+            }
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                dz[c * 8 + 2 * k][p] = (uint16_t)(w[k] & 0xFFFFu);
+                dz[c * 8 + 2 * k + 1][p] = (uint16_t)(w[k] >> 16);
+            }
+        }
+    }
+    for (int e = tid; e < 3 * 18 * kTile; e += 256) {
+        const int dx = e / (18 * kTile), rem = e % (18 * kTile);
+        const int row = rem / kTile, col = rem % kTile;
+        const int gy = y0 + row - 1, gx = x0 + col + dx - 1;
+        const bool inside = gx >= 0 && gy >= 0 && gx < g.width && gy < g.height;
+        const uint16_t* s = in + (plane * img + (size_t)gy * g.width + gx) * CIN;
+#pragma unroll
+        for (int c = 0; c < CIN / 8; ++c) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (inside) v = *reinterpret_cast<const uint4*>(s + c * 8);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xs[dx][c * 8 + 2 * k][row][col] = (uint16_t)(w[k] & 0xFFFFu);
+                xs[dx][c * 8 + 2 * k + 1][row][col] = (uint16_t)(w[k] >> 16);
+            }
+        }
+    }
+    __syncthreads();
+    float* part = partial + (((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * kPart;
+    const int j = lane & 15, grp = lane >> 4;
+    constexpr int kCiTiles = CIN / 16;
+    constexpr int kCombos = kTaps * 2 * kCiTiles;
+    for (int cb = wave; cb < kCombos; cb += 4) {
+        const int t = cb / (2 * kCiTiles), rest = cb % (2 * kCiTiles);
+        const int ct = rest / kCiTiles, it = rest % kCiTiles;
+        const int dy = t / 3, dx = t % 3;
+        float4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int p0 = ks * 32 + 8 * grp;  // eight pixels of one row
+            const int row = p0 / kTile, col = p0 % kTile;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&dz[ct * 16 + j][p0]));
+            const bf16x8 b =
+                __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&xs[dx][it * 16 + j][row + dy][col]));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+        // D[cout][cin]: lane holds cin it*16 + j, couts ct*16 + 4*grp + i
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[(t * kCh + ct * 16 + 4 * grp + i) * CIN + it * 16 + j] = acc[i];
+    }
+    if (tid < kCh) {
+        float s = 0.0f;
+        for (int p = 0; p < 256; ++p) s += bf2f(dz[tid][p]);
+        part[kTaps * kCh * CIN + tid] = s;
+    }
+}
+
+// grad[i] = sum over tiles of partial[tile][i], in tile order
+__global__ void __launch_bounds__(256) k_reduce_partials(const float* partial, int tiles, int size, float* grad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= size) return;
+    float s = 0.0f;
+    for (int t = 0; t < tiles; ++t) s += partial[(size_t)t * size + i];
+    grad[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel prediction: softmax over 25 logits, 5x5 weighted sum of the noisy
+// illumination (taps clamped to the image); loss and its logit gradient.
+// ---------------------------------------------------------------------------
+template <bool kTrain>
+__global__ void __launch_bounds__(256) k_kpn(CnnGeom g, pbrt_cnn_images im, const uint16_t* logits, uint16_t* dlogits,
+                                            float* loss_part) {
+    __shared__ float red[4];
+    const size_t plane = (size_t)g.width * g.height;
+    const size_t n = plane * g.count;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float lsum = 0.0f;
+    if (i < n) {
+        const int img = (int)(i / plane);
+        const int rem = (int)(i % plane);
+        const int y = rem / g.width, x = rem % g.width;
+        const uint16_t* lz = logits + i * kCh;
+        float z[kKernelTaps];
+        float zmax = -3.4e38f;
+#pragma unroll
+        for (int k = 0; k < kKernelTaps; ++k) {
+            z[k] = bf2f(lz[k]);
+            zmax = fmaxf(zmax, z[k]);
+        }
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kKernelTaps; ++k) {
+            z[k] = expf(z[k] - zmax);
+            s += z[k];
+        }
+        const float inv = 1.0f / s;
+        vec3 tap[kKernelTaps];
+        vec3 o = make_vec3(0.0f);
+#pragma unroll
+        for (int k = 0; k < kKernelTaps; ++k) {
+            z[k] *= inv;
+            const int yy = min(max(y + k / 5 - 2, 0), g.height - 1), xx = min(max(x + k % 5 - 2, 0), g.width - 1);
+            tap[k] = xyz(load_half4(im.noisy, plane * img + (size_t)yy * g.width + xx));
+            o += tap[k] * z[k];
+        }
+        if (!kTrain) {
+            if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(o.x, o.y, o.z, 1.0f);
+            if (im.final_image) {
+                const vec3 alb = xyz(load_rgba8(im.albedo, i)) + make_vec3(1e-6f);
+                vec3 t = alb * o;
+                t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
+                store_bgra8(im.final_image, i, make_vec4(vclamp(vpow(t, 0.454545f), 0.0f, 1.0f), 1.0f));
+            }
+        } else {
+            const float4 t4 = reinterpret_cast<const float4*>(im.target)[i];
+            const float t[3] = {t4.x, t4.y, t4.z}, ov[3] = {o.x, o.y, o.z};
+            const float norm = 1.0f / (float)(n * 3);
+            float gch[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float d = ov[c] - t[c], den = t[c] * t[c] + kRelEps;
+                lsum += d * d / den;
+                gch[c] = 2.0f * d / den * norm;
+            }
+            float dw[kKernelTaps], wdw = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kKernelTaps; ++k) {
+                dw[k] = gch[0] * tap[k].x + gch[1] * tap[k].y + gch[2] * tap[k].z;
+                wdw += z[k] * dw[k];
+            }
+            uint32_t packed[kCh / 2];
+#pragma unroll
+            for (int k = 0; k < kCh / 2; ++k) {
+                const float a = 2 * k < kKernelTaps ? z[2 * k] * (dw[2 * k] - wdw) : 0.0f;
+                const float b = 2 * k + 1 < kKernelTaps ? z[2 * k + 1] * (dw[2 * k + 1] - wdw) : 0.0f;
+                packed[k] = f2bf(a) | (f2bf(b) << 16);
+            }
+            uint4* d4 = reinterpret_cast<uint4*>(dlogits + i * kCh);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d4[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+        }
+    }
+    if (kTrain) {
+        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+__global__ void k_loss_total(const float* part, int blocks, float scale, float* loss) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.0f;
+    for (int b = 0; b < blocks; ++b) s += part[b];
+    *loss = s * scale;
+}
+
+// ---------------------------------------------------------------------------
+// Parameters: bf16 refresh, Adam, canonical gather
+// ---------------------------------------------------------------------------
+struct LayerTable {
+    int offset[kLayers + 1];
+    int woffset[kLayers + 1];
+    int cin[kLayers];
+};
+
+__device__ __forceinline__ int layer_of(const LayerTable& lt, int i) {
+    int l = 0;
+    while (l + 1 < kLayers && i >= lt.offset[l + 1]) ++l;
+    return l;
+}
+
+// writes forward bf16 W and the flipped / transposed backward copy for master slot i
+__device__ __forceinline__ void refresh_weight(const LayerTable& lt, int i, float p, uint16_t* wf, uint16_t* wb) {
+    const int l = layer_of(lt, i);
+    const int local = i - lt.offset[l];
+    const int ci_pad = lt.cin[l];
+    if (local >= kTaps * kCh * ci_pad) return;  // bias
+    const int t = local / (kCh * ci_pad), co = (local / ci_pad) % kCh, ci = local % ci_pad;
+    const uint16_t h = (uint16_t)f2bf(p);
+    wf[lt.woffset[l] + local] = h;
+    if (l > 0) wb[lt.woffset[l] + ((kTaps - 1 - t) * ci_pad + ci) * kCh + co] = h;
+}
+
+__global__ void __launch_bounds__(256) k_refresh(LayerTable lt, const float* master, uint16_t* wf, uint16_t* wb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kMasterSize) refresh_weight(lt, i, master[i], wf, wb);
+}
+
+__global__ void __launch_bounds__(256) k_adam(LayerTable lt, float* master, float* m, float* v, const float* grad,
+                                             float lr, float c1, float c2, uint16_t* wf, uint16_t* wb) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kMasterSize) return;
+    const float gi = grad[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float p = master[i] - lr * (mi * c1) / (sqrtf(vi * c2) + 1e-8f);
+    master[i] = p;
+    refresh_weight(lt, i, p, wf, wb);
+}
+
+__global__ void __launch_bounds__(256) k_gather(const float* src, const int* map, int n, float* dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+
+LayerTable layer_table() {
+    LayerTable lt{};
+    for (int l = 0; l <= kLayers; ++l) {
+        lt.offset[l] = layer_offset(l);
+        lt.woffset[l] = weight_offset(l);
+    }
+    for (int l = 0; l < kLayers; ++l) lt.cin[l] = cin_pad(l);
+    return lt;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace
+// ---------------------------------------------------------------------------
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct CnnWork {
+    uint16_t* feat;
+    uint16_t* act[kLayers + 1];  // act[0] = feat; act[1..5] hidden, act[6] logits
+    uint16_t* grad_act[2];       // training: upstream gradients (ping-pong)
+    float* partial;              // training: per-tile weight-gradient partials
+    float* loss_part;
+    float* grad;                 // training: padded gradient
+    size_t bytes;
+};
+
+CnnWork layout(uint32_t w, uint32_t h, uint32_t count, bool training, char* base) {
+    CnnWork k{};
+    const size_t px = (size_t)w * h * count;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off = align256(off + bytes);
+        return p;
+    };
+    k.feat = (uint16_t*)take(px * kFeatCh * 2);
+    k.act[0] = k.feat;
+    if (training) {
+        for (int l = 1; l <= kLayers; ++l) k.act[l] = (uint16_t*)take(px * kCh * 2);
+        k.grad_act[0] = (uint16_t*)take(px * kCh * 2);
+        k.grad_act[1] = (uint16_t*)take(px * kCh * 2);
+        const size_t tiles = (size_t)((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile) * count;
+        k.partial = (float*)take(tiles * (kTaps * kCh * kCh + kCh) * 4);
+        k.loss_part = (float*)take(((px + 255) / 256) * 4);
+        k.grad = (float*)take(kMasterSize * 4);
+    } else {
+        // ping-pong hidden activations; logits in the second buffer
+        uint16_t* a = (uint16_t*)take(px * kCh * 2);
+        uint16_t* b = (uint16_t*)take(px * kCh * 2);
+        for (int l = 1; l <= kLayers; ++l) k.act[l] = (l & 1) ? a : b;
+    }
+    k.bytes = off;
+    return k;
+}
+
+}  // namespace
+}  // namespace pbrt
+
+using namespace pbrt;
+
+struct pbrt_cnn {
+    DeviceBuffer master, m, v, wf, wb, grad_map;
+    uint64_t step = 0;
+};
+
+namespace pbrt {
+namespace {
+
+void check_images(const pbrt_cnn_images* im, bool training, const char* fn) {
+    PBRT_REQUIRE(im, std::string(fn) + ": NULL images");
+    PBRT_REQUIRE(im->width > 0 && im->height > 0 && im->count > 0, std::string(fn) + ": empty batch");
+    PBRT_REQUIRE(im->noisy && im->depth && im->normal && im->albedo && im->spp && im->motion,
+                 std::string(fn) + ": missing input image");
+    PBRT_REQUIRE(!training || im->target, std::string(fn) + ": missing target");
+    const size_t need = layout(im->width, im->height, im->count, training, nullptr).bytes;
+    PBRT_REQUIRE(im->workspace && im->workspace_bytes >= need, std::string(fn) + ": workspace too small");
+}
+
+void forward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, hipStream_t st) {
+    const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
+    const size_t px = (size_t)im.width * im.height * im.count;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, im, k.feat);
+    const dim3 grid((im.width + kTile - 1) / kTile, (im.height + kTile - 1) / kTile, im.count);
+    const uint16_t* wf = net->wf.as<uint16_t>();
+    const float* master = net->master.as<float>();
+    for (int l = 0; l < kLayers; ++l) {
+        const uint16_t* w = wf + weight_offset(l);
+        const float* b = master + layer_offset(l) + layer_weights(l);
+        if (l == 0)
+            hipLaunchKernelGGL((k_conv3x3<kFeatCh, kConvRelu>), grid, dim3(256), 0, st, g, k.act[0], nullptr, w, b,
+                               k.act[1]);
+        else if (l < kLayers - 1)
+            hipLaunchKernelGGL((k_conv3x3<kCh, kConvRelu>), grid, dim3(256), 0, st, g, k.act[l], nullptr, w, b,
+                               k.act[l + 1]);
+        else
+            hipLaunchKernelGGL((k_conv3x3<kCh, kConvLinear>), grid, dim3(256), 0, st, g, k.act[l], nullptr, w, b,
+                               k.act[l + 1]);
+    }
+}
+
+// loss and padded gradient of the batch into k.grad / loss (device)
+void backward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, float* loss, hipStream_t st) {
+    const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
+    const size_t px = (size_t)im.width * im.height * im.count;
+    const int blocks = (int)((px + 255) / 256);
+    hipLaunchKernelGGL(k_kpn<true>, dim3(blocks), dim3(256), 0, st, g, im, k.act[kLayers], k.grad_act[0],
+                       k.loss_part);
+    if (loss) hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, st, k.loss_part, blocks, 1.0f / (float)(px * 3), loss);
+    const dim3 grid((im.width + kTile - 1) / kTile, (im.height + kTile - 1) / kTile, im.count);
+    const int tiles = (int)(grid.x * grid.y * grid.z);
+    const uint16_t* wb = net->wb.as<uint16_t>();
+    int cur = 0;  // grad_act[cur] holds dA of layer l+1 (dLogits for the last layer)
+    for (int l = kLayers - 1; l >= 0; --l) {
+        const uint16_t* dsrc = k.grad_act[cur];
+        const uint16_t* mask = l == kLayers - 1 ? nullptr : k.act[l + 1];
+        const int size = layer_size(l);
+        if (l == 0)
+            hipLaunchKernelGGL(k_conv3x3_wgrad<kFeatCh>, grid, dim3(256), 0, st, g, dsrc, mask, k.act[0], k.partial);
+        else
+            hipLaunchKernelGGL(k_conv3x3_wgrad<kCh>, grid, dim3(256), 0, st, g, dsrc, mask, k.act[l], k.partial);
+        hipLaunchKernelGGL(k_reduce_partials, dim3((size + 255) / 256), dim3(256), 0, st, k.partial, tiles, size,
+                           k.grad + layer_offset(l));
+        if (l > 0) {
+            hipLaunchKernelGGL((k_conv3x3<kCh, kConvDgrad>), grid, dim3(256), 0, st, g, dsrc, mask,
+                               wb + weight_offset(l), nullptr, k.grad_act[cur ^ 1]);
+            cur ^= 1;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace pbrt
+
+extern "C" int32_t pbrt_cnn_parameter_count(uint32_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_cnn_parameter_count: NULL output");
+        *out = (uint32_t)canonical_count();
+    });
+}
+
+extern "C" int32_t pbrt_cnn_create(uint64_t seed, pbrt_cnn** out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_cnn_create: NULL output");
+        *out = nullptr;
+        auto net = std::make_unique<pbrt_cnn>();
+        // He-normal weights from splitmix64 + Box-Muller; last layer scaled by 0.1 so the
+        // initial kernel is close to a 5x5 box filter; zero biases
+        std::vector<float> canon(canonical_count());
+        uint64_t state = seed;
+        auto next = [&]() {
+            uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        auto uniform = [&]() { return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+        size_t o = 0;
+        for (int l = 0; l < kLayers; ++l) {
+            const double std = std::sqrt(2.0 / (kTaps * cin_real(l))) * (l == kLayers - 1 ? 0.1 : 1.0);
+            const int nw = cout_real(l) * cin_real(l) * kTaps;
+            for (int i = 0; i < nw; ++i) {
+                const double u1 = uniform(), u2 = uniform();
+                canon[o++] = (float)(std * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+            }
+            for (int i = 0; i < cout_real(l); ++i) canon[o++] = 0.0f;
+        }
+        const std::vector<int> map = canonical_map();
+        net->grad_map.upload(map.data(), map.size() * sizeof(int));
+        std::vector<float> zeros(kMasterSize, 0.0f);
+        net->master.upload(zeros.data(), zeros.size() * 4);
+        net->m.upload(zeros.data(), zeros.size() * 4);
+        net->v.upload(zeros.data(), zeros.size() * 4);
+        std::vector<uint16_t> zw(kWeightSize, 0);
+        net->wf.upload(zw.data(), zw.size() * 2);
+        net->wb.upload(zw.data(), zw.size() * 2);
+        *out = net.release();
+        int32_t rc = pbrt_cnn_set_parameters(*out, canon.data());
+        if (rc != PBRT_OK) {
+            delete *out;
+            *out = nullptr;
+            fail(rc, "pbrt_cnn_create: initial parameters");
+        }
+    });
+}
+
+extern "C" int32_t pbrt_cnn_destroy(pbrt_cnn* net) {
+    return guard([&] { delete net; });
+}
+
+extern "C" int32_t pbrt_cnn_get_parameters(const pbrt_cnn* net, float* host) {
+    return guard([&] {
+        PBRT_REQUIRE(net && host, "pbrt_cnn_get_parameters: NULL argument");
+        std::vector<float> padded(kMasterSize);
+        PBRT_HIP(hipMemcpy(padded.data(), net->master.ptr, kMasterSize * 4, hipMemcpyDeviceToHost));
+        const std::vector<int> map = canonical_map();
+        for (size_t i = 0; i < map.size(); ++i) host[i] = padded[map[i]];
+    });
+}
+
+extern "C" int32_t pbrt_cnn_set_parameters(pbrt_cnn* net, const float* host) {
+    return guard([&] {
+        PBRT_REQUIRE(net && host, "pbrt_cnn_set_parameters: NULL argument");
+        std::vector<float> padded(kMasterSize, 0.0f);
+        const std::vector<int> map = canonical_map();
+        for (size_t i = 0; i < map.size(); ++i) padded[map[i]] = host[i];
+        PBRT_HIP(hipMemcpy(net->master.ptr, padded.data(), kMasterSize * 4, hipMemcpyHostToDevice));
+        PBRT_HIP(hipMemset(net->m.ptr, 0, kMasterSize * 4));
+        PBRT_HIP(hipMemset(net->v.ptr, 0, kMasterSize * 4));
+        net->step = 0;
+        hipLaunchKernelGGL(k_refresh, dim3((kMasterSize + 255) / 256), dim3(256), 0, 0, layer_table(),
+                           net->master.as<float>(), net->wf.as<uint16_t>(), net->wb.as<uint16_t>());
+        PBRT_HIP(hipGetLastError());
+        PBRT_HIP(hipDeviceSynchronize());
+    });
+}
+
+extern "C" int32_t pbrt_cnn_workspace_size(uint32_t w, uint32_t h, uint32_t count, uint32_t training, size_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_cnn_workspace_size: NULL output");
+        PBRT_REQUIRE(w > 0 && h > 0 && count > 0, "pbrt_cnn_workspace_size: empty batch");
+        *out = layout(w, h, count, training != 0, nullptr).bytes;
+    });
+}
+
+extern "C" int32_t pbrt_cnn_denoise(const pbrt_cnn* net, const pbrt_cnn_images* im, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(net, "pbrt_cnn_denoise: NULL network");
+        check_images(im, false, "pbrt_cnn_denoise");
+        hipStream_t st = (hipStream_t)stream;
+        const CnnWork k = layout(im->width, im->height, im->count, false, static_cast<char*>(im->workspace));
+        forward(net, *im, k, st);
+        const CnnGeom g{(int)im->width, (int)im->height, (int)im->count};
+        const size_t px = (size_t)im->width * im->height * im->count;
+        hipLaunchKernelGGL(k_kpn<false>, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, *im,
+                           k.act[kLayers], nullptr, nullptr);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_cnn_gradients(const pbrt_cnn* net, const pbrt_cnn_images* im, float* loss, float* grad,
+                                      void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(net && grad, "pbrt_cnn_gradients: NULL argument");
+        check_images(im, true, "pbrt_cnn_gradients");
+        hipStream_t st = (hipStream_t)stream;
+        const CnnWork k = layout(im->width, im->height, im->count, true, static_cast<char*>(im->workspace));
+        forward(net, *im, k, st);
+        backward(net, *im, k, loss, st);
+        const int n = canonical_count();
+        hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, k.grad, net->grad_map.as<int>(), n,
+                           grad);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_cnn_train_step(pbrt_cnn* net, const pbrt_cnn_images* im, float lr, float* loss,
+                                       void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(net, "pbrt_cnn_train_step: NULL network");
+        PBRT_REQUIRE(lr > 0.0f && std::isfinite(lr), "pbrt_cnn_train_step: learning rate must be positive");
+        check_images(im, true, "pbrt_cnn_train_step");
+        hipStream_t st = (hipStream_t)stream;
+        const CnnWork k = layout(im->width, im->height, im->count, true, static_cast<char*>(im->workspace));
+        forward(net, *im, k, st);
+        backward(net, *im, k, loss, st);
+        net->step += 1;
+        const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)net->step));
+        const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)net->step));
+        hipLaunchKernelGGL(k_adam, dim3((kMasterSize + 255) / 256), dim3(256), 0, st, layer_table(),
+                           net->master.as<float>(), net->m.as<float>(), net->v.as<float>(), k.grad, lr, c1, c2,
+                           net->wf.as<uint16_t>(), net->wb.as<uint16_t>());
+        PBRT_HIP(hipGetLastError());
+    });
+}
