@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
+    ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn"],
+                    help="denoiser of the timed frame: BMFR 32x32 or the learned kernel-predicting network")
+    ap.add_argument("--no-denoise-eval", action="store_true",
+                    help="skip the learned-vs-BMFR relMSE evaluation on held-out Cornell-box scenes")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -78,8 +82,9 @@ WORKLOADS = {
 class Frame:
     """Device images and the per-frame launch sequence (all on torch's current stream)."""
 
-    def __init__(self, torch, capi, camera, gen, scene, w, h, depth, tracer, bmfr_fit):
+    def __init__(self, torch, capi, camera, gen, scene, w, h, depth, tracer, bmfr_fit, denoiser="bmfr"):
         self.torch, self.capi, self.camera = torch, capi, camera
+        self.denoiser = denoiser
         self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
         self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
         self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
@@ -110,6 +115,20 @@ class Frame:
         capi.check(capi.lib().pbrt_trace_workspace_size(w, h, C.byref(tn)))
         self.trace_ws = t.zeros(tn.value, dtype=t.uint8, device=d)
         self.prev_view = None
+        self.net = None
+        if denoiser == "cnn":
+            self.ensure_cnn()
+
+    def ensure_cnn(self):
+        """The learned denoiser with the committed trained parameters."""
+        if self.net is not None:
+            return
+        from vulkanpbrt_amd.denoiser_train import DEFAULT_WEIGHTS, Network
+        self.net = Network()
+        self.net.load(DEFAULT_WEIGHTS)
+        cn = C.c_size_t()
+        self.capi.check(self.capi.lib().pbrt_cnn_workspace_size(self.w, self.h, 1, 0, C.byref(cn)))
+        self.cnn_ws = self.torch.zeros(cn.value, dtype=self.torch.uint8, device="cuda")
 
     def stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
@@ -159,6 +178,21 @@ class Frame:
                              capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
         capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), self.stream()))
 
+    def cnn(self, frame):
+        capi = self.capi
+        cur = frame & 1
+        im = capi.CnnImages(self.w, self.h, 1, capi.ptr(self.acc[cur]), capi.ptr(self.img["depth"]),
+                            capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]), capi.ptr(self.spp[cur]),
+                            capi.ptr(self.motion), None, capi.ptr(self.bmfr_final), None, capi.ptr(self.cnn_ws),
+                            self.cnn_ws.numel())
+        capi.check(capi.lib().pbrt_cnn_denoise(self.net.handle, C.byref(im), self.stream()))
+
+    def denoise(self, frame):
+        if self.denoiser == "cnn":
+            self.cnn(frame)
+        else:
+            self.bmfr(frame)
+
     def taa(self, frame):
         capi = self.capi
         im = capi.TaaImages(capi.ptr(self.motion), capi.ptr(self.bmfr_final), capi.ptr(self.taa_hist),
@@ -175,13 +209,41 @@ class Frame:
         self.accumulate(frame, view, proj)
         if events:
             events[2].record()
-        self.bmfr(frame)
+        self.denoise(frame)
         if events:
             events[3].record()
         self.taa(frame)
         if events:
             events[4].record()
         self.prev_view = view
+
+
+def denoise_comparison(torch, fr, frame, runs, evaluate):
+    """Both denoisers on the same 1080p frame (the last timed one), `runs` launches each,
+    timed with events on the launch stream; with `evaluate`, their relMSE against
+    256-spp references on 16 held-out Cornell-box scenes (128x128, first frame)."""
+    fr.ensure_cnn()
+    out = {}
+    for name, fn in (("bmfr", fr.bmfr), ("cnn", fr.cnn)):
+        fn(frame)  # warm-up
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(runs):
+            fn(frame)
+        b.record()
+        torch.cuda.synchronize()
+        out[f"{name}_ms"] = round(a.elapsed_time(b) / runs, 4)
+    out["cnn_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn_ms"], 3)
+    if evaluate:
+        from vulkanpbrt_amd import denoiser_train as dt
+        pairs = dt.render_pairs(torch, np.arange(90_000, 90_016), 128, 128, 256)
+        res = dt.evaluate(torch, fr.net, pairs)
+        out.update({k: round(v, 5) for k, v in res.items()})
+        out["eval_set"] = ("16 held-out seed-generated Cornell-box scenes (seeds 90000-90015), 128x128, 1-spp "
+                           "accumulator input, 256-spp references; relMSE of the remodulated radiance "
+                           "(and of the demodulated radiance)")
+    return out
 
 
 def cpu_baseline(gen, cfg, budget_s, frames):
@@ -231,7 +293,7 @@ def main():
     gen = capi.GeneratedScene(args.config, cfg["seed"])
     scene = capi.Scene(gen.desc)
     info = scene.info()
-    fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit)
+    fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit, args.denoiser)
     setup_s = time.perf_counter() - t_setup
 
     for f in range(args.warmup):
@@ -313,9 +375,9 @@ def main():
         "data": "synthetic (seed-generated scene, random-free camera spline)",
         "config": {"workload": cfg["name"], "width": w, "height": h, "spp": 1, "max_depth": cfg["depth"],
                    "triangles": info.triangle_count, "lights": info.light_count, "tracer": args.tracer,
-                   "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"]},
+                   "denoiser": args.denoiser, "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"]},
         "stage_ms": {"trace": round(stage_ms[0], 3), "accumulate": round(stage_ms[1], 3),
-                     "bmfr": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
+                     "denoise": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
         "rays_per_frame": int(rays_per_frame),
         "trace_kernels": kernels,
         "nominal_mrays_reference_formula": round(nominal, 2),
@@ -327,6 +389,8 @@ def main():
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
     }
+    out["denoise_comparison"] = denoise_comparison(torch, fr, args.warmup + K - 1, max(K, 10),
+                                                   not args.no_denoise_eval)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds, range(args.warmup, args.warmup + K))
     print(json.dumps(out))

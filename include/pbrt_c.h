@@ -316,6 +316,11 @@ int32_t pbrt_taa(uint32_t width, uint32_t height, uint32_t frame_number, const p
  * (source/renderModules/FormatConverter.cpp:4-82): RGBA32F -> B8G8R8A8. */
 int32_t pbrt_convert(uint32_t width, uint32_t height, const float* source, uint8_t* bgra8, void* stream);
 
+/* Running mean of RGBA32F frames: dst += (src - dst) / (n + 1) with dst holding the
+ * mean of n earlier frames (n = 0 copies).  No counterpart in the reference; it
+ * builds the many-sample demodulated references the learned denoiser trains on. */
+int32_t pbrt_running_mean(uint32_t width, uint32_t height, const float* src, float* dst, uint32_t n, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* BMFR denoiser: replaces bmfrPre/bmfrFit/bmfrPost.comp (shaders/bmfrPre.comp */
 /* :5-98, bmfrFit.comp:5-92, bmfrPost.comp:5-124) dispatched by               */
@@ -444,7 +449,8 @@ int32_t pbrt_bfr_blend(uint32_t width, uint32_t height, uint32_t filter_radius,
 /*   6 x conv3x3 (12->32->32->32->32->32, ReLU, ->25 logits), BF16 weights and */
 /*   activations, FP32 accumulation on the matrix cores; softmax = per-pixel   */
 /*   5x5 kernel (taps clamped to the image) applied to the noisy illumination. */
-/*   loss: mean (out - target)^2 / (target^2 + 0.01) over pixels and rgb.      */
+/*   loss: relative squared error of the remodulated radiance, mean over pixels */
+/*   and rgb of (a (out - target))^2 / ((a target)^2 + 0.01), a = albedo + 1e-6.*/
 /* Parameters in canonical order, per layer W[cout][cin][ky][kx] then b[cout]. */
 /* ------------------------------------------------------------------------- */
 

@@ -12,7 +12,9 @@ definition, in numpy, independently of the HIP kernels:
   softmax over the 25 logits = a 5x5 kernel (tap k = ky * 5 + kx, offset
   (kx - 2, ky - 2), taps clamped to the image edge) applied to the noisy
   demodulated illumination
-  loss = mean over pixels and rgb of (out - target)^2 / (target^2 + 0.01)
+  loss = mean over pixels and rgb of (a (out - target))^2 / ((a target)^2 + 0.01),
+  a = albedo + 1e-6 (the remodulation of bmfrPost.comp): the relative squared error
+  of the remodulated radiance
 
 Parameters in canonical order, layer by layer: W[cout][cin][ky][kx], then b[cout].
 `bf16=True` mirrors the device storage: weights, features, activations and
@@ -141,14 +143,21 @@ def forward(params, feats, noisy_rgb, bf16_mirror=True):
     return out, dict(acts=acts, wk=wk, taps=taps, layers=layers, rnd=rnd)
 
 
-def loss(out, target):
-    return float(np.mean((out - target) ** 2 / (target ** 2 + EPS_REL)))
+def remod(albedo_u8):
+    return albedo_u8[..., :3].astype(np.float64) / 255.0 + 1e-6
 
 
-def backward(cache, out, target):
+def loss(out, target, albedo):
+    """albedo: (N,H,W,>=3) uint8 G-buffer albedo."""
+    a = remod(albedo)
+    return float(np.mean((a * (out - target)) ** 2 / ((a * target) ** 2 + EPS_REL)))
+
+
+def backward(cache, out, target, albedo):
     """Gradient of loss() wrt the canonical parameters."""
     acts, wk, taps, layers, rnd = cache["acts"], cache["wk"], cache["taps"], cache["layers"], cache["rnd"]
-    g = 2.0 * (out - target) / (target ** 2 + EPS_REL) / out.size
+    a = remod(albedo)
+    g = 2.0 * a * a * (out - target) / ((a * target) ** 2 + EPS_REL) / out.size
     dwk = np.einsum("nhwc,nhwkc->nhwk", g, taps)
     dz = wk * (dwk - (wk * dwk).sum(-1, keepdims=True))
     grads = [None] * LAYERS

@@ -24,7 +24,8 @@ def run_app(tmp_path, tag, *args):
 
 
 @pytest.mark.parametrize("denoiser,block", [("bmfr", "32"), ("bmfr", "8"), ("bfr", "8"), ("bfr", "16"),
-                                            ("bfr", "32"), ("bfr", "blend"), ("bmfr", "blend"), ("none", "32")])
+                                            ("bfr", "32"), ("bfr", "blend"), ("bmfr", "blend"), ("learned", "32"),
+                                            ("none", "32")])
 def test_app_eager_equals_graph(tmp_path, denoiser, block):
     assert APP.exists(), "build the library first (python -m vulkanpbrt_amd.build)"
     args = ["--denoiser", denoiser, "--block", block, "--taa"] if denoiser != "none" else ["--denoiser", "none"]

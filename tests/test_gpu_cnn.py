@@ -123,8 +123,8 @@ def test_gradients_match_oracle(cuda):
     g = grad.cpu().numpy().astype(np.float64)
     out, cache = oracle_forward(params, inp)
     tgt = inp["target"][..., :3].astype(np.float64)
-    ref_loss = cnn.loss(out, tgt)
-    ref = cnn.backward(cache, out, tgt)
+    ref_loss = cnn.loss(out, tgt, inp["albedo"])
+    ref = cnn.backward(cache, out, tgt, inp["albedo"])
     print(f"cnn loss gpu {loss.item():.6f} oracle {ref_loss:.6f}")
     assert abs(loss.item() - ref_loss) <= 2e-3 * ref_loss
     o = 0

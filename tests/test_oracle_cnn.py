@@ -56,8 +56,9 @@ def test_gradient_matches_finite_differences():
     feats = rng.normal(size=(1, 5, 6, 12))
     noisy = rng.uniform(0, 2, size=(1, 5, 6, 3))
     target = rng.uniform(0, 2, size=(1, 5, 6, 3))
+    albedo = rng.integers(0, 256, size=(1, 5, 6, 4), dtype=np.uint8)
     out, cache = cnn.forward(params, feats, noisy, bf16_mirror=False)
-    g = cnn.backward(cache, out, target)
+    g = cnn.backward(cache, out, target, albedo)
     # probe a handful of entries in every layer's weights and bias
     idx, o = [], 0
     for ws, bs in cnn.layer_shapes():
@@ -69,8 +70,8 @@ def test_gradient_matches_finite_differences():
         p1, p2 = params.copy(), params.copy()
         p1[i] += h
         p2[i] -= h
-        l1 = cnn.loss(cnn.forward(p1, feats, noisy, bf16_mirror=False)[0], target)
-        l2 = cnn.loss(cnn.forward(p2, feats, noisy, bf16_mirror=False)[0], target)
+        l1 = cnn.loss(cnn.forward(p1, feats, noisy, bf16_mirror=False)[0], target, albedo)
+        l2 = cnn.loss(cnn.forward(p2, feats, noisy, bf16_mirror=False)[0], target, albedo)
         fd = (l1 - l2) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
 

@@ -4,7 +4,7 @@
 // runs it per frame with updated push constants and camera matrices.
 //
 //   pbrt_frame [--config 1|2] [--seed S] [--width W --height H] [--frames F]
-//              [--depth D] [--denoiser bmfr|bfr|none] [--block 8|16|32|blend] [--taa]
+//              [--depth D] [--denoiser bmfr|bfr|learned|none] [--block 8|16|32|blend] [--taa]
 //              [--graph] [--out final.ppm] [--raw image.bin]
 #include <hip/hip_runtime.h>
 
@@ -62,8 +62,10 @@ int main(int argc, char** argv) {
         auto push_constants = std::make_shared<PushConstants<RayTracingPushConstants>>();
         GBufferPtr g_buffer;
         IlluminationBufferPtr illumination_buffer;
-        bool denoise = denoiser == "bmfr" || denoiser == "bfr";
-        const DenoisingType dtype = denoiser == "bfr" ? DenoisingType::BFR : DenoisingType::BMFR;
+        bool denoise = denoiser == "bmfr" || denoiser == "bfr" || denoiser == "learned";
+        const DenoisingType dtype = denoiser == "bfr"       ? DenoisingType::BFR
+                                    : denoiser == "learned" ? DenoisingType::LEARNED
+                                                            : DenoisingType::BMFR;
         const DenoisingBlockSize dsize = block == "8"    ? DenoisingBlockSize::X8
                                          : block == "16" ? DenoisingBlockSize::X16
                                          : block == "blend" ? DenoisingBlockSize::X8X16X32

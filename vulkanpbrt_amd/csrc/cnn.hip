@@ -353,15 +353,18 @@ __global__ void __launch_bounds__(256) k_kpn(CnnGeom g, pbrt_cnn_images im, cons
                 store_bgra8(im.final_image, i, make_vec4(vclamp(vpow(t, 0.454545f), 0.0f, 1.0f), 1.0f));
             }
         } else {
+            // relative squared error of the remodulated radiance (albedo + 1e-6 as in bmfrPost)
             const float4 t4 = reinterpret_cast<const float4*>(im.target)[i];
-            const float t[3] = {t4.x, t4.y, t4.z}, ov[3] = {o.x, o.y, o.z};
+            const vec3 alb = xyz(load_rgba8(im.albedo, i)) + make_vec3(1e-6f);
+            const float t[3] = {t4.x, t4.y, t4.z}, ov[3] = {o.x, o.y, o.z}, a[3] = {alb.x, alb.y, alb.z};
             const float norm = 1.0f / (float)(n * 3);
             float gch[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float d = ov[c] - t[c], den = t[c] * t[c] + kRelEps;
+                const float at = a[c] * t[c];
+                const float d = a[c] * ov[c] - at, den = at * at + kRelEps;
                 lsum += d * d / den;
-                gch[c] = 2.0f * d / den * norm;
+                gch[c] = 2.0f * a[c] * d / den * norm;
             }
             float dw[kKernelTaps], wdw = 0.0f;
 #pragma unroll

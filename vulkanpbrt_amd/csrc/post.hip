@@ -265,3 +265,26 @@ extern "C" int32_t pbrt_convert(uint32_t width, uint32_t height, const float* so
         PBRT_HIP(hipGetLastError());
     });
 }
+
+namespace pbrt {
+__global__ void k_running_mean(size_t n4, const float4* src, float4* dst, float inv) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    const float4 s = src[i], d = dst[i];
+    dst[i] = make_float4(d.x + (s.x - d.x) * inv, d.y + (s.y - d.y) * inv, d.z + (s.z - d.z) * inv,
+                         d.w + (s.w - d.w) * inv);
+}
+}  // namespace pbrt
+
+extern "C" int32_t pbrt_running_mean(uint32_t width, uint32_t height, const float* src, float* dst, uint32_t n,
+                                     void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(src && dst, "pbrt_running_mean: NULL image");
+        const size_t n4 = (size_t)width * height;
+        if (n4 == 0) return;
+        hipLaunchKernelGGL(k_running_mean, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n4,
+                           reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst),
+                           1.0f / (float)((double)n + 1.0));
+        PBRT_HIP(hipGetLastError());
+    });
+}

@@ -1,9 +1,13 @@
 // Render modules and frame buffers on top of the C ABI (see RenderModules.hpp).
 #include "RenderModules.hpp"
 
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <iostream>
+#include <vector>
 
 namespace vkpbrt {
 
@@ -319,6 +323,73 @@ void BFRBlender::add_dispatch_to_command_graph(const CommandsPtr& command_graph)
 }
 
 // ---------------------------------------------------------------------------
+// LearnedDenoiser
+// ---------------------------------------------------------------------------
+LearnedDenoiser::LearnedDenoiser(uint32_t width, uint32_t height, GBufferPtr g_buffer,
+                                 IlluminationBufferPtr illu_buffer, AccumulationBufferPtr acc_buffer,
+                                 std::string weights_path)
+    : _width(width), _height(height), _g_buffer(g_buffer), _illumination(illu_buffer), _acc(acc_buffer),
+      _weights_path(std::move(weights_path)) {
+    if (illu_buffer->kind != IlluminationKind::Demodulated)
+        throw Exception(PBRT_ERR_UNSUPPORTED,
+                        "Illumination Buffer type is required to be IlluminationBufferDemodulated for the learned "
+                        "denoiser");
+    _final_illumination = std::make_shared<DeviceImage>(width, height, Format::BGRA8);
+}
+
+LearnedDenoiser::~LearnedDenoiser() {
+    if (_net) (void)pbrt_cnn_destroy(_net);
+}
+
+std::string LearnedDenoiser::default_weights_path() {
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void*>(&pbrt_cnn_create), &info) || !info.dli_fname)
+        throw Exception(PBRT_ERR_INTERNAL, "LearnedDenoiser: cannot locate libpbrt.so");
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.find_last_of('/');
+    return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/weights/kpn_cornell.bin";
+}
+
+void LearnedDenoiser::compile(Context&) {
+    _final_illumination->compile();
+    uint32_t count = 0;
+    check(pbrt_cnn_parameter_count(&count));
+    std::ifstream f(_weights_path, std::ios::binary);
+    if (!f) throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: cannot open " + _weights_path);
+    std::vector<float> params(count);
+    f.read(reinterpret_cast<char*>(params.data()), (std::streamsize)(count * sizeof(float)));
+    if (f.gcount() != (std::streamsize)(count * sizeof(float)) || f.peek() != EOF)
+        throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: " + _weights_path + " does not hold " +
+                                                  std::to_string(count) + " float32 parameters");
+    if (!_net) check(pbrt_cnn_create(1, &_net));
+    check(pbrt_cnn_set_parameters(_net, params.data()));
+    size_t bytes = 0;
+    check(pbrt_cnn_workspace_size(_width, _height, 1, 0, &bytes));
+    _workspace = std::make_unique<DeviceBuffer>(bytes);
+    _workspace->compile();
+}
+
+void LearnedDenoiser::add_dispatch_to_command_graph(const CommandsPtr& command_graph) {
+    if (!_workspace) throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: compile() was not called");
+    command_graph->addChild([this](hipStream_t s) {
+        pbrt_cnn_images im{};
+        im.width = _width;
+        im.height = _height;
+        im.count = 1;
+        im.noisy = _illumination->illumination_images[0]->as<uint16_t>();
+        im.depth = _g_buffer->depth->as<float>();
+        im.normal = _g_buffer->normal->as<float>();
+        im.albedo = _g_buffer->albedo->as<uint8_t>();
+        im.spp = _acc->spp->as<uint8_t>();
+        im.motion = _acc->motion->as<uint16_t>();
+        im.final_image = _final_illumination->as<uint8_t>();
+        im.workspace = _workspace->data();
+        im.workspace_bytes = _workspace->bytes();
+        check(pbrt_cnn_denoise(_net, &im, s));
+    });
+}
+
+// ---------------------------------------------------------------------------
 // Taa
 // ---------------------------------------------------------------------------
 Taa::Taa(uint32_t width, uint32_t height, uint32_t, uint32_t, GBufferPtr, AccumulationBufferPtr acc_buffer,
@@ -414,6 +485,15 @@ void add_denoiser_to_commands(DenoisingType denoising_type, DenoisingBlockSize d
                 default:
                     throw Exception(PBRT_ERR_UNSUPPORTED, "add_denoiser_to_commands: block size not supported");
             }
+            break;
+        }
+        case DenoisingType::LEARNED: {
+            auto d = LearnedDenoiser::create(width, height, g_buffer, illumination_buffer, accumulation_buffer);
+            d->compile(compile);
+            d->update_image_layouts(compile);
+            d->add_dispatch_to_command_graph(commands);
+            commands->keep(d);
+            final_descriptor_image = d->get_final_descriptor_image();
             break;
         }
         case DenoisingType::SVG:

@@ -19,6 +19,7 @@
 #include <array>
 #include <memory>
 #include <optional>
+#include <string>
 
 #include "Buffers.hpp"
 #include "Core.hpp"
@@ -35,7 +36,9 @@ struct RayTracingPushConstants {
 };
 using PushConstantsPtr = std::shared_ptr<PushConstants<RayTracingPushConstants>>;
 
-enum class DenoisingType { NONE, BMFR, BFR, SVG };
+// LEARNED (the kernel-predicting network of pbrt_cnn_*) has no counterpart in the
+// reference's DenoiserUtils.hpp; it reads and writes the same images as BMFR.
+enum class DenoisingType { NONE, BMFR, BFR, SVG, LEARNED };
 enum class DenoisingBlockSize { X8, X16, X32, X64, X8X16X32 };
 enum class RayTracingRayOrigin { CAMERA, GBUFFER };
 
@@ -162,6 +165,37 @@ public:
 private:
     uint32_t _width, _height, _filter_radius;
     DeviceImagePtr _average, _average_squared, _den8, _den16, _den32, _final;
+};
+
+// Learned kernel-predicting denoiser (pbrt_cnn_denoise): trained parameters are
+// loaded from a raw little-endian float32 file in the network's canonical order.
+class LearnedDenoiser {
+public:
+    LearnedDenoiser(uint32_t width, uint32_t height, GBufferPtr g_buffer, IlluminationBufferPtr illu_buffer,
+                    AccumulationBufferPtr acc_buffer, std::string weights_path = default_weights_path());
+    ~LearnedDenoiser();
+    LearnedDenoiser(const LearnedDenoiser&) = delete;
+    LearnedDenoiser& operator=(const LearnedDenoiser&) = delete;
+    static std::shared_ptr<LearnedDenoiser> create(uint32_t w, uint32_t h, GBufferPtr g, IlluminationBufferPtr i,
+                                                   AccumulationBufferPtr a) {
+        return std::make_shared<LearnedDenoiser>(w, h, g, i, a);
+    }
+    // weights/kpn_cornell.bin beside libpbrt.so
+    static std::string default_weights_path();
+    void compile(Context& context);
+    void update_image_layouts(Context&) {}
+    void add_dispatch_to_command_graph(const CommandsPtr& command_graph);
+    DeviceImagePtr get_final_descriptor_image() const { return _final_illumination; }
+
+private:
+    uint32_t _width, _height;
+    GBufferPtr _g_buffer;
+    IlluminationBufferPtr _illumination;
+    AccumulationBufferPtr _acc;
+    std::string _weights_path;
+    pbrt_cnn* _net = nullptr;
+    DeviceImagePtr _final_illumination;  // BGRA8
+    std::unique_ptr<DeviceBuffer> _workspace;
 };
 
 class Taa {
