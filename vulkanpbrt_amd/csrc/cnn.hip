@@ -22,6 +22,7 @@
 //                 per-tile partials are summed in a fixed order by k_reduce.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -98,10 +99,8 @@ struct CnnGeom {
 // ---------------------------------------------------------------------------
 // Features
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_features(CnnGeom g, pbrt_cnn_images im, uint16_t* feat) {
-    const size_t n = (size_t)g.width * g.height * g.count;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+// the 16 bf16 feature channels of pixel i (12 used), packed in two 16-byte words
+__device__ __forceinline__ void pixel_features(const pbrt_cnn_images& im, size_t i, uint4& lo, uint4& hi) {
     const vec3 nz = xyz(load_half4(im.noisy, i));
     const vec4 alb = load_rgba8(im.albedo, i);
     const float2 sn = reinterpret_cast<const float2*>(im.normal)[i];
@@ -115,101 +114,243 @@ __global__ void __launch_bounds__(256) k_features(CnnGeom g, pbrt_cnn_images im,
     uint32_t w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) w[k] = f2bf(f[2 * k]) | (f2bf(f[2 * k + 1]) << 16);
+    lo = make_uint4(w[0], w[1], w[2], w[3]);
+    hi = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+__global__ void __launch_bounds__(256) k_features(CnnGeom g, pbrt_cnn_images im, uint16_t* feat) {
+    const size_t n = (size_t)g.width * g.height * g.count;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
     uint4* o = reinterpret_cast<uint4*>(feat + i * kFeatCh);
-    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    pixel_features(im, i, o[0], o[1]);
 }
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution (forward, or input gradient with flipped weights)
 // ---------------------------------------------------------------------------
-enum ConvMode { kConvRelu = 0, kConvLinear = 1, kConvDgrad = 2 };
+// kConvKpn: last layer of inference; the logits stay on chip (LDS) and the softmax
+// 5x5 kernel is applied to the noisy taps staged with the tile.
+enum ConvMode { kConvRelu = 0, kConvLinear = 1, kConvDgrad = 2, kConvKpn = 4 };
 
+// Persistent: each workgroup keeps the layer's weights in registers and walks a
+// strided sequence of 16x16 tiles inside its XCD's contiguous share of the tile
+// range (neighbouring tiles, whose halos overlap, stay in one L2); the next tile's
+// halo is fetched into registers while the current one is computed and is stored
+// into the other LDS buffer afterwards.
 template <int CIN, int MODE>
 __global__ void __launch_bounds__(256) k_conv3x3(CnnGeom g, const uint16_t* __restrict__ in,
                                                  const uint16_t* __restrict__ mask, const uint16_t* __restrict__ wgt,
-                                                 const float* __restrict__ bias, uint16_t* __restrict__ out) {
+                                                 const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                                 pbrt_cnn_images im) {
     constexpr int kStride = CIN + 8;  // bf16 per LDS pixel: conflict-free 16 / 8 byte lane reads
     constexpr int kChunks = CIN / 8;
-    __shared__ __attribute__((aligned(16))) uint16_t tile[18 * 18 * kStride];
+    constexpr int kTilePix = 18 * 18;
+    constexpr int kLoads = (kTilePix * kChunks + 255) / 256;
+    constexpr int kNoisyPix = 20 * 20;  // kernel-prediction taps of a tile (kConvKpn)
+    constexpr int kNoisyLoads = MODE == kConvKpn ? (kNoisyPix + 255) / 256 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[2][kTilePix * kStride];
+    __shared__ __attribute__((aligned(16))) uint2 ntile[MODE == kConvKpn ? 2 : 1][MODE == kConvKpn ? kNoisyPix : 1];
+    __shared__ __attribute__((aligned(16))) uint16_t logit[MODE == kConvKpn ? kTile * kTile * kCh : 8];
+    uint2 nregs[kNoisyLoads];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile, img = blockIdx.z;
-    const size_t plane = (size_t)g.width * g.height;
-    const uint16_t* src = in + plane * img * CIN;
-    const uint16_t* msk = mask ? mask + plane * img * CIN : nullptr;
-    for (int e = tid; e < 18 * 18 * kChunks; e += 256) {
-        const int p = e / kChunks, c = e % kChunks;
-        const int gy = y0 + p / 18 - 1, gx = x0 + p % 18 - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) {
-            const size_t off = ((size_t)gy * g.width + gx) * CIN + c * 8;
-            v = *reinterpret_cast<const uint4*>(src + off);
-            if (MODE == kConvDgrad && msk) v = relu_mask8(v, *reinterpret_cast<const uint4*>(msk + off));
-        }
-        *reinterpret_cast<uint4*>(&tile[p * kStride + c * 8]) = v;
-    }
-    // weights (A operand): lane holds output channel ct*16 + lane%16
     const int j = lane & 15, grp = lane >> 4;
-    float4v acc[4][2];
+    const int tiles_x = (g.width + kTile - 1) / kTile, tiles_y = (g.height + kTile - 1) / kTile;
+    const int per_image = tiles_x * tiles_y, ntiles = per_image * g.count;
+    const int slots = gridDim.x / 8, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int chunk = (ntiles + 7) / 8, begin = xcd * chunk, end = min(ntiles, begin + chunk);
+    int t = begin + slot;
+    if (t >= end) return;
+    const size_t plane = (size_t)g.width * g.height;
+
+    auto fetch = [&](int tt, uint4 (&r)[kLoads]) {
+        const int img = tt / per_image, rem = tt % per_image;
+        const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+        if (MODE == kConvKpn) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
-    __syncthreads();
-    if (CIN == 32) {
-#pragma unroll
-        for (int t = 0; t < kTaps; ++t) {
-            const int dy = t / 3, dx = t % 3;
-            bf16x8 a[2];
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-                a[ct] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(wgt + (t * kCh + ct * 16 + j) * CIN + 8 * grp));
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int ty = wave * 4 + r;
-                const bf16x8 b = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(&tile[((ty + dy) * 18 + j + dx) * kStride + 8 * grp]));
-                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b, acc[r][0], 0, 0, 0);
-                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b, acc[r][1], 0, 0, 0);
+            for (int k = 0; k < kNoisyLoads; ++k) {
+                const int p = tid + k * 256;
+                if (p < kNoisyPix) {
+                    const int yy = min(max(y0 + p / 20 - 2, 0), g.height - 1);
+                    const int xx = min(max(x0 + p % 20 - 2, 0), g.width - 1);
+                    nregs[k] = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
+                }
             }
         }
-    } else {
+        const uint16_t* src = in + plane * img * CIN;
+        const uint16_t* msk = mask ? mask + plane * img * CIN : nullptr;
 #pragma unroll
-        for (int t = 0; t < kTaps; ++t) {
-            const int dy = t / 3, dx = t % 3;
-            short4v a[2];
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-                a[ct] = __builtin_bit_cast(
-                    short4v, *reinterpret_cast<const uint2*>(wgt + (t * kCh + ct * 16 + j) * CIN + 4 * grp));
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int ty = wave * 4 + r;
-                const short4v b = __builtin_bit_cast(
-                    short4v, *reinterpret_cast<const uint2*>(&tile[((ty + dy) * 18 + j + dx) * kStride + 4 * grp]));
-                acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b, acc[r][0], 0, 0, 0);
-                acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b, acc[r][1], 0, 0, 0);
+        for (int k = 0; k < kLoads; ++k) {
+            const int e = tid + k * 256;
+            r[k] = make_uint4(0, 0, 0, 0);
+            if (e < kTilePix * kChunks) {
+                const int p = e / kChunks, c = e % kChunks;
+                const int gy = y0 + p / 18 - 1, gx = x0 + p % 18 - 1;
+                if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) {
+                    const size_t off = ((size_t)gy * g.width + gx) * CIN + c * 8;
+                    r[k] = *reinterpret_cast<const uint4*>(src + off);
+                    if (MODE == kConvDgrad && msk) r[k] = relu_mask8(r[k], *reinterpret_cast<const uint4*>(msk + off));
+                }
             }
         }
-    }
-    // D[cout][pixel]: lane holds pixel j, output channels ct*16 + 4*grp + i
-    const int gx = x0 + j;
+    };
+    auto stash = [&](int b, const uint4 (&r)[kLoads]) {
+        if (MODE == kConvKpn) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int gy = y0 + wave * 4 + r;
-        if (gx >= g.width || gy >= g.height) continue;
-        uint16_t* o = out + (plane * img + (size_t)gy * g.width + gx) * kCh;
+            for (int k = 0; k < kNoisyLoads; ++k) {
+                const int p = tid + k * 256;
+                if (p < kNoisyPix) ntile[b][p] = nregs[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kLoads; ++k) {
+            const int e = tid + k * 256;
+            if (e < kTilePix * kChunks)
+                *reinterpret_cast<uint4*>(&tile[b][(e / kChunks) * kStride + (e % kChunks) * 8]) = r[k];
+        }
+    };
+
+    // weights (A operand, lane holds output channel ct*16 + j), resident for every tile
+    constexpr int kAVec = CIN == 32 ? 8 : 4;  // bf16 per lane per MFMA
+    uint32_t wreg[kTaps][2][kAVec / 2];
+#pragma unroll
+    for (int tp = 0; tp < kTaps; ++tp)
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                v[i] = acc[r][ct][i];
-                if (MODE != kConvDgrad) v[i] += bias[ct * 16 + 4 * grp + i];
-                if (MODE == kConvRelu) v[i] = fmaxf(v[i], 0.0f);
+            const uint16_t* w = wgt + (tp * kCh + ct * 16 + j) * CIN + kAVec * grp;
+            if (CIN == 32) {
+                const uint4 v = *reinterpret_cast<const uint4*>(w);
+                wreg[tp][ct][0] = v.x;
+                wreg[tp][ct][1] = v.y;
+                wreg[tp][ct][2 % (kAVec / 2)] = v.z;
+                wreg[tp][ct][3 % (kAVec / 2)] = v.w;
+            } else {
+                const uint2 v = *reinterpret_cast<const uint2*>(w);
+                wreg[tp][ct][0] = v.x;
+                wreg[tp][ct][1] = v.y;
             }
-            *reinterpret_cast<uint2*>(o + ct * 16 + 4 * grp) =
-                make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
         }
+    float bv[2][4];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bv[ct][i] = MODE != kConvDgrad ? bias[ct * 16 + 4 * grp + i] : 0.0f;
+
+    uint4 regs[kLoads];
+    fetch(t, regs);
+    stash(0, regs);
+    __syncthreads();
+    int buf = 0;
+    for (; t < end; t += slots) {
+        const int tn = t + slots;
+        if (tn < end) fetch(tn, regs);
+        float4v acc[4][2];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r][0] = acc[r][1] = (float4v){0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* tb = tile[buf];
+#pragma unroll
+        for (int tp = 0; tp < kTaps; ++tp) {
+            const int dy = tp / 3, dx = tp % 3;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ty = wave * 4 + r;
+                const uint16_t* bp = &tb[((ty + dy) * 18 + j + dx) * kStride + kAVec * grp];
+                if (CIN == 32) {
+                    const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(bp));
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const bf16x8 a = __builtin_bit_cast(
+                            bf16x8, make_uint4(wreg[tp][ct][0], wreg[tp][ct][1], wreg[tp][ct][2 % (kAVec / 2)],
+                                               wreg[tp][ct][3 % (kAVec / 2)]));
+                        acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[r][ct], 0, 0, 0);
+                    }
+                } else {
+                    const short4v b = __builtin_bit_cast(short4v, *reinterpret_cast<const uint2*>(bp));
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        const short4v a = __builtin_bit_cast(short4v, make_uint2(wreg[tp][ct][0], wreg[tp][ct][1]));
+                        acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[r][ct], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // D[cout][pixel]: lane holds pixel j, output channels ct*16 + 4*grp + i
+        const int img = t / per_image, rem = t % per_image;
+        const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+        const int gx = x0 + j;
+        if (MODE == kConvKpn) {
+            // bf16 logits of the tile to LDS, then one thread per pixel: softmax over the
+            // 25 logits and the 5x5 weighted sum of the staged noisy taps
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = (wave * 4 + r) * kTile + j;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) v[i] = acc[r][ct][i] + bv[ct][i];
+                    *reinterpret_cast<uint2*>(&logit[p * kCh + ct * 16 + 4 * grp]) =
+                        make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                }
+            }
+            __syncthreads();
+            const int py = tid / kTile, px = tid % kTile;
+            const int gy = y0 + py, gxp = x0 + px;
+            if (gxp < g.width && gy < g.height) {
+                float z[kKernelTaps];
+                float zmax = -3.4e38f;
+#pragma unroll
+                for (int c = 0; c < kKernelTaps; ++c) {
+                    z[c] = bf2f(logit[tid * kCh + c]);
+                    zmax = fmaxf(zmax, z[c]);
+                }
+                float sum = 0.0f;
+#pragma unroll
+                for (int c = 0; c < kKernelTaps; ++c) {
+                    z[c] = expf(z[c] - zmax);
+                    sum += z[c];
+                }
+                const float inv = 1.0f / sum;
+                vec3 o = make_vec3(0.0f);
+#pragma unroll
+                for (int c = 0; c < kKernelTaps; ++c) {
+                    // staged taps: tile-local row/col of the clamped image position
+                    const int yy = min(max(gy + c / 5 - 2, 0), g.height - 1) - (y0 - 2);
+                    const int xx = min(max(gxp + c % 5 - 2, 0), g.width - 1) - (x0 - 2);
+                    const uint2 hv = ntile[buf][yy * 20 + xx];
+                    o += make_vec3(h2f(hv.x & 0xFFFFu), h2f(hv.x >> 16), h2f(hv.y & 0xFFFFu)) * (z[c] * inv);
+                }
+                const size_t i = plane * img + (size_t)gy * g.width + gxp;
+                if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(o.x, o.y, o.z, 1.0f);
+                if (im.final_image) {
+                    const vec3 alb = xyz(load_rgba8(im.albedo, i)) + make_vec3(1e-6f);
+                    vec3 tm = alb * o;
+                    tm = make_vec3(fmaxf(tm.x, 0.0f), fmaxf(tm.y, 0.0f), fmaxf(tm.z, 0.0f));
+                    store_bgra8(im.final_image, i, make_vec4(vclamp(vpow(tm, 0.454545f), 0.0f, 1.0f), 1.0f));
+                }
+            }
+        } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gy = y0 + wave * 4 + r;
+            if (gx >= g.width || gy >= g.height) continue;
+            uint16_t* o = out + (plane * img + (size_t)gy * g.width + gx) * kCh;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    v[i] = acc[r][ct][i] + bv[ct][i];
+                    if (MODE == kConvRelu) v[i] = fmaxf(v[i], 0.0f);
+                }
+                *reinterpret_cast<uint2*>(o + ct * 16 + 4 * grp) =
+                    make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+            }
+        }
+        }
+        if (tn < end) stash(buf ^ 1, regs);
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -527,25 +668,65 @@ void check_images(const pbrt_cnn_images* im, bool training, const char* fn) {
     PBRT_REQUIRE(im->workspace && im->workspace_bytes >= need, std::string(fn) + ": workspace too small");
 }
 
-void forward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, hipStream_t st) {
+// Workgroups for a persistent convolution: every CU's resident share, a multiple of
+// the 8 XCDs, no more than the tiles.
+template <class K>
+dim3 persistent_grid(K kernel, int tiles) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        PBRT_HIP(hipGetDevice(&dev));
+        PBRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    int per_cu = 0;
+    PBRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    int blocks = std::max(1, cus * std::max(per_cu, 1));
+    blocks = std::min(blocks, ((tiles + 7) / 8) * 8);
+    return dim3((unsigned)(((blocks + 7) / 8) * 8));
+}
+
+template <int CIN, int MODE>
+void launch_conv(const CnnGeom& g, const uint16_t* in, const uint16_t* mask, const uint16_t* w, const float* b,
+                 uint16_t* out, hipStream_t st, const pbrt_cnn_images& im = pbrt_cnn_images{}) {
+    const int tiles = ((g.width + kTile - 1) / kTile) * ((g.height + kTile - 1) / kTile) * g.count;
+    const dim3 grid = persistent_grid(k_conv3x3<CIN, MODE>, tiles);
+    hipLaunchKernelGGL((k_conv3x3<CIN, MODE>), grid, dim3(256), 0, st, g, in, mask, w, b, out, im);
+}
+
+// inference: kernel prediction fused into the last layer
+void forward_fused(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, hipStream_t st) {
     const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
     const size_t px = (size_t)im.width * im.height * im.count;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, im, k.feat);
-    const dim3 grid((im.width + kTile - 1) / kTile, (im.height + kTile - 1) / kTile, im.count);
     const uint16_t* wf = net->wf.as<uint16_t>();
     const float* master = net->master.as<float>();
     for (int l = 0; l < kLayers; ++l) {
         const uint16_t* w = wf + weight_offset(l);
         const float* b = master + layer_offset(l) + layer_weights(l);
         if (l == 0)
-            hipLaunchKernelGGL((k_conv3x3<kFeatCh, kConvRelu>), grid, dim3(256), 0, st, g, k.act[0], nullptr, w, b,
-                               k.act[1]);
+            launch_conv<kFeatCh, kConvRelu>(g, k.act[0], nullptr, w, b, k.act[1], st);
         else if (l < kLayers - 1)
-            hipLaunchKernelGGL((k_conv3x3<kCh, kConvRelu>), grid, dim3(256), 0, st, g, k.act[l], nullptr, w, b,
-                               k.act[l + 1]);
+            launch_conv<kCh, kConvRelu>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
         else
-            hipLaunchKernelGGL((k_conv3x3<kCh, kConvLinear>), grid, dim3(256), 0, st, g, k.act[l], nullptr, w, b,
-                               k.act[l + 1]);
+            launch_conv<kCh, kConvKpn>(g, k.act[l], nullptr, w, b, nullptr, st, im);
+    }
+}
+
+void forward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, hipStream_t st) {
+    const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
+    const size_t px = (size_t)im.width * im.height * im.count;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, im, k.feat);
+    const uint16_t* wf = net->wf.as<uint16_t>();
+    const float* master = net->master.as<float>();
+    for (int l = 0; l < kLayers; ++l) {
+        const uint16_t* w = wf + weight_offset(l);
+        const float* b = master + layer_offset(l) + layer_weights(l);
+        if (l == 0)
+            launch_conv<kFeatCh, kConvRelu>(g, k.act[0], nullptr, w, b, k.act[1], st);
+        else if (l < kLayers - 1)
+            launch_conv<kCh, kConvRelu>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
+        else
+            launch_conv<kCh, kConvLinear>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
     }
 }
 
@@ -572,8 +753,7 @@ void backward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, 
         hipLaunchKernelGGL(k_reduce_partials, dim3((size + 255) / 256), dim3(256), 0, st, k.partial, tiles, size,
                            k.grad + layer_offset(l));
         if (l > 0) {
-            hipLaunchKernelGGL((k_conv3x3<kCh, kConvDgrad>), grid, dim3(256), 0, st, g, dsrc, mask,
-                               wb + weight_offset(l), nullptr, k.grad_act[cur ^ 1]);
+            launch_conv<kCh, kConvDgrad>(g, dsrc, mask, wb + weight_offset(l), nullptr, k.grad_act[cur ^ 1], st);
             cur ^= 1;
         }
     }
@@ -679,11 +859,7 @@ extern "C" int32_t pbrt_cnn_denoise(const pbrt_cnn* net, const pbrt_cnn_images* 
         check_images(im, false, "pbrt_cnn_denoise");
         hipStream_t st = (hipStream_t)stream;
         const CnnWork k = layout(im->width, im->height, im->count, false, static_cast<char*>(im->workspace));
-        forward(net, *im, k, st);
-        const CnnGeom g{(int)im->width, (int)im->height, (int)im->count};
-        const size_t px = (size_t)im->width * im->height * im->count;
-        hipLaunchKernelGGL(k_kpn<false>, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, *im,
-                           k.act[kLayers], nullptr, nullptr);
+        forward_fused(net, *im, k, st);
         PBRT_HIP(hipGetLastError());
     });
 }
