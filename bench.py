@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
+    ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh"],
+                    help="BVH builder: binned SAH on the host or the Morton-code LBVH on the GPU")
     ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn"],
                     help="denoiser of the timed frame: BMFR 32x32 or the learned kernel-predicting network")
     ap.add_argument("--no-denoise-eval", action="store_true",
@@ -291,6 +293,7 @@ def main():
     w, h = cfg["width"], cfg["height"]
     t_setup = time.perf_counter()
     gen = capi.GeneratedScene(args.config, cfg["seed"])
+    gen.desc.bvh_builder = capi.PBRT_BVH_LBVH_GPU if args.bvh == "lbvh" else capi.PBRT_BVH_SAH_HOST
     scene = capi.Scene(gen.desc)
     info = scene.info()
     fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit, args.denoiser)
@@ -381,7 +384,8 @@ def main():
         "rays_per_frame": int(rays_per_frame),
         "trace_kernels": kernels,
         "nominal_mrays_reference_formula": round(nominal, 2),
-        "bvh": {"nodes": info.bvh_node_count, "depth": info.bvh_depth,
+        "bvh": {"builder": args.bvh, "build_ms": round(info.bvh_build_ms, 2),
+                "nodes": info.bvh_node_count, "depth": info.bvh_depth,
                 "node_visits_per_closest_ray": round(cnt[2] / max(1.0, cnt[0]), 2),
                 "tri_tests_per_closest_ray": round(cnt[3] / max(1.0, cnt[0]), 2),
                 "node_visits_per_shadow_ray": round(cnt[4] / max(1.0, cnt[1]), 2),

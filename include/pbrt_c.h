@@ -137,6 +137,8 @@ typedef struct pbrt_scene_info {
     uint32_t bvh_depth;
     uint32_t light_sampling;   /* enum pbrt_light_sampling chosen at packing */
     uint64_t device_bytes;
+    uint32_t bvh_builder;      /* enum pbrt_bvh_builder that built the tree (LBVH needs >= 2 triangles) */
+    float bvh_build_ms;        /* host wall time (SAH) or device time (LBVH) of the build */
 } pbrt_scene_info;
 
 typedef struct pbrt_scene pbrt_scene;

@@ -106,6 +106,8 @@ class SceneInfo(C.Structure):
         ("bvh_depth", C.c_uint32),
         ("light_sampling", C.c_uint32),
         ("device_bytes", C.c_uint64),
+        ("bvh_builder", C.c_uint32),
+        ("bvh_build_ms", C.c_float),
     ]
 
 

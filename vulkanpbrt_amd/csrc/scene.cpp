@@ -17,12 +17,14 @@
 // buffers and object matrix exactly as ptClosesthit.rchit does.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <vector>
 
 #include "bvh_host.hpp"
+#include "lbvh.hpp"
 #include "capi_common.hpp"
 #include "scene_internal.hpp"
 
@@ -381,27 +383,42 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
                 lo[3 * (size_t)t + a] = std::fmin(v0, std::fmin(v1, v2));
                 hi[3 * (size_t)t + a] = std::fmax(v0, std::fmax(v1, v2));
             }
-        if (desc->bvh_builder == PBRT_BVH_LBVH_GPU)
-            fail(PBRT_ERR_UNSUPPORTED, "pbrt_scene_create: the GPU LBVH builder is not available in this build");
         // A traversal holds at most depth - 1 deferred children, so a tree of depth
         // <= kTraversalStackSize fits the per-thread stack.
-        s->bvh = build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
-        PBRT_REQUIRE(s->bvh.depth <= (uint32_t)kTraversalStackSize,
-                     "pbrt_scene_create: BVH deeper than the traversal stack");
-
-        // BVH-ordered triangle records
-        std::vector<float4> tri(3 * (size_t)ntri);
-        for (uint32_t k = 0; k < ntri; ++k) {
-            uint32_t t = s->bvh.tri_order[k];
-            const float* p = &ps.world_tris[9 * (size_t)t];
-            uint32_t inst = ps.tri_inst[t];
-            uint32_t flags = ps.instances[inst].flags;
-            tri[3 * (size_t)k + 0] = make_float4(p[0], p[1], p[2], uint_as_float(inst));
-            tri[3 * (size_t)k + 1] = make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], uint_as_float(ps.tri_prim[t]));
-            tri[3 * (size_t)k + 2] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], uint_as_float(flags));
+        const auto t0 = std::chrono::steady_clock::now();
+        uint32_t node_count = 0;
+        if (desc->bvh_builder == PBRT_BVH_LBVH_GPU && ntri >= 2) {
+            std::vector<uint32_t> flags(ntri);
+            for (uint32_t t = 0; t < ntri; ++t) flags[t] = ps.instances[ps.tri_inst[t]].flags;
+            LbvhResult r = build_lbvh(ps.world_tris, ps.tri_inst, ps.tri_prim, flags, kTraversalStackSize, s->tris,
+                                      s->nodes);
+            s->bvh.depth = r.depth;
+            node_count = r.node_count;
+            s->info.bvh_build_ms = r.build_ms;
+            s->info.bvh_builder = PBRT_BVH_LBVH_GPU;
+        } else {
+            s->bvh = build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
+            PBRT_REQUIRE(s->bvh.depth <= (uint32_t)kTraversalStackSize,
+                         "pbrt_scene_create: BVH deeper than the traversal stack");
+            // BVH-ordered triangle records
+            std::vector<float4> tri(3 * (size_t)ntri);
+            for (uint32_t k = 0; k < ntri; ++k) {
+                uint32_t t = s->bvh.tri_order[k];
+                const float* p = &ps.world_tris[9 * (size_t)t];
+                uint32_t inst = ps.tri_inst[t];
+                uint32_t flags = ps.instances[inst].flags;
+                tri[3 * (size_t)k + 0] = make_float4(p[0], p[1], p[2], uint_as_float(inst));
+                tri[3 * (size_t)k + 1] =
+                    make_float4(p[3] - p[0], p[4] - p[1], p[5] - p[2], uint_as_float(ps.tri_prim[t]));
+                tri[3 * (size_t)k + 2] = make_float4(p[6] - p[0], p[7] - p[1], p[8] - p[2], uint_as_float(flags));
+            }
+            s->tris.upload(tri.data(), tri.size() * sizeof(float4));
+            s->nodes.upload(s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(float));
+            node_count = (uint32_t)(s->bvh.nodes.size() / 16);
+            s->info.bvh_build_ms =
+                (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            s->info.bvh_builder = PBRT_BVH_SAH_HOST;
         }
-        s->tris.upload(tri.data(), tri.size() * sizeof(float4));
-        s->nodes.upload(s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(float));
         s->instances.upload(ps.instances.data(), ps.instances.size() * sizeof(DeviceInstance));
         s->meshes.upload(ps.meshes.data(), ps.meshes.size() * sizeof(DeviceMesh));
         s->materials.upload(ps.materials.data(), ps.materials.size() * sizeof(DeviceMaterial));
@@ -429,7 +446,7 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
         dv.light_count = (uint32_t)ps.lights.size();
         dv.light_strength_sum = ps.light_strength_sum;
         dv.triangle_count = ntri;
-        dv.node_count = (uint32_t)(s->bvh.nodes.size() / 16);
+        dv.node_count = node_count;
         dv.has_transmissive = 0;
         for (const DeviceMaterial& m : ps.materials)
             if ((int)m.transmittance_illum.w == 7) dv.has_transmissive = 1;
