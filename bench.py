@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2])
     ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the tracer launch shape (pbrt_autotune_trace) on the first frame before timing")
     ap.add_argument("--bvh", default="sah", choices=["sah", "lbvh"],
                     help="BVH builder: binned SAH on the host or the Morton-code LBVH on the GPU")
     ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn"],
@@ -89,6 +91,7 @@ class Frame:
         self.denoiser = denoiser
         self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
         self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+        self.block_threads, self.queue_paths = 0, 0
         self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
         d = "cuda"
         t = torch
@@ -141,13 +144,37 @@ class Frame:
         prev = self.prev_view if self.prev_view is not None else view
         return self.camera.push_constants(view, proj, prev, frame), view, proj
 
+    def trace_params(self):
+        return self.capi.TraceParams(self.w, self.h, self.depth, 0, self.capi.LIGHT_SAMPLE_AUTO,
+                                     self.capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, 1, self.tracer, self.block_threads,
+                                     self.queue_paths)
+
+    def trace_images(self):
+        capi = self.capi
+        return capi.TraceImages(*(capi.ptr(self.img[k]) for k in ("depth", "normal", "material", "albedo",
+                                                                   "illumination")),
+                                capi.ptr(self.trace_ws), self.trace_ws.numel())
+
+    def autotune(self, repeats=3):
+        """pbrt_autotune_trace on the first frame; adopts the fastest launch shape."""
+        capi = self.capi
+        pc, _, _ = self.push_constants(0)
+        tun = capi.TraceTuning()
+        capi.check(capi.lib().pbrt_autotune_trace(self.scene.handle, C.byref(self.trace_params()), C.byref(pc),
+                                                  C.byref(self.trace_images()), repeats, C.byref(tun), self.stream()))
+        self.tracer, self.block_threads, self.queue_paths = tun.best.tracer, tun.best.block_threads, \
+            tun.best.queue_paths
+        names = {capi.TRACER_MEGAKERNEL: "megakernel", capi.TRACER_WAVEFRONT: "wavefront"}
+        return {"best": {"tracer": names[tun.best.tracer], "block_threads": tun.best.block_threads,
+                         "queue_paths": tun.best.queue_paths},
+                "candidates": [{"tracer": names[tun.tracer[i]], "block_threads": tun.block_threads[i],
+                                "queue_paths": tun.queue_paths[i], "ms": round(tun.ms[i], 3)}
+                               for i in range(tun.candidates)]}
+
     def trace(self, pc, counters=None):
         capi = self.capi
-        p = capi.TraceParams(self.w, self.h, self.depth, 0, capi.LIGHT_SAMPLE_AUTO,
-                             capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, 1, self.tracer)
-        ti = capi.TraceImages(*(capi.ptr(self.img[k]) for k in ("depth", "normal", "material", "albedo",
-                                                                 "illumination")),
-                              capi.ptr(self.trace_ws), self.trace_ws.numel())
+        p = self.trace_params()
+        ti = self.trace_images()
         cnt = capi.ptr(counters) if counters is not None else None
         capi.check(capi.lib().pbrt_trace(self.scene.handle, C.byref(p), C.byref(pc), C.byref(ti), cnt,
                                          self.stream()))
@@ -299,6 +326,7 @@ def main():
     fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit, args.denoiser)
     setup_s = time.perf_counter() - t_setup
 
+    tuning = fr.autotune() if args.autotune else None
     for f in range(args.warmup):
         fr.run(f)
     torch.cuda.synchronize()
@@ -342,7 +370,7 @@ def main():
     ext_bytes = 48.0 * cnt[0] + 64.0 * cnt[2] + 48.0 * cnt[3]
     sh_bytes = 48.0 * cnt[1] + 64.0 * cnt[4] + 48.0 * cnt[5]
     kernels = {}
-    if args.tracer == "wavefront":
+    if fr.tracer == capi.TRACER_WAVEFRONT:
         for name, b, ms, n in (("k_wf_extend", ext_bytes, timing.extend_ms, timing.extend_launches),
                                ("k_wf_shadow", sh_bytes, timing.shadow_ms, timing.shadow_launches)):
             kernels[name] = dict(ms_per_frame=round(ms / K, 3), launches=int(n),
@@ -356,7 +384,7 @@ def main():
         kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     traffic, traffic_src = measured_traffic(dom)
-    n_launch = timing.extend_launches if args.tracer == "wavefront" else timing.megakernel_launches
+    n_launch = timing.extend_launches if fr.tracer == capi.TRACER_WAVEFRONT else timing.megakernel_launches
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
                     algorithmic_bytes_per_launch=round(dom_bytes / max(1, n_launch)),
@@ -393,6 +421,9 @@ def main():
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
     }
+    if tuning:
+        out["autotune"] = tuning
+        out["config"]["tracer"] = tuning["best"]["tracer"]
     out["denoise_comparison"] = denoise_comparison(torch, fr, args.warmup + K - 1, max(K, 10),
                                                    not args.no_denoise_eval)
     if not args.no_cpu_baseline:

@@ -198,6 +198,12 @@ typedef struct pbrt_trace_params {
     uint32_t output;              /* enum pbrt_illumination_output */
     uint32_t write_gbuffer;       /* GBUFFER define */
     uint32_t tracer;              /* enum pbrt_tracer_kind */
+    /* Launch shape of the wavefront tracer (0 = defaults; pbrt_autotune_trace picks them):
+     * threads per workgroup of the traversal kernels (64, 128 or 256), and paths in
+     * flight (the frame is traced in slices of this many pixels, rounded up to 8x8
+     * tiles, so the ray queues and path state stay in cache; 0 = the whole frame). */
+    uint32_t block_threads;
+    uint32_t queue_paths;
 } pbrt_trace_params;
 
 typedef struct pbrt_trace_images {
@@ -210,6 +216,26 @@ typedef struct pbrt_trace_images {
                               (path state and ray queues); unused by the megakernel */
     size_t workspace_bytes;
 } pbrt_trace_images;
+
+/* Autotuner: traces the frame once per candidate launch configuration (megakernel;
+ * wavefront with 64 / 128 / 256-thread traversal workgroups x whole-frame, half and
+ * quarter queues), `repeats` timed runs each after a warm-up, on `stream`, and returns
+ * the fastest as `best` (a copy of `params` with tracer, block_threads and
+ * queue_paths set).  No counterpart in the reference (the Vulkan driver schedules
+ * its ray tracing pipeline).  Synchronizes the stream. */
+#define PBRT_TUNING_MAX_CANDIDATES 16
+typedef struct pbrt_trace_tuning {
+    pbrt_trace_params best;
+    uint32_t candidates;
+    uint32_t tracer[PBRT_TUNING_MAX_CANDIDATES];
+    uint32_t block_threads[PBRT_TUNING_MAX_CANDIDATES];
+    uint32_t queue_paths[PBRT_TUNING_MAX_CANDIDATES];
+    float ms[PBRT_TUNING_MAX_CANDIDATES];
+} pbrt_trace_tuning;
+
+int32_t pbrt_autotune_trace(const pbrt_scene* scene, const pbrt_trace_params* params,
+                            const pbrt_push_constants* push_constants, const pbrt_trace_images* images,
+                            uint32_t repeats, pbrt_trace_tuning* out, void* stream);
 
 /* Device scratch the wavefront tracer needs for a width x height launch. */
 int32_t pbrt_trace_workspace_size(uint32_t width, uint32_t height, size_t* out_bytes);

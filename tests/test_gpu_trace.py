@@ -201,3 +201,44 @@ def test_trace_timing_records_every_stage(cuda, cfg1):
     assert t.extend_ms > 0 and t.shade_ms > 0 and t.shadow_ms > 0 and t.megakernel_ms > 0
     # timing is off again: ending twice is an error
     assert lib.pbrt_trace_timing_end(scene.handle, C.byref(t)) != 0
+
+
+@pytest.mark.parametrize("block,queue", [(128, 0), (256, 0), (64, 3000), (256, 7777)])
+def test_wavefront_launch_shapes_give_identical_frames(cuda, cfg1, block, queue):
+    """Traversal workgroup size and frame slicing change only the launch shape: every
+    path runs the same arithmetic, so the frame is bit-identical."""
+    gen, scene, _ = cfg1
+    pc, _, _ = helpers.frame_push_constants(gen, W, H, 3)
+    ref = helpers.gbuffer(cuda, W, H)
+    helpers.trace(cuda, scene, pc, ref, W, H, tracer=capi.TRACER_WAVEFRONT)
+    img = helpers.gbuffer(cuda, W, H)
+    cnt0 = cuda.zeros(6, dtype=cuda.int64, device="cuda")
+    cnt1 = cuda.zeros(6, dtype=cuda.int64, device="cuda")
+    helpers.trace(cuda, scene, pc, ref, W, H, tracer=capi.TRACER_WAVEFRONT, counters=cnt0)
+    helpers.trace(cuda, scene, pc, img, W, H, tracer=capi.TRACER_WAVEFRONT, counters=cnt1, block_threads=block,
+                  queue_paths=queue)
+    cuda.cuda.synchronize()
+    for k in ref:
+        assert cuda.equal(ref[k], img[k]), k
+    assert cuda.equal(cnt0, cnt1)
+
+
+def test_autotune_picks_a_measured_candidate(cuda, cfg1):
+    gen, scene, _ = cfg1
+    pc, _, _ = helpers.frame_push_constants(gen, W, H, 0)
+    img = helpers.gbuffer(cuda, W, H)
+    ws = helpers.trace_workspace(cuda, W, H)
+    p = capi.TraceParams(W, H, 2, 0, capi.LIGHT_SAMPLE_AUTO, capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, 1,
+                         capi.TRACER_WAVEFRONT)
+    ti = capi.TraceImages(*(capi.ptr(img[k]) for k in ("depth", "normal", "material", "albedo", "illumination")),
+                          capi.ptr(ws), ws.numel())
+    tun = capi.TraceTuning()
+    capi.check(capi.lib().pbrt_autotune_trace(scene.handle, C.byref(p), C.byref(pc), C.byref(ti), 2, C.byref(tun),
+                                              helpers.stream(cuda)))
+    ms = [tun.ms[i] for i in range(tun.candidates)]
+    print("autotune candidates (tracer, block, queue, ms):",
+          [(tun.tracer[i], tun.block_threads[i], tun.queue_paths[i], round(tun.ms[i], 3)) for i in range(tun.candidates)])
+    assert tun.candidates == 10 and all(m > 0 for m in ms)
+    best = int(np.argmin(ms))
+    assert (tun.best.tracer, tun.best.block_threads, tun.best.queue_paths) == \
+        (tun.tracer[best], tun.block_threads[best], tun.queue_paths[best])

@@ -114,7 +114,16 @@ class SceneInfo(C.Structure):
 class TraceParams(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "width", "height", "max_recursion_depth", "min_recursion_depth",
-        "light_sampling", "output", "write_gbuffer", "tracer")]
+        "light_sampling", "output", "write_gbuffer", "tracer", "block_threads", "queue_paths")]
+
+
+TUNING_MAX = 16
+
+
+class TraceTuning(C.Structure):
+    _fields_ = [("best", TraceParams), ("candidates", C.c_uint32), ("tracer", C.c_uint32 * TUNING_MAX),
+                ("block_threads", C.c_uint32 * TUNING_MAX), ("queue_paths", C.c_uint32 * TUNING_MAX),
+                ("ms", C.c_float * TUNING_MAX)]
 
 
 class TraceImages(C.Structure):
@@ -214,6 +223,8 @@ SIGNATURES = {
     "pbrt_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
                                C.POINTER(TraceImages), C.c_void_p, C.c_void_p]),
     "pbrt_trace_workspace_size": (C.c_int32, [C.c_uint32, C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pbrt_autotune_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
+                                        C.POINTER(TraceImages), C.c_uint32, C.POINTER(TraceTuning), C.c_void_p]),
     "pbrt_trace_timing_begin": (C.c_int32, [C.c_void_p]),
     "pbrt_trace_timing_end": (C.c_int32, [C.c_void_p, C.POINTER(TraceTiming)]),
     "pbrt_intersect": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -12,6 +12,8 @@
 //   * the motion/alpha/format rules of the other passes live in their files.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "capi_common.hpp"
 #include "scene_internal.hpp"
 #include "trace_common.hpp"
@@ -237,7 +239,7 @@ extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* 
         float4* illum = reinterpret_cast<float4*>(img->illumination);
         TraceTiming* tm = scene_timing(scene);
         if (p->tracer == PBRT_TRACER_WAVEFRONT) {
-            wavefront_trace(sc, c, *img, counters, tm, st);
+            wavefront_trace(sc, c, *img, counters, tm, p->block_threads, p->queue_paths, st);
         } else {
             if (tm) tm->frame_begin(st);
             if (counters)
@@ -285,5 +287,60 @@ extern "C" int32_t pbrt_rng_sequence(uint32_t width, uint32_t height, uint32_t f
         hipLaunchKernelGGL(k_rng_sequence, grid, dim3(256), 0, (hipStream_t)stream, width, height, frame_index, count,
                            out);
         PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_autotune_trace(const pbrt_scene* scene, const pbrt_trace_params* p,
+                                       const pbrt_push_constants* pc, const pbrt_trace_images* img, uint32_t repeats,
+                                       pbrt_trace_tuning* out, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(scene && p && pc && img && out, "pbrt_autotune_trace: NULL argument");
+        PBRT_REQUIRE(repeats >= 1 && repeats <= 1000, "pbrt_autotune_trace: repeats must be in [1, 1000]");
+        hipStream_t st = (hipStream_t)stream;
+        *out = pbrt_trace_tuning{};
+        struct Cand {
+            uint32_t tracer, block, queue;
+        };
+        std::vector<Cand> cands{{PBRT_TRACER_MEGAKERNEL, 0, 0}};
+        const uint32_t pixels = p->width * p->height;
+        if (img->workspace) {
+            for (uint32_t block : {64u, 128u, 256u})
+                for (uint32_t div : {1u, 2u, 4u})
+                    cands.push_back({PBRT_TRACER_WAVEFRONT, block, div == 1 ? 0u : (pixels + div - 1) / div});
+        }
+        PBRT_REQUIRE(cands.size() <= PBRT_TUNING_MAX_CANDIDATES, "pbrt_autotune_trace: too many candidates");
+        hipEvent_t e0, e1;
+        PBRT_HIP(hipEventCreate(&e0));
+        PBRT_HIP(hipEventCreate(&e1));
+        float best = 3.4e38f;
+        for (size_t i = 0; i < cands.size(); ++i) {
+            pbrt_trace_params q = *p;
+            q.tracer = cands[i].tracer;
+            q.block_threads = cands[i].block;
+            q.queue_paths = cands[i].queue;
+            int32_t rc = pbrt_trace(scene, &q, pc, img, nullptr, stream);  // warm-up
+            if (rc != PBRT_OK) fail(rc, std::string("pbrt_autotune_trace: ") + pbrt_last_error());
+            PBRT_HIP(hipEventRecord(e0, st));
+            for (uint32_t r = 0; r < repeats; ++r) {
+                rc = pbrt_trace(scene, &q, pc, img, nullptr, stream);
+                if (rc != PBRT_OK) fail(rc, std::string("pbrt_autotune_trace: ") + pbrt_last_error());
+            }
+            PBRT_HIP(hipEventRecord(e1, st));
+            PBRT_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PBRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= (float)repeats;
+            out->tracer[i] = q.tracer;
+            out->block_threads[i] = q.block_threads;
+            out->queue_paths[i] = q.queue_paths;
+            out->ms[i] = ms;
+            if (ms < best) {
+                best = ms;
+                out->best = q;
+            }
+        }
+        out->candidates = (uint32_t)cands.size();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     });
 }

@@ -55,6 +55,7 @@ __device__ __forceinline__ LightSample sample_light(const DeviceScene& sc, uint3
 class TraceTiming;
 size_t wavefront_workspace_size(uint32_t w, uint32_t h);
 void wavefront_trace(const DeviceScene& sc, const TraceConsts& c, const pbrt_trace_images& img,
-                     unsigned long long* counters, TraceTiming* tm, hipStream_t st);
+                     unsigned long long* counters, TraceTiming* tm, uint32_t block_threads, uint32_t queue_paths,
+                     hipStream_t st);
 
 }  // namespace pbrt

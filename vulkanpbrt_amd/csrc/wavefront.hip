@@ -19,6 +19,8 @@
 // light sample, then BRDF sample (on a copy) and the roulette draw.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "capi_common.hpp"
 #include "scene_internal.hpp"
 #include "trace_common.hpp"
@@ -49,8 +51,13 @@ struct WfBuffers {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static size_t wf_layout(uint32_t w, uint32_t h, char* base, WfBuffers* b) {
-    const size_t tiles = (size_t)((w + kTileX - 1) / kTileX) * ((h + kTileY - 1) / kTileY);
+static size_t frame_tiles(uint32_t w, uint32_t h) {
+    return (size_t)((w + kTileX - 1) / kTileX) * ((h + kTileY - 1) / kTileY);
+}
+
+// queue_tiles: tiles traced at once (0 = the whole frame)
+static size_t wf_layout(uint32_t w, uint32_t h, size_t queue_tiles, char* base, WfBuffers* b) {
+    const size_t tiles = queue_tiles ? std::min(queue_tiles, frame_tiles(w, h)) : frame_tiles(w, h);
     const size_t n = tiles * kTileThreads;  // queue capacity: the tile-padded pixel count
     const size_t np = (size_t)w * h;
     size_t off = 0;
@@ -94,16 +101,18 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(below);
 }
 
-__global__ void k_wf_init(WfBuffers b) {
+// slice_rays: queue entries of this slice (its tiles x 64 threads)
+__global__ void k_wf_init(WfBuffers b, uint32_t slice_rays) {
     const int i = threadIdx.x;
-    if (i < 2 * kMaxWfBounces) b.counters[i] = i == 0 ? b.capacity : 0u;
+    if (i < 2 * kMaxWfBounces) b.counters[i] = i == 0 ? slice_rays : 0u;
 }
 
-__global__ void __launch_bounds__(kWfThreads) k_wf_generate(TraceConsts c, WfBuffers b) {
+__global__ void __launch_bounds__(kWfThreads) k_wf_generate(TraceConsts c, WfBuffers b, uint32_t tile_begin,
+                                                            uint32_t slice_rays) {
     const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
-    if (q >= b.capacity) return;
+    if (q >= slice_rays) return;
     const uint32_t tiles_x = (c.width + kTileX - 1) / kTileX;
-    const uint32_t tile = q / kTileThreads, lid = q % kTileThreads;
+    const uint32_t tile = tile_begin + q / kTileThreads, lid = q % kTileThreads;
     const uint32_t px = (tile % tiles_x) * kTileX + lid % kTileX;
     const uint32_t py = (tile / tiles_x) * kTileY + lid / kTileX;
     if (px >= c.width || py >= c.height) {
@@ -124,11 +133,11 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_generate(TraceConsts c, WfBuf
     b.state[path] = make_int4(0, 0, 0, 0);
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(kWfThreads) k_wf_extend(DeviceScene sc, WfBuffers b, int bounce,
-                                                          unsigned long long* counters) {
-    __shared__ int stack_mem[kTraversalStackSize * kWfThreads];
-    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+template <bool kCount, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_wf_extend(DeviceScene sc, WfBuffers b, int bounce,
+                                                        unsigned long long* counters) {
+    __shared__ int stack_mem[kTraversalStackSize * kThreads];
+    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
     const int qi = bounce & 1;
     TraversalCounters cnt{0, 0};
     if (q < b.counters[bounce] && b.ray_path[qi][q] != kDeadPath) {
@@ -136,7 +145,7 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_extend(DeviceScene sc, WfBuff
         const float4 d = b.ray_d[qi][q];
         Hit h;
         traverse<false, kCount>(sc, make_vec3(o.x, o.y, o.z), make_vec3(d.x, d.y, d.z), o.w, d.w, h,
-                                stack_mem + threadIdx.x, kWfThreads, cnt);
+                                stack_mem + threadIdx.x, kThreads, cnt);
         b.hit[q] = make_float4(h.t, __uint_as_float(h.tri), h.u, h.v);
     }
     if (kCount) {
@@ -272,18 +281,18 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
     }
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(kWfThreads) k_wf_shadow(DeviceScene sc, WfBuffers b, int bounce,
-                                                          unsigned long long* counters) {
-    __shared__ int stack_mem[kTraversalStackSize * kWfThreads];
-    const uint32_t q = blockIdx.x * kWfThreads + threadIdx.x;
+template <bool kCount, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_wf_shadow(DeviceScene sc, WfBuffers b, int bounce,
+                                                        unsigned long long* counters) {
+    __shared__ int stack_mem[kTraversalStackSize * kThreads];
+    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
     TraversalCounters cnt{0, 0};
     if (q < b.counters[kMaxWfBounces + bounce]) {
         const float4 o = b.sh_o[q];
         const float4 d = b.sh_d[q];
         Hit h;
         bool occ = traverse<true, kCount>(sc, make_vec3(o.x, o.y, o.z), make_vec3(d.x, d.y, d.z), o.w, d.w, h,
-                                          stack_mem + threadIdx.x, kWfThreads, cnt);
+                                          stack_mem + threadIdx.x, kThreads, cnt);
         if (!occ) {
             const uint32_t path = b.sh_path[q];
             const float4 p = b.sh_pend[q];
@@ -328,40 +337,80 @@ __global__ void k_wf_count(WfBuffers b, int bounces, uint32_t pixels, unsigned l
     counters[1] += shadow;
 }
 
-size_t wavefront_workspace_size(uint32_t w, uint32_t h) { return wf_layout(w, h, nullptr, nullptr); }
+size_t wavefront_workspace_size(uint32_t w, uint32_t h) { return wf_layout(w, h, 0, nullptr, nullptr); }
+
+template <int kThreads>
+static void launch_traversal(bool shadow, bool count, const DeviceScene& sc, const WfBuffers& b, int bounce,
+                             unsigned long long* counters, hipStream_t st) {
+    const dim3 grid((b.capacity + kThreads - 1) / kThreads);
+    if (shadow) {
+        if (count)
+            hipLaunchKernelGGL((k_wf_shadow<true, kThreads>), grid, dim3(kThreads), 0, st, sc, b, bounce, counters);
+        else
+            hipLaunchKernelGGL((k_wf_shadow<false, kThreads>), grid, dim3(kThreads), 0, st, sc, b, bounce, counters);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_wf_extend<true, kThreads>), grid, dim3(kThreads), 0, st, sc, b, bounce, counters);
+        else
+            hipLaunchKernelGGL((k_wf_extend<false, kThreads>), grid, dim3(kThreads), 0, st, sc, b, bounce, counters);
+    }
+}
+
+static void traversal(uint32_t block_threads, bool shadow, bool count, const DeviceScene& sc, const WfBuffers& b,
+                      int bounce, unsigned long long* counters, hipStream_t st) {
+    if (block_threads == 256)
+        launch_traversal<256>(shadow, count, sc, b, bounce, counters, st);
+    else if (block_threads == 128)
+        launch_traversal<128>(shadow, count, sc, b, bounce, counters, st);
+    else
+        launch_traversal<64>(shadow, count, sc, b, bounce, counters, st);
+}
 
 void wavefront_trace(const DeviceScene& sc, const TraceConsts& c, const pbrt_trace_images& img,
-                     unsigned long long* counters, TraceTiming* tm, hipStream_t st) {
+                     unsigned long long* counters, TraceTiming* tm, uint32_t block_threads, uint32_t queue_paths,
+                     hipStream_t st) {
+    PBRT_REQUIRE(block_threads == 0 || block_threads == 64 || block_threads == 128 || block_threads == 256,
+                 "pbrt_trace: block_threads must be 0, 64, 128 or 256");
+    const size_t all_tiles = frame_tiles(c.width, c.height);
+    const size_t slice_tiles = queue_paths ? std::min(all_tiles, ((size_t)queue_paths + kTileThreads - 1) / kTileThreads)
+                                           : all_tiles;
     WfBuffers b;
-    size_t need = wf_layout(c.width, c.height, static_cast<char*>(img.workspace), &b);
+    size_t need = wf_layout(c.width, c.height, slice_tiles, static_cast<char*>(img.workspace), &b);
     PBRT_REQUIRE(img.workspace != nullptr && img.workspace_bytes >= need,
                  "pbrt_trace: the wavefront tracer needs a workspace of pbrt_trace_workspace_size() bytes");
     const int bounces = (int)c.max_depth + 1 + (sc.has_transmissive ? 10 : 0);
     PBRT_REQUIRE(bounces < kMaxWfBounces, "pbrt_trace: max_recursion_depth too large for the wavefront tracer");
     const dim3 grid((b.capacity + kWfThreads - 1) / kWfThreads);
     float4* illum = reinterpret_cast<float4*>(img.illumination);
+    const uint32_t tiles_x = (c.width + kTileX - 1) / kTileX;
     if (tm) tm->frame_begin(st);
-    hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(2 * kMaxWfBounces), 0, st, b);
-    hipLaunchKernelGGL(k_wf_generate, grid, dim3(kWfThreads), 0, st, c, b);
-    if (tm) tm->mark(kTimeOther, st);
-    for (int bounce = 0; bounce < bounces; ++bounce) {
-        if (counters)
-            hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
-        else
-            hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
-        if (tm) tm->mark(kTimeExtend, st);
-        hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfThreads), 0, st, sc, c, b, bounce, img.depth, img.normal,
-                           img.material, img.albedo);
-        if (tm) tm->mark(kTimeShade, st);
-        if (counters)
-            hipLaunchKernelGGL(k_wf_shadow<true>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
-        else
-            hipLaunchKernelGGL(k_wf_shadow<false>, grid, dim3(kWfThreads), 0, st, sc, b, bounce, counters);
-        if (tm) tm->mark(kTimeShadow, st);
+    for (size_t t0 = 0; t0 < all_tiles; t0 += slice_tiles) {
+        const size_t nt = std::min(slice_tiles, all_tiles - t0);
+        const uint32_t slice_rays = (uint32_t)(nt * kTileThreads);
+        hipLaunchKernelGGL(k_wf_init, dim3(1), dim3(2 * kMaxWfBounces), 0, st, b, slice_rays);
+        hipLaunchKernelGGL(k_wf_generate, grid, dim3(kWfThreads), 0, st, c, b, (uint32_t)t0, slice_rays);
+        if (tm) tm->mark(kTimeOther, st);
+        for (int bounce = 0; bounce < bounces; ++bounce) {
+            traversal(block_threads, false, counters != nullptr, sc, b, bounce, counters, st);
+            if (tm) tm->mark(kTimeExtend, st);
+            hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfThreads), 0, st, sc, c, b, bounce, img.depth, img.normal,
+                               img.material, img.albedo);
+            if (tm) tm->mark(kTimeShade, st);
+            traversal(block_threads, true, counters != nullptr, sc, b, bounce, counters, st);
+            if (tm) tm->mark(kTimeShadow, st);
+        }
+        if (counters) {
+            // primary rays: the slice's pixels inside the image
+            uint32_t pixels = 0;
+            for (size_t t = t0; t < t0 + nt; ++t) {
+                const uint32_t x = (uint32_t)(t % tiles_x) * kTileX, y = (uint32_t)(t / tiles_x) * kTileY;
+                pixels += std::min<uint32_t>(kTileX, c.width - x) * std::min<uint32_t>(kTileY, c.height - y);
+            }
+            hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, st, b, bounces, pixels, counters);
+        }
     }
     const uint32_t np = c.width * c.height;
     hipLaunchKernelGGL(k_wf_finalize, dim3((np + 255) / 256), dim3(256), 0, st, c, b, illum);
-    if (counters) hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, st, b, bounces, np, counters);
     if (tm) tm->mark(kTimeOther, st);
 }
 
