@@ -395,6 +395,7 @@ struct pbrt_generated_scene {
     std::vector<pbrt_mesh_desc> mesh_descs;
     std::vector<pbrt_texture_desc> tex_descs;
     std::vector<pbrt_camera_pose> camera_keys;
+    uint32_t path_frames = 64;  // frames of one loop of the camera path
     pbrt_camera_pose camera{};
 };
 
@@ -725,6 +726,128 @@ void build_atrium(pbrt_generated_scene& g, uint64_t seed) {
     g.camera = g.camera_keys[0];
 }
 
+// Config 5: a city of 4096 axis-aligned towers (6 subdivided box variants of
+// ~1.2k triangles each, log-uniform heights) on a 64 x 64 block grid, a ground
+// plane, 65,536 small emissive window quads on random facades (131,072 emissive
+// triangles, so light sampling switches to UNIFORM) and a dim sun; a 128-frame
+// Catmull-Rom flight over the roofs.
+void build_city(pbrt_generated_scene& g, uint64_t seed) {
+    Pcg32 rng(seed);
+    float ident[16];
+    identity(ident);
+    const int grid = 64;
+    const float spacing = 20.0f, half = 0.5f * grid * spacing;
+    {
+        Mesh ground;
+        float p0[3] = {-half, -half, 0}, p1[3] = {half, -half, 0}, p3[3] = {-half, half, 0};
+        add_quad(ground, p0, p1, p3, 64, 64, 64.0f, 64.0f, true);
+        ground.mat = make_material(0.18f, 0.18f, 0.19f, 0.8f, false);
+        ground.mat.category_id = 1;
+        g.meshes.push_back(ground);
+        add_instance(g, 0, ident);
+    }
+    // building variants: unit box [-0.5, 0.5]^2 x [0, 1], sides subdivided 8 x 16, roof 8 x 8
+    const int variants = 6;
+    const uint32_t first_building = (uint32_t)g.meshes.size();
+    for (int v = 0; v < variants; ++v) {
+        Mesh m;
+        m.use16 = true;
+        const float c[4][3] = {{-0.5f, -0.5f, 0}, {0.5f, -0.5f, 0}, {0.5f, 0.5f, 0}, {-0.5f, 0.5f, 0}};
+        for (int f = 0; f < 4; ++f) {
+            const float* a = c[f];
+            const float* b = c[(f + 1) % 4];
+            float top[3] = {a[0], a[1], 1.0f};
+            add_quad(m, a, b, top, 8, 16, 4.0f, 8.0f, true);
+        }
+        float r0[3] = {-0.5f, -0.5f, 1.0f}, r1[3] = {0.5f, -0.5f, 1.0f}, r3[3] = {-0.5f, 0.5f, 1.0f};
+        add_quad(m, r0, r1, r3, 8, 8, 1.0f, 1.0f, true);
+        const float tone = 0.35f + 0.1f * v;
+        const bool metal = v == 5;
+        m.mat = make_material(tone, tone * (0.95f + 0.02f * v), tone * 0.9f, 0.3f + 0.1f * v, metal);
+        m.mat.category_id = (uint32_t)(2 + v);
+        g.meshes.push_back(m);
+    }
+    struct Tower {
+        float x, y, sx, sy, sz;
+    };
+    std::vector<Tower> towers;
+    towers.reserve(grid * grid);
+    for (int j = 0; j < grid; ++j)
+        for (int i = 0; i < grid; ++i) {
+            Tower t;
+            t.x = -half + (i + 0.5f) * spacing + rng.uniform(-2.0f, 2.0f);
+            t.y = -half + (j + 0.5f) * spacing + rng.uniform(-2.0f, 2.0f);
+            t.sx = rng.uniform(8.0f, 14.0f);
+            t.sy = rng.uniform(8.0f, 14.0f);
+            t.sz = rng.log_uniform(10.0f, 120.0f);
+            float tr[16];
+            trs(tr, t.x, t.y, 0.0f, 0.0f, t.sx, t.sy, t.sz);
+            add_instance(g, first_building + (uint32_t)(rng.next() % variants), tr);
+            towers.push_back(t);
+        }
+    // emissive windows: a unit quad in the local y-z plane facing +x
+    {
+        Mesh w;
+        float p0[3] = {0, -0.5f, -0.5f}, p1[3] = {0, 0.5f, -0.5f}, p3[3] = {0, -0.5f, 0.5f};
+        add_quad(w, p0, p1, p3, 1, 1, 1.0f, 1.0f, true);
+        w.mat = make_material(0.9f, 0.85f, 0.7f, 1.0f, false);
+        w.mat.emission_texture_id[0] = 6.0f;
+        w.mat.emission_texture_id[1] = 5.0f;
+        w.mat.emission_texture_id[2] = 3.5f;
+        w.mat.category_id = 9;
+        g.meshes.push_back(w);
+        const uint32_t window = (uint32_t)g.meshes.size() - 1;
+        for (int k = 0; k < 65536; ++k) {
+            const Tower& t = towers[rng.next() % towers.size()];
+            const int face = (int)(rng.next() % 4);  // +x, +y, -x, -y
+            const float along = rng.uniform(-0.45f, 0.45f), z = rng.uniform(2.0f, t.sz - 1.0f);
+            float x = t.x, y = t.y, ang = 0.0f;
+            const float off = 0.02f;
+            if (face == 0) {
+                x += 0.5f * t.sx + off;
+                y += along * t.sy;
+            } else if (face == 1) {
+                y += 0.5f * t.sy + off;
+                x += along * t.sx;
+                ang = 1.5707963f;
+            } else if (face == 2) {
+                x -= 0.5f * t.sx + off;
+                y += along * t.sy;
+                ang = 3.1415927f;
+            } else {
+                y -= 0.5f * t.sy + off;
+                x += along * t.sx;
+                ang = 4.7123890f;
+            }
+            float tr[16];
+            trs(tr, x, y, z, ang, 1.0f, 0.8f, 1.2f);
+            add_instance(g, window, tr);
+        }
+    }
+    {
+        pbrt_light l{};
+        l.v0_type[3] = 1.0f;
+        float d[3] = {0.3f, 0.5f, -0.8f};
+        float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int a = 0; a < 3; ++a) l.dir_angle2[a] = d[a] / len;
+        for (int a = 0; a < 3; ++a) {
+            l.col_ambient[a] = 0.3f;
+            l.col_diffuse[a] = 0.3f;
+        }
+        l.strengths[0] = 0.5f;
+        g.lights.push_back(l);
+    }
+    const float keys[8][6] = {{-500, -500, 160, 0, 0, 20},   {-100, -560, 120, 100, 0, 30},
+                              {420, -420, 180, 0, 0, 10},     {560, 60, 140, -100, 50, 40},
+                              {380, 420, 100, 0, 0, 30},      {0, 560, 150, 0, -100, 20},
+                              {-420, 380, 200, 50, 0, 10},    {-580, -60, 130, 0, 0, 40}};
+    for (auto& k : keys)
+        g.camera_keys.push_back(
+            pbrt_camera_pose{{k[0], k[1], k[2]}, {k[3], k[4], k[5]}, {0, 0, 1}, 60.0f, 0.5f, 5000.0f});
+    g.path_frames = 128;
+    g.camera = g.camera_keys[0];
+}
+
 void finalize(pbrt_generated_scene& g) {
     g.mesh_descs.clear();
     for (Mesh& m : g.meshes) {
@@ -771,6 +894,8 @@ extern "C" int32_t pbrt_generate_scene(uint32_t config, uint64_t seed, pbrt_gene
             build_cornell(*g, seed);
         else if (config == 2)
             build_atrium(*g, seed);
+        else if (config == 5)
+            build_city(*g, seed);
         else
             fail(PBRT_ERR_UNSUPPORTED, "pbrt_generate_scene: unknown config " + std::to_string(config));
         finalize(*g);
@@ -802,9 +927,9 @@ extern "C" int32_t pbrt_generated_scene_camera(const pbrt_generated_scene* g, ui
             *out = g->camera;
             return;
         }
-        // 64 frames over the closed loop of keys
+        // path_frames frames over the closed loop of keys
         const uint32_t n = (uint32_t)g->camera_keys.size();
-        float s = (float)(frame % 64) / 64.0f * n;
+        float s = (float)(frame % g->path_frames) / (float)g->path_frames * n;
         uint32_t i1 = (uint32_t)s % n;
         float t = s - std::floor(s);
         const pbrt_camera_pose& k0 = g->camera_keys[(i1 + n - 1) % n];
