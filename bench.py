@@ -44,6 +44,11 @@ def parse():
                     help="denoiser of the timed frame: BMFR 32x32 or the learned kernel-predicting network")
     ap.add_argument("--no-denoise-eval", action="store_true",
                     help="skip the learned-vs-BMFR relMSE evaluation on held-out Cornell-box scenes")
+    ap.add_argument("--graph", dest="graph", action="store_true", default=None,
+                    help="capture each frame (trace, accumulate, denoise, TAA) into one hipGraph and launch it "
+                         "(default for the city; the per-kernel breakdown then comes from an eager pass over "
+                         "the same frames after the timed region)")
+    ap.add_argument("--eager", dest="graph", action="store_false", help="launch the frame's kernels one by one")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -52,15 +57,16 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summaries (lines `traffic_bytes_per_launch: <kernel> <bytes>` under profiles/)."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` (e.g. "atrium-1080p-sah") from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (lines
+    `traffic_bytes_per_launch: <kernel> <workload> <bytes>` under profiles/)."""
     import re
     for path in sorted((ROOT / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
-            m = re.match(r"traffic_bytes_per_launch:\s+(\S+)\s+(\d+)", line)
-            if m and m.group(1) == kernel:
-                return int(m.group(2)), path.name
+            m = re.match(r"traffic_bytes_per_launch:\s+(\S+)\s+(\S+)\s+(\d+)", line)
+            if m and m.group(1) == kernel and m.group(2) == workload:
+                return int(m.group(3)), path.name
     return None, None
 
 
@@ -77,179 +83,15 @@ def copy_calibration(torch, mib):
 
 WORKLOADS = {
     1: dict(name="cornell box + 16 icospheres (5.1k tris), 256x256, 1 spp, 2 bounces, accumulator + BMFR + TAA",
-            width=256, height=256, depth=2, seed=1),
+            width=256, height=256, depth=2, seed=1, tag="cornell-256"),
     2: dict(name="atrium (~267k tris, 8 textures, 3 area quads + sun), 1920x1080, 1 spp, 4 bounces, "
                  "accumulator + BMFR 32x32 + TAA over the 64-frame camera spline",
-            width=1920, height=1080, depth=4, seed=2),
+            width=1920, height=1080, depth=4, seed=2, tag="atrium-1080p"),
     5: dict(name="city stress (~4.9M tris: 4096 towers, 65,536 emissive window quads = 131k triangle lights, "
                  "UNIFORM light sampling), 3840x2160, 1 spp, 8 bounces with Russian roulette, accumulator + "
                  "BMFR + TAA over the 128-frame camera path",
-            width=3840, height=2160, depth=8, seed=5, bvh="lbvh"),
+            width=3840, height=2160, depth=8, seed=5, bvh="lbvh", graph=True, tag="city-2160p"),
 }
-
-
-class Frame:
-    """Device images and the per-frame launch sequence (all on torch's current stream)."""
-
-    def __init__(self, torch, capi, camera, gen, scene, w, h, depth, tracer, bmfr_fit, denoiser="bmfr"):
-        self.torch, self.capi, self.camera = torch, capi, camera
-        self.denoiser = denoiser
-        self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
-        self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
-        self.block_threads, self.queue_paths = 0, 0
-        self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
-        d = "cuda"
-        t = torch
-        self.img = dict(
-            depth=t.zeros((h, w), dtype=t.float32, device=d),
-            normal=t.zeros((h, w, 2), dtype=t.float32, device=d),
-            material=t.zeros((h, w, 4), dtype=t.uint8, device=d),
-            albedo=t.zeros((h, w, 4), dtype=t.uint8, device=d),
-            illumination=t.zeros((h, w, 4), dtype=t.float32, device=d),
-        )
-        # ping-pong history: index = frame parity (replaces the copy-back image copies)
-        self.depth_hist = [t.zeros((h, w), dtype=t.float32, device=d) for _ in range(2)]
-        self.spp = [t.zeros((h, w), dtype=t.uint8, device=d) for _ in range(2)]
-        self.acc = [t.zeros((h, w, 4), dtype=t.int16, device=d) for _ in range(2)]
-        self.motion = t.zeros((h, w, 2), dtype=t.int16, device=d)
-        self.denoised = t.zeros((2, h, w, 4), dtype=t.int16, device=d)
-        self.bmfr_final = t.zeros((h, w, 4), dtype=t.uint8, device=d)
-        self.taa_hist = t.zeros((h, w, 4), dtype=t.uint8, device=d)
-        self.final = t.zeros((h, w, 4), dtype=t.uint8, device=d)
-        bp = capi.BmfrParams(w, h, 32, 0, self.fit)
-        n = C.c_size_t()
-        capi.check(capi.lib().pbrt_bmfr_workspace_size(C.byref(bp), C.byref(n)))
-        self.ws = t.zeros(n.value, dtype=t.uint8, device=d)
-        self.ws_bytes = n.value
-        tn = C.c_size_t()
-        capi.check(capi.lib().pbrt_trace_workspace_size(w, h, C.byref(tn)))
-        self.trace_ws = t.zeros(tn.value, dtype=t.uint8, device=d)
-        self.prev_view = None
-        self.net = None
-        if denoiser == "cnn":
-            self.ensure_cnn()
-
-    def ensure_cnn(self):
-        """The learned denoiser with the committed trained parameters."""
-        if self.net is not None:
-            return
-        from vulkanpbrt_amd.denoiser_train import DEFAULT_WEIGHTS, Network
-        self.net = Network()
-        self.net.load(DEFAULT_WEIGHTS)
-        cn = C.c_size_t()
-        self.capi.check(self.capi.lib().pbrt_cnn_workspace_size(self.w, self.h, 1, 0, C.byref(cn)))
-        self.cnn_ws = self.torch.zeros(cn.value, dtype=self.torch.uint8, device="cuda")
-
-    def stream(self):
-        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
-
-    def push_constants(self, frame):
-        pose = self.gen.camera(frame)
-        view, proj = self.camera.pose_matrices(pose, self.w, self.h)
-        prev = self.prev_view if self.prev_view is not None else view
-        return self.camera.push_constants(view, proj, prev, frame), view, proj
-
-    def trace_params(self):
-        return self.capi.TraceParams(self.w, self.h, self.depth, 0, self.capi.LIGHT_SAMPLE_AUTO,
-                                     self.capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, 1, self.tracer, self.block_threads,
-                                     self.queue_paths)
-
-    def trace_images(self):
-        capi = self.capi
-        return capi.TraceImages(*(capi.ptr(self.img[k]) for k in ("depth", "normal", "material", "albedo",
-                                                                   "illumination")),
-                                capi.ptr(self.trace_ws), self.trace_ws.numel())
-
-    def autotune(self, repeats=3):
-        """pbrt_autotune_trace on the first frame; adopts the fastest launch shape."""
-        capi = self.capi
-        pc, _, _ = self.push_constants(0)
-        tun = capi.TraceTuning()
-        capi.check(capi.lib().pbrt_autotune_trace(self.scene.handle, C.byref(self.trace_params()), C.byref(pc),
-                                                  C.byref(self.trace_images()), repeats, C.byref(tun), self.stream()))
-        self.tracer, self.block_threads, self.queue_paths = tun.best.tracer, tun.best.block_threads, \
-            tun.best.queue_paths
-        names = {capi.TRACER_MEGAKERNEL: "megakernel", capi.TRACER_WAVEFRONT: "wavefront"}
-        return {"best": {"tracer": names[tun.best.tracer], "block_threads": tun.best.block_threads,
-                         "queue_paths": tun.best.queue_paths},
-                "candidates": [{"tracer": names[tun.tracer[i]], "block_threads": tun.block_threads[i],
-                                "queue_paths": tun.queue_paths[i], "ms": round(tun.ms[i], 3)}
-                               for i in range(tun.candidates)]}
-
-    def trace(self, pc, counters=None):
-        capi = self.capi
-        p = self.trace_params()
-        ti = self.trace_images()
-        cnt = capi.ptr(counters) if counters is not None else None
-        capi.check(capi.lib().pbrt_trace(self.scene.handle, C.byref(p), C.byref(pc), C.byref(ti), cnt,
-                                         self.stream()))
-
-    def accumulate(self, frame, view, proj):
-        capi, cam = self.capi, self.camera
-        cur, prv = frame & 1, (frame & 1) ^ 1
-        p = capi.AccumulateParams(self.w, self.h, frame, 0)
-        p.inv_proj[:] = cam.colmajor(np.linalg.inv(proj)).tolist()
-        p.inv_view[:] = cam.colmajor(np.linalg.inv(view)).tolist()
-        pv = self.prev_view if self.prev_view is not None else view
-        p.prev_view[:] = cam.colmajor(pv).tolist()
-        o = np.linalg.inv(pv)[:, 3].astype(np.float32)
-        o[3] = 1.0
-        p.prev_origin[:] = o.tolist()
-        # the G-buffer depth of this frame is written into depth_hist[cur] after the pass
-        im = capi.AccumulateImages(capi.ptr(self.img["illumination"]), capi.ptr(self.img["depth"]),
-                                   capi.ptr(self.depth_hist[prv]), capi.ptr(self.spp[prv]), capi.ptr(self.acc[prv]),
-                                   capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]))
-        capi.check(capi.lib().pbrt_accumulate(C.byref(p), C.byref(im), self.stream()))
-        capi.check(capi.lib().pbrt_copy_back(self.w, self.h, capi.ptr(self.img["depth"]), None, None,
-                                             capi.ptr(self.depth_hist[cur]), None, None, self.stream()))
-
-    def bmfr(self, frame):
-        capi = self.capi
-        cur = frame & 1
-        p = capi.BmfrParams(self.w, self.h, 32, frame, self.fit)
-        im = capi.BmfrImages(capi.ptr(self.img["depth"]), capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]),
-                             capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]),
-                             capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
-        capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), self.stream()))
-
-    def cnn(self, frame):
-        capi = self.capi
-        cur = frame & 1
-        im = capi.CnnImages(self.w, self.h, 1, capi.ptr(self.acc[cur]), capi.ptr(self.img["depth"]),
-                            capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]), capi.ptr(self.spp[cur]),
-                            capi.ptr(self.motion), None, capi.ptr(self.bmfr_final), None, capi.ptr(self.cnn_ws),
-                            self.cnn_ws.numel())
-        capi.check(capi.lib().pbrt_cnn_denoise(self.net.handle, C.byref(im), self.stream()))
-
-    def denoise(self, frame):
-        if self.denoiser == "cnn":
-            self.cnn(frame)
-        else:
-            self.bmfr(frame)
-
-    def taa(self, frame):
-        capi = self.capi
-        im = capi.TaaImages(capi.ptr(self.motion), capi.ptr(self.bmfr_final), capi.ptr(self.taa_hist),
-                            capi.ptr(self.final))
-        capi.check(capi.lib().pbrt_taa(self.w, self.h, frame, C.byref(im), self.stream()))
-
-    def run(self, frame, events=None):
-        pc, view, proj = self.push_constants(frame)
-        if events:
-            events[0].record()
-        self.trace(pc)
-        if events:
-            events[1].record()
-        self.accumulate(frame, view, proj)
-        if events:
-            events[2].record()
-        self.denoise(frame)
-        if events:
-            events[3].record()
-        self.taa(frame)
-        if events:
-            events[4].record()
-        self.prev_view = view
 
 
 def denoise_comparison(torch, fr, frame, runs, evaluate):
@@ -323,6 +165,7 @@ def main():
         sys.exit(2)
     import torch
     from vulkanpbrt_amd import camera, capi
+    from vulkanpbrt_amd.frame import Frame
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
@@ -341,27 +184,56 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     tuning = fr.autotune() if args.autotune else None
+    use_graph = cfg.get("graph", False) if args.graph is None else args.graph
+    graph = None
+    if use_graph:
+        graph = C.c_void_p()
+        capi.check(capi.lib().pbrt_frame_graph_create(C.byref(graph)))
+        # a capture needs a stream other than the null stream
+        side = torch.cuda.Stream()
+        torch.cuda.set_stream(side)
     for f in range(args.warmup):
-        fr.run(f)
+        fr.run(f, graph=graph)
     torch.cuda.synchronize()
 
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(K)]
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     lib = capi.lib()
-    torch.cuda.synchronize()
-    # per-stage tracer time: HIP events recorded by the library on the launch stream
-    capi.check(lib.pbrt_trace_timing_begin(scene.handle))
-    start.record()
-    for k in range(K):
-        fr.run(args.warmup + k, ev[k])
-    end.record()
-    torch.cuda.synchronize()
-    timing = capi.TraceTiming()
-    capi.check(lib.pbrt_trace_timing_end(scene.handle, C.byref(timing)))
-    total_ms = start.elapsed_time(end)
-    stage = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)] for k in range(K)])
-    stage_ms = stage.mean(axis=0)
+
+    def eager_pass(first):
+        """K frames launched one by one with per-stage events and the library's
+        per-kernel tracer timing (HIP events on the launch stream)."""
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(K)]
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        capi.check(lib.pbrt_trace_timing_begin(scene.handle))
+        start.record()
+        for k in range(K):
+            fr.run(first + k, ev[k])
+        end.record()
+        torch.cuda.synchronize()
+        tm = capi.TraceTiming()
+        capi.check(lib.pbrt_trace_timing_end(scene.handle, C.byref(tm)))
+        st = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)] for k in range(K)])
+        return start.elapsed_time(end), st.mean(axis=0), tm
+
+    graph_info = None
+    if use_graph:
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for k in range(K):
+            fr.run(args.warmup + k, graph=graph)
+        end.record()
+        torch.cuda.synchronize()
+        total_ms = start.elapsed_time(end)
+        inst_timed = fr.graph_instantiations
+        # the same frames again, eagerly, for the per-stage and per-kernel breakdown
+        # (the accumulator history differs; the traced rays do not depend on it)
+        eager_ms, stage_ms, timing = eager_pass(args.warmup)
+        graph_info = {"frames_per_graph": 1, "kernel_nodes": fr.graph_nodes,
+                      "instantiations": inst_timed, "eager_ms_per_step": round(eager_ms / K, 3)}
+    else:
+        total_ms, stage_ms, timing = eager_pass(args.warmup)
     ms_per_step = total_ms / K
 
     # rays and traversal work of exactly the timed frames: the counting variant of the
@@ -397,7 +269,7 @@ def main():
         dom, dom_bytes, dom_ms = "pt_megakernel", ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
         kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    traffic, traffic_src = measured_traffic(dom)
+    traffic, traffic_src = measured_traffic(dom, f"{cfg['tag']}-{bvh}")
     n_launch = timing.extend_launches if fr.tracer == capi.TRACER_WAVEFRONT else timing.megakernel_launches
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
@@ -420,7 +292,8 @@ def main():
         "data": "synthetic (seed-generated scene, random-free camera spline)",
         "config": {"workload": cfg["name"], "width": w, "height": h, "spp": 1, "max_depth": cfg["depth"],
                    "triangles": info.triangle_count, "lights": info.light_count, "tracer": args.tracer,
-                   "denoiser": args.denoiser, "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"]},
+                   "denoiser": args.denoiser, "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"],
+                   "launch": "hipgraph per frame" if use_graph else "eager"},
         "stage_ms": {"trace": round(stage_ms[0], 3), "accumulate": round(stage_ms[1], 3),
                      "denoise": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
         "rays_per_frame": int(rays_per_frame),
@@ -435,6 +308,8 @@ def main():
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
     }
+    if graph_info:
+        out["hipgraph"] = graph_info
     if tuning:
         out["autotune"] = tuning
         out["config"]["tracer"] = tuning["best"]["tracer"]
@@ -443,6 +318,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds, range(args.warmup, args.warmup + K))
     print(json.dumps(out))
+    if graph is not None:
+        capi.check(lib.pbrt_frame_graph_destroy(graph))
 
 
 if __name__ == "__main__":

@@ -519,6 +519,26 @@ int32_t pbrt_cnn_gradients(const pbrt_cnn* net, const pbrt_cnn_images* images, f
 int32_t pbrt_cnn_train_step(pbrt_cnn* net, const pbrt_cnn_images* images, float learning_rate, float* loss,
                             void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Frame graph: replaces the per-frame recordAndSubmit of the recorded command */
+/* graph (source/VulkanPBRT.cpp:543,588).  begin() starts capturing, on        */
+/* `stream`, the launches the calling thread issues there (every entry point  */
+/* of a frame: trace, accumulate, denoise, TAA, convert); end() closes the     */
+/* capture, updates the executable graph in place when the launch topology is  */
+/* unchanged (kernel arguments such as the camera change every frame) or       */
+/* instantiates it anew, and launches it on `stream`.  `stream` must not be    */
+/* the null stream.  The frame entry points issue no memset, host copy or      */
+/* synchronisation, so a whole frame captures into one graph.                  */
+/* ------------------------------------------------------------------------- */
+typedef struct pbrt_frame_graph pbrt_frame_graph;
+
+int32_t pbrt_frame_graph_create(pbrt_frame_graph** out_graph);
+int32_t pbrt_frame_graph_destroy(pbrt_frame_graph* graph); /* ends an open capture unlaunched */
+int32_t pbrt_frame_graph_begin(pbrt_frame_graph* graph, void* stream);
+/* node_count (may be NULL): kernel nodes of the captured frame; instantiated (may be
+ * NULL): 1 when the executable graph was rebuilt rather than updated. */
+int32_t pbrt_frame_graph_end(pbrt_frame_graph* graph, void* stream, uint32_t* node_count, uint32_t* instantiated);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
