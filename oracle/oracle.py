@@ -56,11 +56,17 @@ def lib():
         L.orc_bfr.argtypes = [C.c_uint32] * 4 + [C.c_int32] + [vp] * 9
         L.orc_bfr_blend.argtypes = [C.c_uint32] * 3 + [vp] * 6
         L.orc_bmfr_post.argtypes = [C.c_uint32] * 4 + [vp] * 9
+        L.orc_brdf_probe.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.orc_sample_brdf_probe.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp]
+        L.orc_sample_light_probe.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, vp]
+        L.orc_shade_probe.argtypes = [vp] * 4
+        L.orc_shade_probe.restype = C.c_int32
         _lib = L
     return _lib
 
 
 def _p(a):
+    """Address of an array's data; the caller keeps the array alive across the call."""
     return None if a is None else a.ctypes.data
 
 
@@ -126,6 +132,27 @@ class OracleScene:
         return dict(depth=depth, normal=normal, material=material, albedo=albedo, illumination=illum,
                     radiance=radiance, rays=int(stats[0]), shadow_rays=int(stats[1]))
 
+    def sample_light(self, mode, pos, normal, count, frame=1):
+        """`count` draws of sampleLight (lighting.glsl:18-227) at `pos` with `normal`:
+        dict of ok (bool), col (n,3), l (n,3), pdf, tmax."""
+        out = np.zeros((count, 9), np.float64)
+        pos, normal = np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(normal, np.float64)
+        lib().orc_sample_light_probe(self.handle, mode, _p(pos), _p(normal), count, frame, _p(out))
+        return dict(ok=out[:, 0] > 0, col=out[:, 1:4], l=out[:, 4:7], pdf=out[:, 7], tmax=out[:, 8])
+
+    def shade(self, origin, direction):
+        """The surface the closest-hit shader (ptClosesthit.rchit:18-92) builds at the
+        closest hit of one ray, or None on a miss."""
+        out = np.zeros(34, np.float64)
+        o, d = np.ascontiguousarray(origin, np.float64), np.ascontiguousarray(direction, np.float64)
+        tri = lib().orc_shade_probe(self.handle, _p(o), _p(d), _p(out))
+        if tri < 0:
+            return None
+        return dict(tri=tri, position=out[0:3], normal=out[3:6], diffuse=out[6:9], r0=out[9:12], r90=out[12],
+                    alpha_roughness=out[13], metalness=out[14], emissive=out[15:18],
+                    basis=out[18:27].reshape(3, 3), perceptual_roughness=out[27], t=out[28], u=out[29],
+                    v=out[30], specular=out[31:34])
+
     def __del__(self):
         try:
             if self.handle:
@@ -133,6 +160,44 @@ class OracleScene:
                 self.handle = None
         except Exception:
             pass
+
+
+# --------------------------------------------------------------------------
+# Single shading functions (brdf.glsl, sampling.glsl), FP64
+# --------------------------------------------------------------------------
+def surface(perceptual_roughness=0.5, metalness=0.0, alpha_roughness=None, r0=(0.04,) * 3, r90=(1.0,) * 3,
+            diffuse=(0.5,) * 3, specular=(0.04,) * 3, normal=(0, 0, 1), basis=None):
+    """The 27-double surface record of the probes; basis rows = tangent, bitangent, normal."""
+    if alpha_roughness is None:
+        alpha_roughness = perceptual_roughness ** 2
+    n = np.asarray(normal, np.float64)
+    if basis is None:
+        t = np.cross(n, [0.0, 1.0, 0.0] if abs(n[0]) > 0.9 else [1.0, 0.0, 0.0])
+        t /= np.linalg.norm(t)
+        b = np.cross(n, t)
+        basis = np.stack([t, b, n])
+    return np.concatenate([[perceptual_roughness, metalness, alpha_roughness], r0, r90, diffuse, specular, n,
+                           np.asarray(basis, np.float64).ravel()]).astype(np.float64)
+
+
+def brdf(surf, v, l):
+    """BRDF (brdf.glsl:132-164), pdf_brdf(l, normalize(v + l)) and the specular weight for
+    direction pairs v, l of shape (3,) or (n, 3): (brdf (n,3), pdf (n), spec_weight (n))."""
+    v = np.ascontiguousarray(np.atleast_2d(v), np.float64)
+    l = np.ascontiguousarray(np.atleast_2d(l), np.float64)
+    v, l = np.broadcast_arrays(v, l)
+    v, l = np.ascontiguousarray(v), np.ascontiguousarray(l)
+    out = np.zeros((v.shape[0], 5), np.float64)
+    lib().orc_brdf_probe(_p(surf), v.shape[0], _p(v), _p(l), _p(out))
+    return out[:, :3], out[:, 3], out[:, 4]
+
+
+def sample_brdf(surf, v, count, frame=1):
+    """`count` draws of sampleBRDF (sampling.glsl:11-78): l (n,3), pdf (n), brdf (n,3)."""
+    out = np.zeros((count, 7), np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    lib().orc_sample_brdf_probe(_p(surf), _p(v), count, frame, _p(out))
+    return out[:, 0:3], out[:, 3], out[:, 4:7]
 
 
 # --------------------------------------------------------------------------

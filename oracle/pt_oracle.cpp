@@ -1040,6 +1040,124 @@ void orc_intersect(void* s, const float* rays, uint32_t n, int32_t brute, int32_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Probes of single shading functions (FP64), for pinning the oracle against
+// closed forms and distributions in its own tests.
+// ---------------------------------------------------------------------------
+// surf: perceptual_roughness, metalness, alpha_roughness, r0[3], r90[3],
+// diffuse[3], specular[3], normal[3], basis[3][3] (27 doubles); illum 0.
+static Surface<double> probe_surface(const double* f) {
+    Surface<double> s{};
+    s.perceptual_roughness = f[0];
+    s.metalness = f[1];
+    s.alpha_roughness = f[2];
+    s.r0 = V3<double>(f[3], f[4], f[5]);
+    s.r90 = V3<double>(f[6], f[7], f[8]);
+    s.diffuse = V3<double>(f[9], f[10], f[11]);
+    s.specular = V3<double>(f[12], f[13], f[14]);
+    s.normal = V3<double>(f[15], f[16], f[17]);
+    for (int k = 0; k < 3; ++k) s.basis[k] = V3<double>(f[18 + 3 * k], f[19 + 3 * k], f[20 + 3 * k]);
+    s.illum = 0;
+    s.ior = 1;
+    return s;
+}
+
+// For each of `count` direction pairs (v[i], l[i]): brdf rgb, pdf_brdf(l, h) and
+// spec_weight, h = normalize(v + l); out holds 5 doubles per pair.
+void orc_brdf_probe(const double* surf, uint32_t count, const double* v, const double* l, double* out) {
+    Surface<double> s = probe_surface(surf);
+    for (uint32_t i = 0; i < count; ++i) {
+        V3<double> vv(v[3 * i], v[3 * i + 1], v[3 * i + 2]), ll(l[3 * i], l[3 * i + 1], l[3 * i + 2]);
+        V3<double> h = normalize(vv + ll);
+        V3<double> b = brdf(vv, ll, h, s);
+        double* o = out + 5 * (size_t)i;
+        o[0] = b.x;
+        o[1] = b.y;
+        o[2] = b.z;
+        o[3] = pdf_brdf(s, ll, h);
+        o[4] = spec_weight(s);
+    }
+}
+
+// `count` draws of sampleBRDF from the engines rEInit(i % 4096, i / 4096, frame);
+// out per draw: l[3], pdf, brdf rgb.
+void orc_sample_brdf_probe(const double* surf, const double* v, uint32_t count, uint32_t frame, double* out) {
+    Surface<double> s = probe_surface(surf);
+    V3<double> vv(v[0], v[1], v[2]);
+    for (uint32_t i = 0; i < count; ++i) {
+        V3<double> l;
+        double pdf = 0;
+        V3<double> b = sample_brdf(s, re_init(i % 4096u, i / 4096u, frame), vv, l, pdf);
+        double* o = out + 7 * (size_t)i;
+        o[0] = l.x;
+        o[1] = l.y;
+        o[2] = l.z;
+        o[3] = pdf;
+        o[4] = b.x;
+        o[5] = b.y;
+        o[6] = b.z;
+    }
+}
+
+// `count` draws of sampleLight (mode: 0 surface strength, 1 light strength,
+// 2 uniform) at pos with normal n; out per draw: ok, col[3], l[3], pdf, tmax.
+void orc_sample_light_probe(void* scene, uint32_t mode, const double* pos, const double* n, uint32_t count,
+                            uint32_t frame, double* out) {
+    const SceneO& sc = *(const SceneO*)scene;
+    V3<double> p(pos[0], pos[1], pos[2]), nn(n[0], n[1], n[2]);
+    for (uint32_t i = 0; i < count; ++i) {
+        RandomEngine re = re_init(i % 4096u, i / 4096u, frame);
+        V3<double> col(0.0), l(0.0);
+        double pdf = 0, tmax = 0;
+        bool ok = sample_light(sc, mode, p, nn, re, col, l, pdf, tmax);
+        double* o = out + 9 * (size_t)i;
+        o[0] = ok ? 1.0 : 0.0;
+        o[1] = col.x;
+        o[2] = col.y;
+        o[3] = col.z;
+        o[4] = l.x;
+        o[5] = l.y;
+        o[6] = l.z;
+        o[7] = pdf;
+        o[8] = tmax;
+    }
+}
+
+// Closest hit of the ray (o, d) (brute force) and the surface the closest-hit
+// shader builds there.  Returns the triangle or -1.  out (34 doubles): position[3],
+// normal[3], diffuse[3], r0[3], r90, alpha_roughness, metalness, emissive[3],
+// basis[3][3], perceptual_roughness, t, u, v, specular[3].
+int32_t orc_shade_probe(void* scene, const double* o, const double* d, double* out) {
+    const SceneO& sc = *(const SceneO*)scene;
+    HitO h = query(sc, o, d, 1e-4, 1e30, false, true);
+    if (h.tri < 0) return -1;
+    Surface<double> s{};
+    V3<double> position;
+    uint32_t cat = 0;
+    closest_hit<double>(sc, h, V3<double>(d[0], d[1], d[2]), s, position, cat);
+    double* q = out;
+    auto put = [&](V3<double> v) {
+        *q++ = v.x;
+        *q++ = v.y;
+        *q++ = v.z;
+    };
+    put(position);
+    put(s.normal);
+    put(s.diffuse);
+    put(s.r0);
+    *q++ = s.r90.x;
+    *q++ = s.alpha_roughness;
+    *q++ = s.metalness;
+    put(s.emissive);
+    for (int k = 0; k < 3; ++k) put(s.basis[k]);
+    *q++ = s.perceptual_roughness;
+    *q++ = h.t;
+    *q++ = h.u;
+    *q++ = h.v;
+    put(s.specular);
+    return h.tri;
+}
+
 // Trace rows [y0, y1) of a frame.  precision 32 = FP32 mirror, 64 = FP64.
 // Outputs are full-size images (only the traced rows are written):
 // depth W*H, normal 2*W*H, material/albedo 4*W*H bytes, illumination 4*W*H,
