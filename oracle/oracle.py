@@ -56,7 +56,7 @@ def lib():
         L.orc_bfr.argtypes = [C.c_uint32] * 4 + [C.c_int32] + [vp] * 9
         L.orc_bfr_blend.argtypes = [C.c_uint32] * 3 + [vp] * 6
         L.orc_bmfr_post.argtypes = [C.c_uint32] * 4 + [vp] * 9
-        L.orc_brdf_probe.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.orc_brdf_probe.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
         L.orc_sample_brdf_probe.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp]
         L.orc_sample_light_probe.argtypes = [vp, C.c_uint32, vp, vp, C.c_uint32, C.c_uint32, vp]
         L.orc_shade_probe.argtypes = [vp] * 4
@@ -180,15 +180,15 @@ def surface(perceptual_roughness=0.5, metalness=0.0, alpha_roughness=None, r0=(0
                            np.asarray(basis, np.float64).ravel()]).astype(np.float64)
 
 
-def brdf(surf, v, l):
-    """BRDF (brdf.glsl:132-164), pdf_brdf(l, normalize(v + l)) and the specular weight for
-    direction pairs v, l of shape (3,) or (n, 3): (brdf (n,3), pdf (n), spec_weight (n))."""
-    v = np.ascontiguousarray(np.atleast_2d(v), np.float64)
-    l = np.ascontiguousarray(np.atleast_2d(l), np.float64)
-    v, l = np.broadcast_arrays(v, l)
-    v, l = np.ascontiguousarray(v), np.ascontiguousarray(l)
-    out = np.zeros((v.shape[0], 5), np.float64)
-    lib().orc_brdf_probe(_p(surf), v.shape[0], _p(v), _p(l), _p(out))
+def brdf(surf, v, l, h=None):
+    """BRDF (brdf.glsl:132-164), pdf_brdf(l, h) and the specular weight for direction
+    pairs v, l of shape (3,) or (n, 3), with h = normalize(v + l) unless given:
+    (brdf (n,3), pdf (n), spec_weight (n))."""
+    arrs = [np.atleast_2d(np.asarray(x, np.float64)) for x in ((v, l) if h is None else (v, l, h))]
+    arrs = [np.ascontiguousarray(x) for x in np.broadcast_arrays(*arrs)]
+    n = arrs[0].shape[0]
+    out = np.zeros((n, 5), np.float64)
+    lib().orc_brdf_probe(_p(surf), n, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]) if h is not None else None, _p(out))
     return out[:, :3], out[:, 3], out[:, 4]
 
 

@@ -1063,12 +1063,14 @@ static Surface<double> probe_surface(const double* f) {
 }
 
 // For each of `count` direction pairs (v[i], l[i]): brdf rgb, pdf_brdf(l, h) and
-// spec_weight, h = normalize(v + l); out holds 5 doubles per pair.
-void orc_brdf_probe(const double* surf, uint32_t count, const double* v, const double* l, double* out) {
+// spec_weight, with h = h_in[i] or, when h_in is NULL, normalize(v + l); out holds
+// 5 doubles per pair.
+void orc_brdf_probe(const double* surf, uint32_t count, const double* v, const double* l, const double* h_in,
+                    double* out) {
     Surface<double> s = probe_surface(surf);
     for (uint32_t i = 0; i < count; ++i) {
         V3<double> vv(v[3 * i], v[3 * i + 1], v[3 * i + 2]), ll(l[3 * i], l[3 * i + 1], l[3 * i + 2]);
-        V3<double> h = normalize(vv + ll);
+        V3<double> h = h_in ? V3<double>(h_in[3 * i], h_in[3 * i + 1], h_in[3 * i + 2]) : normalize(vv + ll);
         V3<double> b = brdf(vv, ll, h, s);
         double* o = out + 5 * (size_t)i;
         o[0] = b.x;

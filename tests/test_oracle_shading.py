@@ -485,3 +485,143 @@ def test_emission_only_from_the_front(oracle_lib):
     back = sc.shade([0.5, 0.5, -1.0], [0, 0, 1.0])
     np.testing.assert_allclose(front["emissive"], [2.0, 1.0, 0.5], rtol=1e-7)
     np.testing.assert_array_equal(back["emissive"], [0.0, 0.0, 0.0])
+
+
+# ---------------------------------------------------------------------------
+# the path: next-event estimation with MIS and the first bounce (R10, R11)
+# ---------------------------------------------------------------------------
+def sky(d):
+    """The sky colour of lighting.glsl:231-240 (the environment a missed ray sees)."""
+    up = np.array([0.3, 0.5, 0.92]) ** 2.2
+    up = 1.0 + (up - 1.0) * max(d[2], 0.0)
+    t = np.clip((d[2] + 0.02) / 0.04, 0, 1)
+    w = t * t * (3 - 2 * t)
+    return 0.2 + (up - 0.2) * w
+
+
+def sky_many(d):
+    return np.stack([sky(x) for x in d]) if len(d) else np.zeros((0, 3))
+
+
+class FloorUnderSun:
+    """A large rough floor (z = 0) lit by one directional light; every pixel of the
+    frame traces the same camera ray (the projection ignores the pixel position), so
+    each pixel of each frame is one independent sample of the same path."""
+
+    light_dir = unit([0.3, 0.2, 1.0])  # towards the light
+    light_col = np.array([6.0, 4.8, 3.6])  # bright enough that NEE outweighs any negative bounce term
+    cam = np.array([0.3, -0.2, 3.0])
+    view_dir = unit([0.1, 0.2, -1.0])
+
+    def __init__(self, occluder=False):
+        floor = dict(positions=[[-50, -50, 0], [50, -50, 0], [50, 50, 0], [-50, 50, 0]], normals=[[0, 0, 1]] * 4,
+                     uvs=[[0, 0], [1, 0], [1, 1], [0, 1]], indices=[0, 1, 2, 0, 2, 3], diffuse=(0.6, 0.5, 0.4),
+                     roughness=0.6)
+        meshes = [floor]
+        self.hit = self.cam + self.view_dir * (-self.cam[2] / self.view_dir[2])
+        if occluder:
+            c = self.hit + self.light_dir * 1.0
+            q = 0.1
+            meshes.append(dict(positions=[c + [-q, -q, 0], c + [q, -q, 0], c + [q, q, 0], c + [-q, q, 0]],
+                               indices=[0, 1, 2, 0, 2, 3]))
+        sun = capi.Light()
+        sun.v0_type[:] = [0.0, 0.0, 0.0, 1.0]
+        sun.dir_angle2[:] = list(-self.light_dir) + [0.0]
+        sun.col_diffuse[:] = list(self.light_col) + [0.0]
+        sun.strengths[:] = [1.0, 0.0, 0.0, 0.0]
+        self.d = Desc(meshes, lights=[sun])
+        self.sc = oracle.OracleScene(self.d.desc)
+        z = self.view_dir
+        x = unit(np.cross(z, [0.0, 0.0, 1.0]))
+        y = np.cross(z, x)
+        m = np.eye(4)
+        m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = x, y, z, self.cam
+        self.inv_view = m.T.ravel().astype(np.float32)  # column-major
+        p = np.zeros((4, 4))
+        p[2, 2] = p[3, 3] = 1.0  # every pixel maps to view direction +z
+        self.inv_proj = p.T.ravel().astype(np.float32)
+
+    def surface(self):
+        s = self.sc.shade(self.cam, self.view_dir)
+        np.testing.assert_allclose(s["position"], self.hit, atol=1e-5)
+        rec = oracle.surface(perceptual_roughness=s["perceptual_roughness"], metalness=s["metalness"],
+                             alpha_roughness=s["alpha_roughness"], r0=s["r0"], r90=(s["r90"],) * 3,
+                             diffuse=s["diffuse"], specular=s["specular"], normal=s["normal"], basis=s["basis"])
+        return rec, s
+
+    def trace(self, frames, max_depth, light_mode=2, size=32):
+        out = []
+        for f in frames:
+            r = self.sc.trace(size, size, self.inv_view, self.inv_proj, frame=f, max_depth=max_depth,
+                              light_mode=light_mode, precision=64, threads=8)
+            out.append(r["radiance"].reshape(-1, 3))
+        return np.concatenate(out)
+
+
+@pytest.mark.parametrize("light_mode", [0, 1, 2])
+def test_nee_with_mis_closed_form(oracle_lib, light_mode):
+    """No bounce (max depth 0): the pixel is the light's term times the BRDF times the
+    power-heuristic weight of the light's selection probability (1, one light) against
+    the BRDF's pdf: L cos B / (1 + pdf_B^2); zero where an occluder blocks the light."""
+    scene = FloorUnderSun()
+    rec, _ = scene.surface()
+    v = -scene.view_dir
+    b, pdf_b, _ = oracle.brdf(rec, v, scene.light_dir)
+    want = scene.light_col * scene.light_dir[2] * b[0] / (1.0 + pdf_b[0] ** 2)
+    got = scene.trace([1], 0, light_mode, size=2)
+    # the light, camera and vertices reach the oracle as fp32 values
+    np.testing.assert_allclose(got, np.tile(want, (4, 1)), rtol=1e-6)
+    shadowed = FloorUnderSun(occluder=True).trace([1], 0, light_mode, size=2)
+    np.testing.assert_array_equal(shadowed, 0.0)
+
+
+def test_first_bounce_expectation(oracle_lib):
+    """One bounce to the sky (max depth 1): a path's radiance is NEE + sky(l) B(l) cos(l) /
+    pdf(l), clamped per channel to [0, 10]; its mean over many paths equals the
+    expectation of that under sampleBRDF's mixture - the specular lobe's half vectors
+    distributed as D cos (integrated over h, with l the mirror of v) weighted by the
+    specular weight, the diffuse lobe's cosine-weighted directions by the rest; draws
+    whose pdf falls under 1e-6 end the path after NEE.  (Mirrored directions below the
+    horizon get a negative cosine and the BRDF pdf can be tiny there, so the clamp
+    matters for a few paths in 10^4.)"""
+    scene = FloorUnderSun()
+    rec, s = scene.surface()
+    v = -scene.view_dir
+    b, pdf_b, sw = oracle.brdf(rec, v, scene.light_dir)
+    nee = scene.light_col * scene.light_dir[2] * b[0] / (1.0 + pdf_b[0] ** 2)
+    sw = sw[0]
+    basis = s["basis"]  # rows t, b, n
+    alpha = s["alpha_roughness"]
+
+    def grid(nt, nphi, power):
+        st = (np.arange(nt) + 0.5) / nt
+        theta = st ** power * (np.pi / 2)
+        dtheta = power * st ** (power - 1) * (np.pi / 2) / nt
+        phi = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+        T, P = np.meshgrid(theta, phi, indexing="ij")
+        local = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+        dw = (np.sin(T) * dtheta[:, None] * (2 * np.pi / nphi)).reshape(-1)
+        return local @ basis, np.cos(T).reshape(-1), dw
+
+    def path_radiance(l, h):
+        f, pdf, _ = oracle.brdf(rec, np.tile(v, (len(l), 1)), l, h)
+        cos_l = l @ basis[2]
+        keep = pdf >= 1e-6
+        val = np.tile(nee, (len(l), 1))
+        val[keep] += sky_many(l[keep]) * f[keep] * cos_l[keep, None] / pdf[keep, None]
+        return np.clip(val, 0.0, 10.0)
+
+    h, cos_h, dw = grid(1200, 256, 2)
+    w_spec = ggx_d(cos_h, alpha) * cos_h * dw
+    assert abs(w_spec.sum() - 1) < 1e-4
+    l = 2 * (h @ v)[:, None] * h - v
+    spec = path_radiance(l, h).T @ w_spec
+    l, cos_l, dw = grid(600, 256, 1)
+    w_diff = cos_l / np.pi * dw
+    assert abs(w_diff.sum() - 1) < 1e-4
+    diff = path_radiance(l, unit(v + l)).T @ w_diff
+    want = sw * spec + (1 - sw) * diff
+    got = scene.trace(range(1, 65), 1)  # 64 frames x 1024 pixels
+    mean, se = got.mean(0), got.std(0) / np.sqrt(len(got))
+    print("first bounce: mean", mean, "+-", se, "expected", want, "(NEE", nee, "sw", sw, ")")
+    assert np.all(np.abs(mean - want) < 5 * se + 2e-4 * want)
