@@ -242,3 +242,30 @@ def test_autotune_picks_a_measured_candidate(cuda, cfg1):
     best = int(np.argmin(ms))
     assert (tun.best.tracer, tun.best.block_threads, tun.best.queue_paths) == \
         (tun.tracer[best], tun.block_threads[best], tun.queue_paths[best])
+
+
+@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("max_depth", [0, 1])
+def test_tracer_on_the_pinned_floor_scene(cuda, pbrt_lib, oracle_lib, tracer, max_depth):
+    """The scene whose NEE and first bounce the oracle pins in closed form
+    (tests/test_oracle_shading.py): the product's frame equals the oracle's FP32 mirror
+    pixel by pixel, every pixel being a fresh path along the same camera ray."""
+    from tests.test_oracle_shading import FloorUnderSun
+    fl = FloorUnderSun()
+    scene = capi.Scene(fl.d.desc)
+    n = 32
+    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    for frame in (1, 2):
+        pc = capi.PushConstants()
+        pc.view_inverse[:] = fl.inv_view.tolist()
+        pc.proj_inverse[:] = fl.inv_proj.tolist()
+        pc.prev_view[:] = np.eye(4, dtype=np.float32).ravel().tolist()
+        pc.frame_number = frame
+        img = helpers.gbuffer(cuda, n, n)
+        helpers.trace(cuda, scene, pc, img, n, n, max_depth=max_depth, tracer=kind)
+        cuda.cuda.synchronize()
+        gpu = helpers.to_np(img)
+        ref = fl.sc.trace(n, n, fl.inv_view, fl.inv_proj, frame=frame, max_depth=max_depth, precision=32)
+        frac = _compare_frame(gpu, ref, f"{tracer} floor depth {max_depth} frame {frame}")
+        assert frac["depth"] == 1.0 and frac["albedo"] == 1.0
+        assert frac["radiance"] >= (1.0 if max_depth == 0 else 0.99)
