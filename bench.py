@@ -25,6 +25,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 32.0  # one quantized BVH2 node
 
 
 def parse():
@@ -251,10 +252,10 @@ def main():
     nominal = 1000.0 / ms_per_step * (2 * cfg["depth"]) * w * h / 1e6  # Gui.hpp:41-43 formula
 
     # roofline of the dominant kernel.  Algorithmic bytes: 48 B ray I/O per traced
-    # ray + 64 B per BVH node visit + 48 B per triangle record tested (+ 36 B/px of
-    # G-buffer and illumination output for the megakernel, which also shades).
-    ext_bytes = 48.0 * cnt[0] + 64.0 * cnt[2] + 48.0 * cnt[3]
-    sh_bytes = 48.0 * cnt[1] + 64.0 * cnt[4] + 48.0 * cnt[5]
+    # ray + 32 B per (quantized) BVH node visit + 48 B per triangle record tested
+    # (+ 36 B/px of G-buffer and illumination output for the megakernel, which also shades).
+    ext_bytes = 48.0 * cnt[0] + NODE_BYTES * cnt[2] + 48.0 * cnt[3]
+    sh_bytes = 48.0 * cnt[1] + NODE_BYTES * cnt[4] + 48.0 * cnt[5]
     kernels = {}
     if fr.tracer == capi.TRACER_WAVEFRONT:
         for name, b, ms, n in (("k_wf_extend", ext_bytes, timing.extend_ms, timing.extend_launches),

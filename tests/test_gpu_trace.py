@@ -206,7 +206,10 @@ def test_trace_timing_records_every_stage(cuda, cfg1):
 @pytest.mark.parametrize("block,queue", [(128, 0), (256, 0), (64, 3000), (256, 7777)])
 def test_wavefront_launch_shapes_give_identical_frames(cuda, cfg1, block, queue):
     """Traversal workgroup size and frame slicing change only the launch shape: every
-    path runs the same arithmetic, so the frame is bit-identical."""
+    path runs the same arithmetic, so the frame is bit-identical and so are the ray
+    counts.  Node and triangle visit counts depend slightly on which rays share a
+    wave (a lane sets its first leaf aside until every lane of the wave holds one),
+    and queue order follows the atomics, so they agree to 0.5 %."""
     gen, scene, _ = cfg1
     pc, _, _ = helpers.frame_push_constants(gen, W, H, 3)
     ref = helpers.gbuffer(cuda, W, H)
@@ -220,7 +223,10 @@ def test_wavefront_launch_shapes_give_identical_frames(cuda, cfg1, block, queue)
     cuda.cuda.synchronize()
     for k in ref:
         assert cuda.equal(ref[k], img[k]), k
-    assert cuda.equal(cnt0, cnt1)
+    c0, c1 = cnt0.cpu().numpy().astype(float), cnt1.cpu().numpy().astype(float)
+    assert c0[0] == c1[0] and c0[1] == c1[1]
+    assert abs(c0[2:] - c1[2:]).max() <= 5e-3 * c0[2:].max()
+    assert (abs(c0[2:] - c1[2:]) <= 5e-3 * c0[2:]).all()
 
 
 def test_autotune_picks_a_measured_candidate(cuda, cfg1):

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "../../include/pbrt_c.h"
 
@@ -83,6 +84,10 @@ struct DeviceBuffer {
         if (n == 0) return;
         PBRT_HIP(hipMalloc(&ptr, n));
         bytes = n;
+    }
+    void swap(DeviceBuffer& other) {
+        std::swap(ptr, other.ptr);
+        std::swap(bytes, other.bytes);
     }
     template <class T>
     T* as() const {

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "bvh_host.hpp"
+#include "bvh_quant.hpp"
 #include "lbvh.hpp"
 #include "capi_common.hpp"
 #include "scene_internal.hpp"
@@ -419,6 +420,7 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
                 (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             s->info.bvh_builder = PBRT_BVH_SAH_HOST;
         }
+        quantize_bvh_nodes(s->nodes, node_count);
         s->instances.upload(ps.instances.data(), ps.instances.size() * sizeof(DeviceInstance));
         s->meshes.upload(ps.meshes.data(), ps.meshes.size() * sizeof(DeviceMesh));
         s->materials.upload(ps.materials.data(), ps.materials.size() * sizeof(DeviceMaterial));
@@ -432,7 +434,7 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
 
         DeviceScene& dv = s->dev;
         dv.tris = s->tris.as<float4>();
-        dv.nodes = s->nodes.as<float4>();
+        dv.nodes = s->nodes.as<uint4>();
         dv.instances = s->instances.as<DeviceInstance>();
         dv.meshes = s->meshes.as<DeviceMesh>();
         dv.materials = s->materials.as<DeviceMaterial>();

@@ -3,9 +3,12 @@
 //   tris      3 x float4 per BVH-ordered world-space triangle:
 //             {v0.xyz, instance index bits}, {e1 = v1 - v0, primitive index bits},
 //             {e2 = v2 - v0, flags (bit 0: instance is alpha tested)}
-//   nodes     BVH2, 4 x float4 per node: child 0/1 bounds as
-//             {c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y}, {c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y},
-//             {c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z}, {child0, child1, 0, 0} as int bits.
+//   nodes     BVH2, 2 x uint4 per node (quantized, see bvh_quant.hip): per-axis
+//             origin + scale exponent, child 0 reference, 8-bit child bounds,
+//             child 1 reference.  The builders emit full-precision nodes of
+//             4 x float4 first: {c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y},
+//             {c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y}, {c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z},
+//             {child0, child1, 0, 0} as int bits.
 //             child >= 0: inner node index; child < 0: leaf ~(first_tri << 4 | (count - 1)).
 //   instances per instance: object matrix, normal matrix, mesh id, index stride, flags
 //   meshes    per mesh: offsets into the concatenated vertex/uv/index pools + texture ids
@@ -71,7 +74,7 @@ struct DeviceTexture {
 
 struct DeviceScene {
     const float4* tris;
-    const float4* nodes;
+    const uint4* nodes;  // quantized BVH2
     const DeviceInstance* instances;
     const DeviceMesh* meshes;
     const DeviceMaterial* materials;

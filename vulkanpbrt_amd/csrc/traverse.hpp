@@ -10,6 +10,8 @@
 // the first accepted candidate.
 #pragma once
 
+#include <climits>
+
 #include "pt_device.hpp"
 #include "scene_types.hpp"
 
@@ -112,94 +114,160 @@ __device__ __forceinline__ float safe_rcp(float d) {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
 }
 
+// A ray prepared for slab tests: origin, direction, reciprocal direction and
+// origin * reciprocal.
+struct TraversalRay {
+    vec3 o, d;
+    float ix, iy, iz, ox, oy, oz, tmin;
+};
+
+__device__ __forceinline__ TraversalRay make_traversal_ray(vec3 o, vec3 d, float tmin) {
+    TraversalRay r;
+    r.o = o;
+    r.d = d;
+    r.ix = safe_rcp(d.x);
+    r.iy = safe_rcp(d.y);
+    r.iz = safe_rcp(d.z);
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    r.tmin = tmin;
+    return r;
+}
+
+// Lane holds no node to visit: its stack is empty (or it has no ray).  Leaf
+// codes are ~(first << kLeafCountBits | count - 1) with first + count < 2^27,
+// so INT_MIN is never a leaf.
+constexpr int kNodeDone = INT_MIN;
+
+__device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
+    if (sp == 0) return kNodeDone;
+    --sp;
+    return stack[sp * stride];
+}
+
+// One inner node (quantized layout, 2 x uint4): dequantize both child boxes
+// (bound = fma(q, 2^e, origin), exactly as the quantization pass checked them)
+// and slab-test them against [tmin, tcur]; returns the nearer hit child (the
+// farther one pushed) or the next node off the stack.
+__device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
+                                           float tcur, int* stack, int stride, int& sp) {
+    const uint4 na = nodes[2 * node + 0];
+    const uint4 nb = nodes[2 * node + 1];
+    const float gx = __uint_as_float(na.x & ~0xFFu), gy = __uint_as_float(na.y & ~0xFFu),
+                gz = __uint_as_float(na.z & ~0xFFu);
+    const float sx = __uint_as_float((na.x & 0xFFu) << 23), sy = __uint_as_float((na.y & 0xFFu) << 23),
+                sz = __uint_as_float((na.z & 0xFFu) << 23);
+    auto q = [](uint32_t w, int byte) { return (float)((w >> (8 * byte)) & 0xFFu); };
+    const float l0x = fmaf(q(nb.x, 0), sx, gx), l0y = fmaf(q(nb.x, 1), sy, gy), l0z = fmaf(q(nb.x, 2), sz, gz);
+    const float h0x = fmaf(q(nb.x, 3), sx, gx), h0y = fmaf(q(nb.y, 0), sy, gy), h0z = fmaf(q(nb.y, 1), sz, gz);
+    const float l1x = fmaf(q(nb.y, 2), sx, gx), l1y = fmaf(q(nb.y, 3), sy, gy), l1z = fmaf(q(nb.z, 0), sz, gz);
+    const float h1x = fmaf(q(nb.z, 1), sx, gx), h1y = fmaf(q(nb.z, 2), sy, gy), h1z = fmaf(q(nb.z, 3), sz, gz);
+    // child 0
+    float a0 = l0x * r.ix - r.ox, a1 = h0x * r.ix - r.ox;
+    float b0 = l0y * r.iy - r.oy, b1 = h0y * r.iy - r.oy;
+    float c0 = l0z * r.iz - r.oz, c1 = h0z * r.iz - r.oz;
+    float near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
+    float far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
+    // child 1
+    float d0 = l1x * r.ix - r.ox, d1 = h1x * r.ix - r.ox;
+    float e0 = l1y * r.iy - r.oy, e1 = h1y * r.iy - r.oy;
+    float f0 = l1z * r.iz - r.oz, f1 = h1z * r.iz - r.oz;
+    float near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
+    float far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
+    bool h0 = near0 <= far0;
+    bool h1 = near1 <= far1;
+    int ch0 = (int)na.w;
+    int ch1 = (int)nb.w;
+    if (h0 && h1) {
+        bool swap = near1 < near0;
+        stack[sp * stride] = swap ? ch0 : ch1;
+        ++sp;
+        return swap ? ch1 : ch0;
+    }
+    if (h0) return ch0;
+    if (h1) return ch1;
+    return pop_node(stack, stride, sp);
+}
+
+// The triangles of one leaf (Moller-Trumbore, alpha test on alpha-tested
+// instances); closest hit keeps the nearest accepted t, any-hit returns true on
+// the first accepted candidate.
+template <bool kAnyHit, bool kCount>
+__device__ __forceinline__ bool test_leaf(const DeviceScene& sc, int node, const TraversalRay& r, Hit& hit,
+                                          TraversalCounters& cnt) {
+    const float4* __restrict__ tris = sc.tris;
+    const vec3 o = r.o, d = r.d;
+    uint32_t code = ~(uint32_t)node;
+    uint32_t first = code >> kLeafCountBits;
+    uint32_t count = (code & (kMaxLeafSize - 1)) + 1;
+    for (uint32_t k = first; k < first + count; ++k) {
+        if (kCount) cnt.tris++;
+        const float4 t0 = tris[3 * k + 0];
+        const float4 t1 = tris[3 * k + 1];
+        const float4 t2 = tris[3 * k + 2];
+        vec3 v0 = make_vec3(t0.x, t0.y, t0.z);
+        vec3 e1 = make_vec3(t1.x, t1.y, t1.z);
+        vec3 e2 = make_vec3(t2.x, t2.y, t2.z);
+        vec3 pv = cross(d, e2);
+        float det = dot(e1, pv);
+        if (det == 0.0f) continue;
+        float inv = 1.0f / det;
+        vec3 tv = o - v0;
+        float u = dot(tv, pv) * inv;
+        if (!(u >= 0.0f && u <= 1.0f)) continue;
+        vec3 qv = cross(tv, e1);
+        float v = dot(d, qv) * inv;
+        if (!(v >= 0.0f && u + v <= 1.0f)) continue;
+        float t = dot(e2, qv) * inv;
+        if (!(t >= r.tmin && t < hit.t)) continue;
+        uint32_t flags = __float_as_uint(t2.w);
+        if (flags & kTriFlagAlphaTested) {
+            if (alpha_rejects(sc, __float_as_uint(t0.w), __float_as_uint(t1.w), u, v)) continue;
+        }
+        hit.t = t;
+        hit.tri = k;
+        hit.u = u;
+        hit.v = v;
+        if (kAnyHit) return true;
+    }
+    return false;
+}
+
+// Per-ray walk, organised for wave64 (speculative while-while): a lane walks
+// inner nodes and sets the first leaf it meets aside, continuing down the tree
+// until every lane of the wave holds a leaf (or has no inner node left); then
+// the lanes test one leaf each.  Leaf tests thus run for many lanes at once
+// instead of interleaving with node visits.  The triangles tested are those of
+// the plain nearest-child-first walk; only the order of leaves can differ, which
+// changes nothing but the choice among hits at exactly equal t.
 template <bool kAnyHit, bool kCount>
 __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax, Hit& hit, int* stack,
                          int stride, TraversalCounters& cnt) {
-    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
-    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
+    const TraversalRay r = make_traversal_ray(o, d, tmin);
     hit.tri = kNoHit;
     hit.t = tmax;
     int sp = 0;
     int node = 0;
-    const float4* __restrict__ nodes = sc.nodes;
-    const float4* __restrict__ tris = sc.tris;
-    while (true) {
-        if (node >= 0) {
+    int leaf = 0;  // leaf set aside (0: none; leaf codes are negative)
+    while (node != kNodeDone || leaf != 0) {
+        while (node >= 0) {
             if (kCount) cnt.nodes++;
-            const float4 n0 = nodes[4 * node + 0];
-            const float4 n1 = nodes[4 * node + 1];
-            const float4 n2 = nodes[4 * node + 2];
-            const float4 n3 = nodes[4 * node + 3];
-            float tcur = hit.t;
-            // child 0
-            float a0 = n0.x * ix - ox, a1 = n0.y * ix - ox;
-            float b0 = n0.z * iy - oy, b1 = n0.w * iy - oy;
-            float c0 = n2.x * iz - oz, c1 = n2.y * iz - oz;
-            float near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
-            float far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
-            // child 1
-            float d0 = n1.x * ix - ox, d1 = n1.y * ix - ox;
-            float e0 = n1.z * iy - oy, e1 = n1.w * iy - oy;
-            float f0 = n2.z * iz - oz, f1 = n2.w * iz - oz;
-            float near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), tmin));
-            float far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
-            bool h0 = near0 <= far0;
-            bool h1 = near1 <= far1;
-            int ch0 = __float_as_int(n3.x);
-            int ch1 = __float_as_int(n3.y);
-            if (h0 && h1) {
-                bool swap = near1 < near0;
-                int first = swap ? ch1 : ch0;
-                int second = swap ? ch0 : ch1;
-                stack[sp * stride] = second;
-                ++sp;
-                node = first;
-                continue;
-            } else if (h0) {
-                node = ch0;
-                continue;
-            } else if (h1) {
-                node = ch1;
-                continue;
+            node = visit_inner(sc.nodes, node, r, hit.t, stack, stride, sp);
+            if (node < 0 && node != kNodeDone && leaf == 0) {
+                leaf = node;
+                node = pop_node(stack, stride, sp);
             }
-        } else {
-            uint32_t code = ~(uint32_t)node;
-            uint32_t first = code >> kLeafCountBits;
-            uint32_t count = (code & (kMaxLeafSize - 1)) + 1;
-            for (uint32_t k = first; k < first + count; ++k) {
-                if (kCount) cnt.tris++;
-                const float4 t0 = tris[3 * k + 0];
-                const float4 t1 = tris[3 * k + 1];
-                const float4 t2 = tris[3 * k + 2];
-                vec3 v0 = make_vec3(t0.x, t0.y, t0.z);
-                vec3 e1 = make_vec3(t1.x, t1.y, t1.z);
-                vec3 e2 = make_vec3(t2.x, t2.y, t2.z);
-                vec3 pv = cross(d, e2);
-                float det = dot(e1, pv);
-                if (det == 0.0f) continue;
-                float inv = 1.0f / det;
-                vec3 tv = o - v0;
-                float u = dot(tv, pv) * inv;
-                if (!(u >= 0.0f && u <= 1.0f)) continue;
-                vec3 qv = cross(tv, e1);
-                float v = dot(d, qv) * inv;
-                if (!(v >= 0.0f && u + v <= 1.0f)) continue;
-                float t = dot(e2, qv) * inv;
-                if (!(t >= tmin && t < hit.t)) continue;
-                uint32_t flags = __float_as_uint(t2.w);
-                if (flags & kTriFlagAlphaTested) {
-                    if (alpha_rejects(sc, __float_as_uint(t0.w), __float_as_uint(t1.w), u, v)) continue;
-                }
-                hit.t = t;
-                hit.tri = k;
-                hit.u = u;
-                hit.v = v;
-                if (kAnyHit) return true;
-            }
+            if (!__any(leaf == 0)) break;
         }
-        if (sp == 0) break;
-        --sp;
-        node = stack[sp * stride];
+        const int lf = leaf != 0 ? leaf : (node != kNodeDone ? node : 0);
+        if (lf != 0) {
+            if (test_leaf<kAnyHit, kCount>(sc, lf, r, hit, cnt)) return true;
+            if (leaf != 0)
+                leaf = 0;
+            else
+                node = pop_node(stack, stride, sp);
+        }
     }
     return hit.tri != kNoHit;
 }
