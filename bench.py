@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--bvh", default=None, choices=["sah", "lbvh"],
                     help="BVH builder: binned SAH on the host or the Morton-code LBVH on the GPU "
                          "(default: SAH, LBVH for the 5M-triangle city)")
-    ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn"],
-                    help="denoiser of the timed frame: BMFR 32x32 or the learned kernel-predicting network")
+    ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn", "cnn2"],
+                    help="denoiser of the timed frame: BMFR 32x32, the 6-layer learned kernel-predicting network "
+                         "or the fused 2-layer one")
     ap.add_argument("--no-denoise-eval", action="store_true",
                     help="skip the learned-vs-BMFR relMSE evaluation on held-out Cornell-box scenes")
     ap.add_argument("--graph", dest="graph", action="store_true", default=None,
@@ -96,12 +97,15 @@ WORKLOADS = {
 
 
 def denoise_comparison(torch, fr, frame, runs, evaluate):
-    """Both denoisers on the same 1080p frame (the last timed one), `runs` launches each,
-    timed with events on the launch stream; with `evaluate`, their relMSE against
-    256-spp references on 16 held-out Cornell-box scenes (128x128, first frame)."""
-    fr.ensure_cnn()
+    """BMFR and the learned denoisers on the same frame (the last timed one), `runs`
+    launches each, timed with events on the launch stream; with `evaluate`, their
+    relMSE against 256-spp references on 16 held-out Cornell-box scenes (128x128,
+    first frame).  The two-layer network is the one that matches BMFR's quality: the
+    equal-relMSE comparison times it."""
+    fr.ensure_cnn(6)
+    fr.ensure_cnn(2)
     out = {}
-    for name, fn in (("bmfr", fr.bmfr), ("cnn", fr.cnn)):
+    for name, fn in (("bmfr", fr.bmfr), ("cnn6", lambda f: fr.cnn(f, 6)), ("cnn2", lambda f: fr.cnn(f, 2))):
         fn(frame)  # warm-up
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -111,12 +115,20 @@ def denoise_comparison(torch, fr, frame, runs, evaluate):
         b.record()
         torch.cuda.synchronize()
         out[f"{name}_ms"] = round(a.elapsed_time(b) / runs, 4)
-    out["cnn_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn_ms"], 3)
+    out["cnn2_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn2_ms"], 3)
+    out["cnn6_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn6_ms"], 3)
     if evaluate:
         from vulkanpbrt_amd import denoiser_train as dt
         pairs = dt.render_pairs(torch, np.arange(90_000, 90_016), 128, 128, 256)
-        res = dt.evaluate(torch, fr.net, pairs)
-        out.update({k: round(v, 5) for k, v in res.items()})
+        rel = {}
+        for layers in (6, 2):
+            res = dt.evaluate(torch, fr.nets[layers], pairs)
+            rel["noisy"], rel["bmfr"] = res["relmse_noisy"], res["relmse_bmfr"]
+            rel[f"cnn{layers}"] = res["relmse_cnn"]
+            rel[f"cnn{layers}_demodulated"] = res["relmse_cnn_demodulated"]
+        rel["bmfr_demodulated"] = res["relmse_bmfr_demodulated"]
+        out["relmse"] = {k: round(v, 5) for k, v in rel.items()}
+        out["cnn2_at_or_below_bmfr_relmse"] = bool(rel["cnn2"] <= rel["bmfr"])
         out["eval_set"] = ("16 held-out seed-generated Cornell-box scenes (seeds 90000-90015), 128x128, 1-spp "
                            "accumulator input, 256-spp references; relMSE of the remodulated radiance "
                            "(and of the demodulated radiance)")

@@ -484,6 +484,17 @@ int32_t pbrt_bfr_blend(uint32_t width, uint32_t height, uint32_t filter_radius,
 
 typedef struct pbrt_cnn pbrt_cnn;
 
+/* A network of `layers` (2..6) convolutions: 12 -> 32 -> ... -> 32 -> 25 logits,
+ * otherwise as above.  pbrt_cnn_create is the 6-layer network of the learned-
+ * denoiser configuration.  A two-layer network runs inference as one fused kernel
+ * per 16x16 tile with every intermediate in LDS (images.workspace is not used):
+ * the equal-quality, lower-cost alternative to BMFR (no counterpart in the
+ * reference; BMFR::add_dispatch_to_command_graph, BMFR.cpp:206-229, is what it
+ * stands in for).  Parameter count and order follow the layer count. */
+int32_t pbrt_cnn_create_layers(uint32_t layers, uint64_t seed, pbrt_cnn** out_net);
+int32_t pbrt_cnn_layers(const pbrt_cnn* net, uint32_t* out_layers);
+int32_t pbrt_cnn_layers_parameter_count(uint32_t layers, uint32_t* out_count);
+
 typedef struct pbrt_cnn_images {
     uint32_t width;
     uint32_t height;

@@ -16,6 +16,10 @@ definition, in numpy, independently of the HIP kernels:
   a = albedo + 1e-6 (the remodulation of bmfrPost.comp): the relative squared error
   of the remodulated radiance
 
+A network of L layers (2..6, shape_of(L)) is the same definition with L
+convolutions, 12 -> 32 -> ... -> 32 -> 25; the two-layer one is the fused
+equal-quality alternative to BMFR.
+
 Parameters in canonical order, layer by layer: W[cout][cin][ky][kx], then b[cout].
 `bf16=True` mirrors the device storage: weights, features, activations and
 logits rounded to bfloat16 (round to nearest even); arithmetic in float64.
@@ -28,20 +32,27 @@ CIN = [12, 32, 32, 32, 32, 32]
 COUT = [32, 32, 32, 32, 32, 25]
 LAYERS = 6
 EPS_REL = 0.01
+DEFAULT = (CIN, COUT)
 
 
-def layer_shapes():
-    return [((co, ci, 3, 3), (co,)) for ci, co in zip(CIN, COUT)]
+def shape_of(layers):
+    """Layer shapes of a network of `layers` convolutions: 12 -> 32 -> ... -> 32 -> 25."""
+    return ([12] + [32] * (layers - 1), [32] * (layers - 1) + [25])
 
 
-def parameter_count():
-    return sum(int(np.prod(w)) + b[0] for w, b in layer_shapes())
+def layer_shapes(shape=DEFAULT):
+    cin, cout = shape
+    return [((co, ci, 3, 3), (co,)) for ci, co in zip(cin, cout)]
 
 
-def unpack(params):
+def parameter_count(shape=DEFAULT):
+    return sum(int(np.prod(w)) + b[0] for w, b in layer_shapes(shape))
+
+
+def unpack(params, shape=DEFAULT):
     """Canonical flat vector -> list of (W, b)."""
     out, o = [], 0
-    for ws, bs in layer_shapes():
+    for ws, bs in layer_shapes(shape):
         n = int(np.prod(ws))
         out.append((params[o:o + n].reshape(ws), params[o + n:o + n + bs[0]]))
         o += n + bs[0]
@@ -124,15 +135,15 @@ def gather_taps(img):
     return out
 
 
-def forward(params, feats, noisy_rgb, bf16_mirror=True):
+def forward(params, feats, noisy_rgb, bf16_mirror=True, shape=DEFAULT):
     """Returns (output rgb (N,H,W,3), cache for backward)."""
-    layers = unpack(np.asarray(params, np.float64))
+    layers = unpack(np.asarray(params, np.float64), shape)
     rnd = bf16 if bf16_mirror else (lambda v: np.asarray(v, np.float64))
     a = rnd(feats)
     acts = [a]
     for li, (w, b) in enumerate(layers):
         z = conv3x3(a, rnd(w), np.asarray(b, np.float64))
-        a = rnd(np.maximum(z, 0.0)) if li < LAYERS - 1 else rnd(z)
+        a = rnd(np.maximum(z, 0.0)) if li < len(layers) - 1 else rnd(z)
         acts.append(a)
     logits = acts[-1]
     m = logits.max(-1, keepdims=True)
@@ -160,8 +171,8 @@ def backward(cache, out, target, albedo):
     g = 2.0 * a * a * (out - target) / ((a * target) ** 2 + EPS_REL) / out.size
     dwk = np.einsum("nhwc,nhwkc->nhwk", g, taps)
     dz = wk * (dwk - (wk * dwk).sum(-1, keepdims=True))
-    grads = [None] * LAYERS
-    for li in range(LAYERS - 1, -1, -1):
+    grads = [None] * len(layers)
+    for li in range(len(layers) - 1, -1, -1):
         w, _ = layers[li]
         dx, dw, db = conv3x3_backward(acts[li], rnd(w), dz)
         grads[li] = (dw, db)
@@ -170,11 +181,12 @@ def backward(cache, out, target, albedo):
     return pack(grads)
 
 
-def init_parameters(seed):
+def init_parameters(seed, shape=DEFAULT):
     """He-normal weights (last layer scaled by 0.1), zero biases."""
     rng = np.random.default_rng(seed)
     layers = []
-    for li, ((co, ci, _, _), _) in enumerate(layer_shapes()):
-        std = np.sqrt(2.0 / (9 * ci)) * (0.1 if li == LAYERS - 1 else 1.0)
+    shapes = layer_shapes(shape)
+    for li, ((co, ci, _, _), _) in enumerate(shapes):
+        std = np.sqrt(2.0 / (9 * ci)) * (0.1 if li == len(shapes) - 1 else 1.0)
         layers.append((rng.normal(0.0, std, (co, ci, 3, 3)), np.zeros(co)))
     return pack(layers)

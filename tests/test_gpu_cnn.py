@@ -44,9 +44,10 @@ def synthetic_batch(seed):
 
 
 class Net:
-    def __init__(self, seed=1):
+    def __init__(self, seed=1, layers=6):
         self.h = C.c_void_p()
-        capi.check(capi.lib().pbrt_cnn_create(seed, C.byref(self.h)))
+        self.layers = layers
+        capi.check(capi.lib().pbrt_cnn_create_layers(layers, seed, C.byref(self.h)))
 
     def set(self, params):
         p = np.ascontiguousarray(params, np.float32)
@@ -54,7 +55,7 @@ class Net:
 
     def get(self):
         n = C.c_uint32()
-        capi.check(capi.lib().pbrt_cnn_parameter_count(C.byref(n)))
+        capi.check(capi.lib().pbrt_cnn_layers_parameter_count(self.layers, C.byref(n)))
         out = np.zeros(n.value, np.float32)
         capi.check(capi.lib().pbrt_cnn_get_parameters(self.h, out.ctypes.data))
         return out
@@ -76,19 +77,22 @@ def device_images(torch, inp, training, w=W, h=H, n=N):
     return im, t
 
 
-def oracle_forward(params, inp):
+def oracle_forward(params, inp, layers=6):
     feats = cnn.features(inp["noisy"], inp["depth"], inp["normal"], inp["albedo"], inp["spp"], inp["motion"])
     noisy = cnn.half_to_float(inp["noisy"])[..., :3]
-    out, cache = cnn.forward(params, feats, noisy, bf16_mirror=True)
+    out, cache = cnn.forward(params, feats, noisy, bf16_mirror=True, shape=cnn.shape_of(layers))
     return out, cache
 
 
-def test_forward_matches_oracle(cuda):
+@pytest.mark.parametrize("layers", [6, 2])
+def test_forward_matches_oracle(cuda, layers):
+    """Six layers: the layer-by-layer path with kernel prediction fused into the last
+    layer; two layers: the single fused kernel (features to filtered radiance in LDS)."""
     inp = synthetic_batch(3)
-    net = Net(7)
+    net = Net(7, layers)
     params = net.get()
     # a non-trivial kernel: larger last-layer weights than the init
-    o = cnn.parameter_count() - 25 - 9 * 32 * 25
+    o = cnn.parameter_count(cnn.shape_of(layers)) - 25 - 9 * 32 * 25
     params[o:o + 9 * 32 * 25] *= 10.0
     net.set(params)
     np.testing.assert_array_equal(net.get(), params)
@@ -96,9 +100,9 @@ def test_forward_matches_oracle(cuda):
     capi.check(capi.lib().pbrt_cnn_denoise(net.h, C.byref(im), helpers.stream(cuda)))
     cuda.cuda.synchronize()
     g = t["denoised"].cpu().numpy()[..., :3]
-    ref, _ = oracle_forward(params.astype(np.float64), inp)
+    ref, _ = oracle_forward(params.astype(np.float64), inp, layers)
     rel = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3)
-    print(f"cnn forward: rel err median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}")
+    print(f"cnn forward ({layers} layers): rel err median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}")
     assert np.quantile(rel, 0.99) < 2e-2 and np.median(rel) < 2e-3
     # remodulated final image: BGRA8 of pow(albedo * out, 1/2.2)
     f = t["final"].cpu().numpy()
@@ -109,26 +113,27 @@ def test_forward_matches_oracle(cuda):
     net.close()
 
 
-def test_gradients_match_oracle(cuda):
+@pytest.mark.parametrize("layers", [6, 2])
+def test_gradients_match_oracle(cuda, layers):
     inp = synthetic_batch(4)
-    net = Net(9)
+    net = Net(9, layers)
     params = net.get().astype(np.float64)
     im, t = device_images(cuda, inp, training=True)
-    n = cnn.parameter_count()
+    n = cnn.parameter_count(cnn.shape_of(layers))
     grad = cuda.zeros(n, dtype=cuda.float32, device="cuda")
     loss = cuda.zeros(1, dtype=cuda.float32, device="cuda")
     capi.check(capi.lib().pbrt_cnn_gradients(net.h, C.byref(im), capi.ptr(loss), capi.ptr(grad),
                                              helpers.stream(cuda)))
     cuda.cuda.synchronize()
     g = grad.cpu().numpy().astype(np.float64)
-    out, cache = oracle_forward(params, inp)
+    out, cache = oracle_forward(params, inp, layers)
     tgt = inp["target"][..., :3].astype(np.float64)
     ref_loss = cnn.loss(out, tgt, inp["albedo"])
     ref = cnn.backward(cache, out, tgt, inp["albedo"])
     print(f"cnn loss gpu {loss.item():.6f} oracle {ref_loss:.6f}")
     assert abs(loss.item() - ref_loss) <= 2e-3 * ref_loss
     o = 0
-    for li, (ws, bs) in enumerate(cnn.layer_shapes()):
+    for li, (ws, bs) in enumerate(cnn.layer_shapes(cnn.shape_of(layers))):
         sz = int(np.prod(ws)) + bs[0]
         a, b = g[o:o + sz], ref[o:o + sz]
         cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
@@ -173,3 +178,14 @@ def test_invalid_arguments(cuda):
     assert capi.lib().pbrt_cnn_denoise(net.h, C.byref(im), None) != 0
     assert capi.lib().pbrt_cnn_train_step(net.h, C.byref(im), C.c_float(1e-3), None, None) != 0  # no target / ws
     net.close()
+    h = C.c_void_p()
+    for bad in (0, 1, 7):
+        assert capi.lib().pbrt_cnn_create_layers(bad, 1, C.byref(h)) != 0
+    two = Net(1, 2)
+    im, t = device_images(cuda, inp, training=False)
+    im.denoised = None
+    im.final_image = None
+    assert capi.lib().pbrt_cnn_denoise(two.h, C.byref(im), None) != 0  # nothing to write
+    n = C.c_uint32()
+    assert capi.lib().pbrt_cnn_layers(two.h, C.byref(n)) == 0 and n.value == 2
+    two.close()

@@ -24,3 +24,17 @@ def test_trained_network_beats_untrained_noisy_and_bmfr(cuda, pbrt_lib):
     assert t["relmse_cnn"] < 0.01 * u["relmse_cnn"]
     trained.close()
     untrained.close()
+
+
+def test_two_layer_network_beats_bmfr(cuda, pbrt_lib):
+    """The fused two-layer network (vulkanpbrt_amd/weights/kpn2_cornell.bin) reaches
+    BMFR's relMSE with room to spare on held-out scenes, so it is the network the
+    equal-quality timing comparison runs."""
+    pairs = dt.render_pairs(cuda, np.arange(95_000, 95_008), 128, 128, 256)
+    net = dt.Network(1, layers=2)
+    net.load(dt.TWO_LAYER_WEIGHTS)
+    r = dt.evaluate(cuda, net, pairs)
+    print(f"two layers: relMSE noisy {r['relmse_noisy']:.4f} bmfr {r['relmse_bmfr']:.4f} cnn {r['relmse_cnn']:.4f}")
+    assert r["relmse_cnn"] < 0.5 * r["relmse_bmfr"]
+    assert r["relmse_cnn"] < 0.25 * r["relmse_noisy"]
+    net.close()

@@ -23,7 +23,8 @@ def snapshot(fr):
                                                      ("bmfr_final", fr.bmfr_final))}
 
 
-@pytest.mark.parametrize("tracer,denoiser", [("wavefront", "bmfr"), ("megakernel", "bmfr"), ("wavefront", "cnn")])
+@pytest.mark.parametrize("tracer,denoiser", [("wavefront", "bmfr"), ("megakernel", "bmfr"), ("wavefront", "cnn"),
+                                             ("wavefront", "cnn2")])
 def test_frame_graph_equals_eager(cuda, tracer, denoiser):
     torch = cuda
     gen = capi.GeneratedScene(1, 1)

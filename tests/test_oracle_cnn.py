@@ -49,19 +49,21 @@ def test_uniform_kernel_is_box_filter():
     np.testing.assert_allclose(out, box, rtol=1e-12)
 
 
-def test_gradient_matches_finite_differences():
+@pytest.mark.parametrize("layers", [6, 2])
+def test_gradient_matches_finite_differences(layers):
     rng = np.random.default_rng(2)
-    params = cnn.init_parameters(5)
+    shape = cnn.shape_of(layers)
+    params = cnn.init_parameters(5, shape)
     params[-25:] = rng.normal(0, 0.3, 25)  # non-trivial last-layer bias
     feats = rng.normal(size=(1, 5, 6, 12))
     noisy = rng.uniform(0, 2, size=(1, 5, 6, 3))
     target = rng.uniform(0, 2, size=(1, 5, 6, 3))
     albedo = rng.integers(0, 256, size=(1, 5, 6, 4), dtype=np.uint8)
-    out, cache = cnn.forward(params, feats, noisy, bf16_mirror=False)
+    out, cache = cnn.forward(params, feats, noisy, bf16_mirror=False, shape=shape)
     g = cnn.backward(cache, out, target, albedo)
     # probe a handful of entries in every layer's weights and bias
     idx, o = [], 0
-    for ws, bs in cnn.layer_shapes():
+    for ws, bs in cnn.layer_shapes(shape):
         n = int(np.prod(ws))
         idx += list(rng.choice(n, 4, replace=False) + o) + [o + n, o + n + bs[0] - 1]
         o += n + bs[0]
@@ -70,14 +72,16 @@ def test_gradient_matches_finite_differences():
         p1, p2 = params.copy(), params.copy()
         p1[i] += h
         p2[i] -= h
-        l1 = cnn.loss(cnn.forward(p1, feats, noisy, bf16_mirror=False)[0], target, albedo)
-        l2 = cnn.loss(cnn.forward(p2, feats, noisy, bf16_mirror=False)[0], target, albedo)
+        l1 = cnn.loss(cnn.forward(p1, feats, noisy, bf16_mirror=False, shape=shape)[0], target, albedo)
+        l2 = cnn.loss(cnn.forward(p2, feats, noisy, bf16_mirror=False, shape=shape)[0], target, albedo)
         fd = (l1 - l2) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
 
 
 def test_parameter_count():
     assert cnn.parameter_count() == 9 * 12 * 32 + 32 + 4 * (9 * 32 * 32 + 32) + 9 * 32 * 25 + 25
+    assert cnn.shape_of(6) == cnn.DEFAULT
+    assert cnn.parameter_count(cnn.shape_of(2)) == 9 * 12 * 32 + 32 + 9 * 32 * 25 + 25
 
 
 def test_bf16_rounding_is_nearest_even():

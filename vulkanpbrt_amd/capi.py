@@ -239,6 +239,9 @@ SIGNATURES = {
     "pbrt_bmfr_batched_workspace_size": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.POINTER(C.c_size_t)]),
     "pbrt_bmfr_batched": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbrt_running_mean": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pbrt_cnn_create_layers": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pbrt_cnn_layers": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pbrt_cnn_layers_parameter_count": (C.c_int32, [C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbrt_frame_graph_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
     "pbrt_frame_graph_destroy": (C.c_int32, [C.c_void_p]),
     "pbrt_frame_graph_begin": (C.c_int32, [C.c_void_p, C.c_void_p]),

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -33,7 +34,7 @@
 namespace pbrt {
 namespace {
 
-constexpr int kLayers = 6;
+constexpr int kMaxLayers = 6;  // the config-4 network; a network has 2..6 layers
 constexpr int kTaps = 9;
 constexpr int kCh = 32;
 constexpr int kFeatCh = 16;
@@ -44,28 +45,28 @@ constexpr float kRelEps = 0.01f;
 
 constexpr int cin_pad(int l) { return l == 0 ? kFeatCh : kCh; }
 constexpr int cin_real(int l) { return l == 0 ? kFeatUsed : kCh; }
-constexpr int cout_real(int l) { return l == kLayers - 1 ? kKernelTaps : kCh; }
+constexpr int cout_real(int l, int layers) { return l == layers - 1 ? kKernelTaps : kCh; }
 constexpr int layer_weights(int l) { return kTaps * kCh * cin_pad(l); }
 constexpr int layer_size(int l) { return layer_weights(l) + kCh; }
 constexpr int layer_offset(int l) { return l == 0 ? 0 : layer_offset(l - 1) + layer_size(l - 1); }
-constexpr int kMasterSize = layer_offset(kLayers);
+constexpr int master_size(int layers) { return layer_offset(layers); }
 constexpr int weight_offset(int l) { return l == 0 ? 0 : weight_offset(l - 1) + layer_weights(l - 1); }
-constexpr int kWeightSize = weight_offset(kLayers);
+constexpr int weight_size(int layers) { return weight_offset(layers); }
 
-int canonical_count() {
+int canonical_count(int layers) {
     int n = 0;
-    for (int l = 0; l < kLayers; ++l) n += cout_real(l) * cin_real(l) * kTaps + cout_real(l);
+    for (int l = 0; l < layers; ++l) n += cout_real(l, layers) * cin_real(l) * kTaps + cout_real(l, layers);
     return n;
 }
 
 // canonical index -> padded master index
-std::vector<int> canonical_map() {
+std::vector<int> canonical_map(int layers) {
     std::vector<int> map;
-    for (int l = 0; l < kLayers; ++l) {
-        for (int co = 0; co < cout_real(l); ++co)
+    for (int l = 0; l < layers; ++l) {
+        for (int co = 0; co < cout_real(l, layers); ++co)
             for (int ci = 0; ci < cin_real(l); ++ci)
                 for (int t = 0; t < kTaps; ++t) map.push_back(layer_offset(l) + (t * kCh + co) * cin_pad(l) + ci);
-        for (int co = 0; co < cout_real(l); ++co) map.push_back(layer_offset(l) + layer_weights(l) + co);
+        for (int co = 0; co < cout_real(l, layers); ++co) map.push_back(layer_offset(l) + layer_weights(l) + co);
     }
     return map;
 }
@@ -75,11 +76,14 @@ typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ uint32_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
+// round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// min(x^(1/2.2), 1) for x >= 0 as exp2(log2(x) / 2.2) on v_log_f32 / v_exp_f32:
+// the result is rounded to 8 bits, far above their error
+__device__ __forceinline__ float fast_gamma(float x) {
+    return fminf(__builtin_amdgcn_exp2f(0.454545f * __builtin_amdgcn_logf(x)), 1.0f);
 }
+
 __device__ __forceinline__ bool bf_positive(uint32_t h) { return (h & 0x8000u) == 0 && (h & 0x7FFFu) != 0; }
 
 // zero the 16-bit lanes of v whose mask value is not > 0
@@ -102,14 +106,21 @@ struct CnnGeom {
 // the 16 bf16 feature channels of pixel i (12 used), packed in two 16-byte words
 __device__ __forceinline__ void pixel_features(const pbrt_cnn_images& im, size_t i, uint4& lo, uint4& hi) {
     const vec3 nz = xyz(load_half4(im.noisy, i));
-    const vec4 alb = load_rgba8(im.albedo, i);
+    const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
+    const float k255 = 1.0f / 255.0f;
+    const vec3 alb = make_vec3((float)ac.x * k255, (float)ac.y * k255, (float)ac.z * k255);
     const float2 sn = reinterpret_cast<const float2*>(im.normal)[i];
-    const vec3 nn = decode_normal(sn.x, sn.y);
+    // fast sine / cosine / log: the features are rounded to bf16 (8 significand bits)
+    float st, ct, sp, cp;
+    __sincosf(sn.x, &st, &ct);
+    __sincosf(sn.y, &sp, &cp);
+    const vec3 nn = make_vec3(cp * st, sp * st, ct);
     const float d = im.depth[i];
-    const float spp = unorm8_to_float(im.spp[i]) * 256.0f;
+    const float spp = (float)im.spp[i] * (256.0f / 255.0f);
     const float mvx = h2f(reinterpret_cast<const ushort2*>(im.motion)[i].x);
-    float f[kFeatCh] = {log1pf(fmaxf(nz.x, 0.0f)), log1pf(fmaxf(nz.y, 0.0f)), log1pf(fmaxf(nz.z, 0.0f)),
-                        alb.x, alb.y, alb.z, nn.x, nn.y, nn.z, d / (1.0f + d), fminf(spp, 16.0f) / 16.0f,
+    float f[kFeatCh] = {__logf(1.0f + fmaxf(nz.x, 0.0f)), __logf(1.0f + fmaxf(nz.y, 0.0f)),
+                        __logf(1.0f + fmaxf(nz.z, 0.0f)),
+                        alb.x, alb.y, alb.z, nn.x, nn.y, nn.z, __fdividef(d, 1.0f + d), fminf(spp, 16.0f) * 0.0625f,
                         mvx >= 0.0f ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t w[8];
 #pragma unroll
@@ -326,7 +337,8 @@ __global__ void __launch_bounds__(256) k_conv3x3(CnnGeom g, const uint16_t* __re
                     const vec3 alb = xyz(load_rgba8(im.albedo, i)) + make_vec3(1e-6f);
                     vec3 tm = alb * o;
                     tm = make_vec3(fmaxf(tm.x, 0.0f), fmaxf(tm.y, 0.0f), fmaxf(tm.z, 0.0f));
-                    store_bgra8(im.final_image, i, make_vec4(vclamp(vpow(tm, 0.454545f), 0.0f, 1.0f), 1.0f));
+                    store_bgra8(im.final_image, i,
+                                make_vec4(fast_gamma(tm.x), fast_gamma(tm.y), fast_gamma(tm.z), 1.0f));
                 }
             }
         } else {
@@ -544,14 +556,15 @@ __global__ void k_loss_total(const float* part, int blocks, float scale, float* 
 // Parameters: bf16 refresh, Adam, canonical gather
 // ---------------------------------------------------------------------------
 struct LayerTable {
-    int offset[kLayers + 1];
-    int woffset[kLayers + 1];
-    int cin[kLayers];
+    int offset[kMaxLayers + 1];
+    int woffset[kMaxLayers + 1];
+    int cin[kMaxLayers];
+    int layers;
 };
 
 __device__ __forceinline__ int layer_of(const LayerTable& lt, int i) {
     int l = 0;
-    while (l + 1 < kLayers && i >= lt.offset[l + 1]) ++l;
+    while (l + 1 < lt.layers && i >= lt.offset[l + 1]) ++l;
     return l;
 }
 
@@ -569,13 +582,13 @@ __device__ __forceinline__ void refresh_weight(const LayerTable& lt, int i, floa
 
 __global__ void __launch_bounds__(256) k_refresh(LayerTable lt, const float* master, uint16_t* wf, uint16_t* wb) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < kMasterSize) refresh_weight(lt, i, master[i], wf, wb);
+    if (i < lt.offset[lt.layers]) refresh_weight(lt, i, master[i], wf, wb);
 }
 
 __global__ void __launch_bounds__(256) k_adam(LayerTable lt, float* master, float* m, float* v, const float* grad,
                                              float lr, float c1, float c2, uint16_t* wf, uint16_t* wb) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= kMasterSize) return;
+    if (i >= lt.offset[lt.layers]) return;
     const float gi = grad[i];
     const float mi = 0.9f * m[i] + 0.1f * gi;
     const float vi = 0.999f * v[i] + 0.001f * gi * gi;
@@ -591,14 +604,196 @@ __global__ void __launch_bounds__(256) k_gather(const float* src, const int* map
     if (i < n) dst[i] = src[map[i]];
 }
 
-LayerTable layer_table() {
+LayerTable layer_table(int layers) {
     LayerTable lt{};
-    for (int l = 0; l <= kLayers; ++l) {
+    for (int l = 0; l <= layers; ++l) {
         lt.offset[l] = layer_offset(l);
         lt.woffset[l] = weight_offset(l);
     }
-    for (int l = 0; l < kLayers; ++l) lt.cin[l] = cin_pad(l);
+    for (int l = 0; l < layers; ++l) lt.cin[l] = cin_pad(l);
+    lt.layers = layers;
     return lt;
+}
+
+// ---------------------------------------------------------------------------
+// Two-layer network, fused: features -> 3x3 conv 16->32 (ReLU) -> 3x3 conv
+// 32->25 logits -> softmax 5x5 kernel -> filtered radiance, one workgroup per
+// 16x16 output tile with every intermediate in LDS (inference only).
+// ---------------------------------------------------------------------------
+// The tile's 20x20 feature halo is computed from the G-buffer into LDS; the
+// first convolution turns it into the 18x18 hidden region, the second into the
+// 16x16 logits.  Output pixels of a region are processed as a linear sequence of
+// 16-pixel chunks: one MFMA per tap and 16 output channels multiplies W_tap
+// (registers, A operand) by the chunk pixels' shifted inputs (LDS, B operand).
+// Hidden positions outside the image hold zero (the second convolution's zero
+// padding).  Weights, biases and the bf16 rounding points are those of the
+// layer-by-layer path, so both give the same logits.
+constexpr int kTwoHalo = kTile + 4, kTwoHid = kTile + 2;
+constexpr int kFeatStride = kFeatCh + 8;  // bf16 per LDS pixel (16 channels)
+constexpr int kHidStride = kCh + 8;       // bf16 per LDS pixel (32 channels)
+
+__global__ void __launch_bounds__(256) k_two_layer(CnnGeom g, pbrt_cnn_images im, const uint16_t* __restrict__ wf,
+                                                   const float* __restrict__ master) {
+    __shared__ __attribute__((aligned(16))) uint16_t buf_f[kTile * kTile * kHidStride];  // features, then logits
+    __shared__ __attribute__((aligned(16))) uint16_t buf_h[kTwoHid * kTwoHid * kHidStride];
+    __shared__ __attribute__((aligned(16))) float4 ntile[kTwoHalo * kTwoHalo];  // noisy taps, converted once
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, grp = lane >> 4;
+    const int tiles_x = (g.width + kTile - 1) / kTile, tiles_y = (g.height + kTile - 1) / kTile;
+    const int per_image = tiles_x * tiles_y, ntiles = per_image * g.count;
+    // contiguous tile range per XCD (workgroup b runs on XCD b % 8): neighbouring
+    // tiles, whose halos overlap, share an L2
+    const int chunk = (ntiles + 7) / 8;
+    const int t = (blockIdx.x % 8) * chunk + blockIdx.x / 8;
+    if (t >= ntiles) return;
+    const int img = t / per_image, rem = t % per_image;
+    const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+    const size_t plane = (size_t)g.width * g.height;
+
+    for (int p = tid; p < kTwoHalo * kTwoHalo; p += 256) {
+        const int gx = x0 - 2 + p % kTwoHalo, gy = y0 - 2 + p / kTwoHalo;
+        uint4 lo = make_uint4(0, 0, 0, 0), hi = lo;
+        if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height)
+            pixel_features(im, plane * img + (size_t)gy * g.width + gx, lo, hi);
+        *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride]) = lo;
+        *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride + 8]) = hi;
+        // the clamped noisy taps share the 20x20 footprint
+        const int yy = min(max(gy, 0), g.height - 1), xx = min(max(gx, 0), g.width - 1);
+        const uint2 hv = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
+        ntile[p] = make_float4(h2f(hv.x & 0xFFFFu), h2f(hv.x >> 16), h2f(hv.y & 0xFFFFu), 0.0f);
+    }
+    const float* b0 = master + layer_offset(0) + layer_weights(0);
+    const float* b1 = master + layer_offset(1) + layer_weights(1);
+    {
+        uint2 w0[kTaps][2];
+#pragma unroll
+        for (int tp = 0; tp < kTaps; ++tp)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                w0[tp][ct] = *reinterpret_cast<const uint2*>(wf + weight_offset(0) + (tp * kCh + ct * 16 + j) * kFeatCh +
+                                                             4 * grp);
+        float bv[2][4];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[ct][i] = b0[ct * 16 + 4 * grp + i];
+        __syncthreads();
+        constexpr int npix = kTwoHid * kTwoHid, nchunks = (npix + 15) / 16;
+        for (int c = wave; c < nchunks; c += 4) {
+            const int idx = c * 16 + j;
+            const int ic = idx < npix ? idx : npix - 1;
+            const int oy = ic / kTwoHid, ox = ic % kTwoHid;
+            const uint16_t* base = buf_f + (oy * kTwoHalo + ox) * kFeatStride + 4 * grp;
+            float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+            for (int tp = 0; tp < kTaps; ++tp) {
+                const short4v bx = __builtin_bit_cast(
+                    short4v, *reinterpret_cast<const uint2*>(base + ((tp / 3) * kTwoHalo + tp % 3) * kFeatStride));
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, w0[tp][ct]), bx,
+                                                                        acc[ct], 0, 0, 0);
+            }
+            if (idx < npix) {
+                const int gx = x0 - 1 + ox, gy = y0 - 1 + oy;
+                const bool inside = gx >= 0 && gy >= 0 && gx < g.width && gy < g.height;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[ct][i] + bv[ct][i], 0.0f) : 0.0f;
+                    *reinterpret_cast<uint2*>(&buf_h[idx * kHidStride + ct * 16 + 4 * grp]) =
+                        make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    {
+        uint4 w1[kTaps][2];
+#pragma unroll
+        for (int tp = 0; tp < kTaps; ++tp)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                w1[tp][ct] = *reinterpret_cast<const uint4*>(wf + weight_offset(1) + (tp * kCh + ct * 16 + j) * kCh +
+                                                             8 * grp);
+        float bv[2][4];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bv[ct][i] = b1[ct * 16 + 4 * grp + i];
+        constexpr int nchunks = kTile * kTile / 16;
+        for (int c = wave; c < nchunks; c += 4) {
+            const int idx = c * 16 + j;
+            const int oy = idx / kTile, ox = idx % kTile;
+            const uint16_t* base = buf_h + (oy * kTwoHid + ox) * kHidStride + 8 * grp;
+            float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+            for (int tp = 0; tp < kTaps; ++tp) {
+                const bf16x8 bx = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(base + ((tp / 3) * kTwoHid + tp % 3) * kHidStride));
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w1[tp][ct]), bx,
+                                                                      acc[ct], 0, 0, 0);
+            }
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = acc[ct][i] + bv[ct][i];
+                *reinterpret_cast<uint2*>(&buf_f[idx * kHidStride + ct * 16 + 4 * grp]) =
+                    make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+            }
+        }
+    }
+    __syncthreads();
+
+    // softmax over this thread's pixel's 25 logits and the 5x5 weighted sum
+    const int py = tid / kTile, px = tid % kTile;
+    const int gy = y0 + py, gx = x0 + px;
+    if (gx >= g.width || gy >= g.height) return;
+    float z[kKernelTaps];
+    float zmax = -3.4e38f;
+#pragma unroll
+    for (int c = 0; c < kKernelTaps; ++c) {
+        z[c] = bf2f(buf_f[tid * kHidStride + c]);
+        zmax = fmaxf(zmax, z[c]);
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kKernelTaps; ++c) {
+        z[c] = __expf(z[c] - zmax);
+        sum += z[c];
+    }
+    const float inv = 1.0f / sum;
+    // tile-local rows / columns of the clamped 5x5 neighbourhood
+    int ry[5], rx[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        ry[k] = (min(max(gy + k - 2, 0), g.height - 1) - (y0 - 2)) * kTwoHalo;
+        rx[k] = min(max(gx + k - 2, 0), g.width - 1) - (x0 - 2);
+    }
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kKernelTaps; ++c) {
+        const float4 nv = ntile[ry[c / 5] + rx[c % 5]];
+        const float w = z[c] * inv;
+        ox += nv.x * w;
+        oy += nv.y * w;
+        oz += nv.z * w;
+    }
+    const size_t i = plane * img + (size_t)gy * g.width + gx;
+    if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(ox, oy, oz, 1.0f);
+    if (im.final_image) {
+        // remodulated, gamma 1/2.2
+        const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
+        const float k = 1.0f / 255.0f;
+        const float tx = fmaxf(((float)ac.x * k + 1e-6f) * ox, 0.0f);
+        const float ty = fmaxf(((float)ac.y * k + 1e-6f) * oy, 0.0f);
+        const float tz = fmaxf(((float)ac.z * k + 1e-6f) * oz, 0.0f);
+        store_bgra8(im.final_image, i, make_vec4(fast_gamma(tx), fast_gamma(ty), fast_gamma(tz), 1.0f));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -608,7 +803,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct CnnWork {
     uint16_t* feat;
-    uint16_t* act[kLayers + 1];  // act[0] = feat; act[1..5] hidden, act[6] logits
+    uint16_t* act[kMaxLayers + 1];  // act[0] = feat; act[1..L-1] hidden, act[L] logits
     uint16_t* grad_act[2];       // training: upstream gradients (ping-pong)
     float* partial;              // training: per-tile weight-gradient partials
     float* loss_part;
@@ -616,7 +811,7 @@ struct CnnWork {
     size_t bytes;
 };
 
-CnnWork layout(uint32_t w, uint32_t h, uint32_t count, bool training, char* base) {
+CnnWork layout(uint32_t w, uint32_t h, uint32_t count, bool training, int layers, char* base) {
     CnnWork k{};
     const size_t px = (size_t)w * h * count;
     size_t off = 0;
@@ -628,18 +823,18 @@ CnnWork layout(uint32_t w, uint32_t h, uint32_t count, bool training, char* base
     k.feat = (uint16_t*)take(px * kFeatCh * 2);
     k.act[0] = k.feat;
     if (training) {
-        for (int l = 1; l <= kLayers; ++l) k.act[l] = (uint16_t*)take(px * kCh * 2);
+        for (int l = 1; l <= layers; ++l) k.act[l] = (uint16_t*)take(px * kCh * 2);
         k.grad_act[0] = (uint16_t*)take(px * kCh * 2);
         k.grad_act[1] = (uint16_t*)take(px * kCh * 2);
         const size_t tiles = (size_t)((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile) * count;
         k.partial = (float*)take(tiles * (kTaps * kCh * kCh + kCh) * 4);
         k.loss_part = (float*)take(((px + 255) / 256) * 4);
-        k.grad = (float*)take(kMasterSize * 4);
+        k.grad = (float*)take(master_size(layers) * 4);
     } else {
         // ping-pong hidden activations; logits in the second buffer
         uint16_t* a = (uint16_t*)take(px * kCh * 2);
         uint16_t* b = (uint16_t*)take(px * kCh * 2);
-        for (int l = 1; l <= kLayers; ++l) k.act[l] = (l & 1) ? a : b;
+        for (int l = 1; l <= layers; ++l) k.act[l] = (l & 1) ? a : b;
     }
     k.bytes = off;
     return k;
@@ -653,18 +848,23 @@ using namespace pbrt;
 struct pbrt_cnn {
     DeviceBuffer master, m, v, wf, wb, grad_map;
     uint64_t step = 0;
+    int layers = kMaxLayers;
 };
 
 namespace pbrt {
 namespace {
 
-void check_images(const pbrt_cnn_images* im, bool training, const char* fn) {
+void check_images(const pbrt_cnn_images* im, bool training, int layers, const char* fn) {
     PBRT_REQUIRE(im, std::string(fn) + ": NULL images");
     PBRT_REQUIRE(im->width > 0 && im->height > 0 && im->count > 0, std::string(fn) + ": empty batch");
     PBRT_REQUIRE(im->noisy && im->depth && im->normal && im->albedo && im->spp && im->motion,
                  std::string(fn) + ": missing input image");
     PBRT_REQUIRE(!training || im->target, std::string(fn) + ": missing target");
-    const size_t need = layout(im->width, im->height, im->count, training, nullptr).bytes;
+    if (layers == 0) {  // the fused two-layer network needs no workspace
+        PBRT_REQUIRE(im->denoised || im->final_image, std::string(fn) + ": no output image");
+        return;
+    }
+    const size_t need = layout(im->width, im->height, im->count, training, layers, nullptr).bytes;
     PBRT_REQUIRE(im->workspace && im->workspace_bytes >= need, std::string(fn) + ": workspace too small");
 }
 
@@ -693,6 +893,15 @@ void launch_conv(const CnnGeom& g, const uint16_t* in, const uint16_t* mask, con
     hipLaunchKernelGGL((k_conv3x3<CIN, MODE>), grid, dim3(256), 0, st, g, in, mask, w, b, out, im);
 }
 
+// inference of a two-layer network: one fused kernel, no workspace
+void forward_two_layer(const pbrt_cnn* net, const pbrt_cnn_images& im, hipStream_t st) {
+    const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
+    const size_t tiles = (size_t)((im.width + kTile - 1) / kTile) * ((im.height + kTile - 1) / kTile) * im.count;
+    PBRT_REQUIRE(tiles < (1u << 30), "pbrt_cnn_denoise: batch too large");
+    hipLaunchKernelGGL(k_two_layer, dim3((unsigned)(((tiles + 7) / 8) * 8)), dim3(256), 0, st, g, im,
+                       net->wf.as<uint16_t>(), net->master.as<float>());
+}
+
 // inference: kernel prediction fused into the last layer
 void forward_fused(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, hipStream_t st) {
     const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
@@ -700,12 +909,13 @@ void forward_fused(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork
     hipLaunchKernelGGL(k_features, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, im, k.feat);
     const uint16_t* wf = net->wf.as<uint16_t>();
     const float* master = net->master.as<float>();
-    for (int l = 0; l < kLayers; ++l) {
+    const int layers = net->layers;
+    for (int l = 0; l < layers; ++l) {
         const uint16_t* w = wf + weight_offset(l);
         const float* b = master + layer_offset(l) + layer_weights(l);
         if (l == 0)
             launch_conv<kFeatCh, kConvRelu>(g, k.act[0], nullptr, w, b, k.act[1], st);
-        else if (l < kLayers - 1)
+        else if (l < layers - 1)
             launch_conv<kCh, kConvRelu>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
         else
             launch_conv<kCh, kConvKpn>(g, k.act[l], nullptr, w, b, nullptr, st, im);
@@ -718,12 +928,13 @@ void forward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, h
     hipLaunchKernelGGL(k_features, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st, g, im, k.feat);
     const uint16_t* wf = net->wf.as<uint16_t>();
     const float* master = net->master.as<float>();
-    for (int l = 0; l < kLayers; ++l) {
+    const int layers = net->layers;
+    for (int l = 0; l < layers; ++l) {
         const uint16_t* w = wf + weight_offset(l);
         const float* b = master + layer_offset(l) + layer_weights(l);
         if (l == 0)
             launch_conv<kFeatCh, kConvRelu>(g, k.act[0], nullptr, w, b, k.act[1], st);
-        else if (l < kLayers - 1)
+        else if (l < layers - 1)
             launch_conv<kCh, kConvRelu>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
         else
             launch_conv<kCh, kConvLinear>(g, k.act[l], nullptr, w, b, k.act[l + 1], st);
@@ -735,16 +946,17 @@ void backward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, 
     const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
     const size_t px = (size_t)im.width * im.height * im.count;
     const int blocks = (int)((px + 255) / 256);
-    hipLaunchKernelGGL(k_kpn<true>, dim3(blocks), dim3(256), 0, st, g, im, k.act[kLayers], k.grad_act[0],
+    const int layers = net->layers;
+    hipLaunchKernelGGL(k_kpn<true>, dim3(blocks), dim3(256), 0, st, g, im, k.act[layers], k.grad_act[0],
                        k.loss_part);
     if (loss) hipLaunchKernelGGL(k_loss_total, dim3(1), dim3(64), 0, st, k.loss_part, blocks, 1.0f / (float)(px * 3), loss);
     const dim3 grid((im.width + kTile - 1) / kTile, (im.height + kTile - 1) / kTile, im.count);
     const int tiles = (int)(grid.x * grid.y * grid.z);
     const uint16_t* wb = net->wb.as<uint16_t>();
     int cur = 0;  // grad_act[cur] holds dA of layer l+1 (dLogits for the last layer)
-    for (int l = kLayers - 1; l >= 0; --l) {
+    for (int l = layers - 1; l >= 0; --l) {
         const uint16_t* dsrc = k.grad_act[cur];
-        const uint16_t* mask = l == kLayers - 1 ? nullptr : k.act[l + 1];
+        const uint16_t* mask = l == layers - 1 ? nullptr : k.act[l + 1];
         const int size = layer_size(l);
         if (l == 0)
             hipLaunchKernelGGL(k_conv3x3_wgrad<kFeatCh>, grid, dim3(256), 0, st, g, dsrc, mask, k.act[0], k.partial);
@@ -765,18 +977,41 @@ void backward(const pbrt_cnn* net, const pbrt_cnn_images& im, const CnnWork& k, 
 extern "C" int32_t pbrt_cnn_parameter_count(uint32_t* out) {
     return guard([&] {
         PBRT_REQUIRE(out, "pbrt_cnn_parameter_count: NULL output");
-        *out = (uint32_t)canonical_count();
+        *out = (uint32_t)canonical_count(kMaxLayers);
+    });
+}
+
+extern "C" int32_t pbrt_cnn_layers_parameter_count(uint32_t layers, uint32_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_cnn_layers_parameter_count: NULL output");
+        PBRT_REQUIRE(layers >= 2 && layers <= (uint32_t)kMaxLayers,
+                     "pbrt_cnn_layers_parameter_count: layers must be 2..6");
+        *out = (uint32_t)canonical_count((int)layers);
+    });
+}
+
+extern "C" int32_t pbrt_cnn_layers(const pbrt_cnn* net, uint32_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(net && out, "pbrt_cnn_layers: NULL argument");
+        *out = (uint32_t)net->layers;
     });
 }
 
 extern "C" int32_t pbrt_cnn_create(uint64_t seed, pbrt_cnn** out) {
+    return pbrt_cnn_create_layers((uint32_t)kMaxLayers, seed, out);
+}
+
+extern "C" int32_t pbrt_cnn_create_layers(uint32_t layers, uint64_t seed, pbrt_cnn** out) {
     return guard([&] {
         PBRT_REQUIRE(out, "pbrt_cnn_create: NULL output");
         *out = nullptr;
+        PBRT_REQUIRE(layers >= 2 && layers <= (uint32_t)kMaxLayers, "pbrt_cnn_create: layers must be 2..6");
+        const int L = (int)layers;
         auto net = std::make_unique<pbrt_cnn>();
+        net->layers = L;
         // He-normal weights from splitmix64 + Box-Muller; last layer scaled by 0.1 so the
         // initial kernel is close to a 5x5 box filter; zero biases
-        std::vector<float> canon(canonical_count());
+        std::vector<float> canon(canonical_count(L));
         uint64_t state = seed;
         auto next = [&]() {
             uint64_t z = (state += 0x9E3779B97F4A7C15ull);
@@ -786,22 +1021,22 @@ extern "C" int32_t pbrt_cnn_create(uint64_t seed, pbrt_cnn** out) {
         };
         auto uniform = [&]() { return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
         size_t o = 0;
-        for (int l = 0; l < kLayers; ++l) {
-            const double std = std::sqrt(2.0 / (kTaps * cin_real(l))) * (l == kLayers - 1 ? 0.1 : 1.0);
-            const int nw = cout_real(l) * cin_real(l) * kTaps;
+        for (int l = 0; l < L; ++l) {
+            const double std = std::sqrt(2.0 / (kTaps * cin_real(l))) * (l == L - 1 ? 0.1 : 1.0);
+            const int nw = cout_real(l, L) * cin_real(l) * kTaps;
             for (int i = 0; i < nw; ++i) {
                 const double u1 = uniform(), u2 = uniform();
                 canon[o++] = (float)(std * std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
             }
-            for (int i = 0; i < cout_real(l); ++i) canon[o++] = 0.0f;
+            for (int i = 0; i < cout_real(l, L); ++i) canon[o++] = 0.0f;
         }
-        const std::vector<int> map = canonical_map();
+        const std::vector<int> map = canonical_map(L);
         net->grad_map.upload(map.data(), map.size() * sizeof(int));
-        std::vector<float> zeros(kMasterSize, 0.0f);
+        std::vector<float> zeros(master_size(L), 0.0f);
         net->master.upload(zeros.data(), zeros.size() * 4);
         net->m.upload(zeros.data(), zeros.size() * 4);
         net->v.upload(zeros.data(), zeros.size() * 4);
-        std::vector<uint16_t> zw(kWeightSize, 0);
+        std::vector<uint16_t> zw(weight_size(L), 0);
         net->wf.upload(zw.data(), zw.size() * 2);
         net->wb.upload(zw.data(), zw.size() * 2);
         *out = net.release();
@@ -821,9 +1056,10 @@ extern "C" int32_t pbrt_cnn_destroy(pbrt_cnn* net) {
 extern "C" int32_t pbrt_cnn_get_parameters(const pbrt_cnn* net, float* host) {
     return guard([&] {
         PBRT_REQUIRE(net && host, "pbrt_cnn_get_parameters: NULL argument");
-        std::vector<float> padded(kMasterSize);
-        PBRT_HIP(hipMemcpy(padded.data(), net->master.ptr, kMasterSize * 4, hipMemcpyDeviceToHost));
-        const std::vector<int> map = canonical_map();
+        const int n = master_size(net->layers);
+        std::vector<float> padded(n);
+        PBRT_HIP(hipMemcpy(padded.data(), net->master.ptr, n * 4, hipMemcpyDeviceToHost));
+        const std::vector<int> map = canonical_map(net->layers);
         for (size_t i = 0; i < map.size(); ++i) host[i] = padded[map[i]];
     });
 }
@@ -831,14 +1067,15 @@ extern "C" int32_t pbrt_cnn_get_parameters(const pbrt_cnn* net, float* host) {
 extern "C" int32_t pbrt_cnn_set_parameters(pbrt_cnn* net, const float* host) {
     return guard([&] {
         PBRT_REQUIRE(net && host, "pbrt_cnn_set_parameters: NULL argument");
-        std::vector<float> padded(kMasterSize, 0.0f);
-        const std::vector<int> map = canonical_map();
+        const int n = master_size(net->layers);
+        std::vector<float> padded(n, 0.0f);
+        const std::vector<int> map = canonical_map(net->layers);
         for (size_t i = 0; i < map.size(); ++i) padded[map[i]] = host[i];
-        PBRT_HIP(hipMemcpy(net->master.ptr, padded.data(), kMasterSize * 4, hipMemcpyHostToDevice));
-        PBRT_HIP(hipMemset(net->m.ptr, 0, kMasterSize * 4));
-        PBRT_HIP(hipMemset(net->v.ptr, 0, kMasterSize * 4));
+        PBRT_HIP(hipMemcpy(net->master.ptr, padded.data(), n * 4, hipMemcpyHostToDevice));
+        PBRT_HIP(hipMemset(net->m.ptr, 0, n * 4));
+        PBRT_HIP(hipMemset(net->v.ptr, 0, n * 4));
         net->step = 0;
-        hipLaunchKernelGGL(k_refresh, dim3((kMasterSize + 255) / 256), dim3(256), 0, 0, layer_table(),
+        hipLaunchKernelGGL(k_refresh, dim3((n + 255) / 256), dim3(256), 0, 0, layer_table(net->layers),
                            net->master.as<float>(), net->wf.as<uint16_t>(), net->wb.as<uint16_t>());
         PBRT_HIP(hipGetLastError());
         PBRT_HIP(hipDeviceSynchronize());
@@ -849,17 +1086,23 @@ extern "C" int32_t pbrt_cnn_workspace_size(uint32_t w, uint32_t h, uint32_t coun
     return guard([&] {
         PBRT_REQUIRE(out, "pbrt_cnn_workspace_size: NULL output");
         PBRT_REQUIRE(w > 0 && h > 0 && count > 0, "pbrt_cnn_workspace_size: empty batch");
-        *out = layout(w, h, count, training != 0, nullptr).bytes;
+        *out = layout(w, h, count, training != 0, kMaxLayers, nullptr).bytes;
     });
 }
 
 extern "C" int32_t pbrt_cnn_denoise(const pbrt_cnn* net, const pbrt_cnn_images* im, void* stream) {
     return guard([&] {
         PBRT_REQUIRE(net, "pbrt_cnn_denoise: NULL network");
-        check_images(im, false, "pbrt_cnn_denoise");
         hipStream_t st = (hipStream_t)stream;
-        const CnnWork k = layout(im->width, im->height, im->count, false, static_cast<char*>(im->workspace));
-        forward_fused(net, *im, k, st);
+        if (net->layers == 2) {
+            check_images(im, false, 0, "pbrt_cnn_denoise");
+            forward_two_layer(net, *im, st);
+        } else {
+            check_images(im, false, net->layers, "pbrt_cnn_denoise");
+            const CnnWork k =
+                layout(im->width, im->height, im->count, false, net->layers, static_cast<char*>(im->workspace));
+            forward_fused(net, *im, k, st);
+        }
         PBRT_HIP(hipGetLastError());
     });
 }
@@ -868,12 +1111,13 @@ extern "C" int32_t pbrt_cnn_gradients(const pbrt_cnn* net, const pbrt_cnn_images
                                       void* stream) {
     return guard([&] {
         PBRT_REQUIRE(net && grad, "pbrt_cnn_gradients: NULL argument");
-        check_images(im, true, "pbrt_cnn_gradients");
+        check_images(im, true, net->layers, "pbrt_cnn_gradients");
         hipStream_t st = (hipStream_t)stream;
-        const CnnWork k = layout(im->width, im->height, im->count, true, static_cast<char*>(im->workspace));
+        const CnnWork k =
+            layout(im->width, im->height, im->count, true, net->layers, static_cast<char*>(im->workspace));
         forward(net, *im, k, st);
         backward(net, *im, k, loss, st);
-        const int n = canonical_count();
+        const int n = canonical_count(net->layers);
         hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, k.grad, net->grad_map.as<int>(), n,
                            grad);
         PBRT_HIP(hipGetLastError());
@@ -885,15 +1129,17 @@ extern "C" int32_t pbrt_cnn_train_step(pbrt_cnn* net, const pbrt_cnn_images* im,
     return guard([&] {
         PBRT_REQUIRE(net, "pbrt_cnn_train_step: NULL network");
         PBRT_REQUIRE(lr > 0.0f && std::isfinite(lr), "pbrt_cnn_train_step: learning rate must be positive");
-        check_images(im, true, "pbrt_cnn_train_step");
+        check_images(im, true, net->layers, "pbrt_cnn_train_step");
         hipStream_t st = (hipStream_t)stream;
-        const CnnWork k = layout(im->width, im->height, im->count, true, static_cast<char*>(im->workspace));
+        const CnnWork k =
+            layout(im->width, im->height, im->count, true, net->layers, static_cast<char*>(im->workspace));
         forward(net, *im, k, st);
         backward(net, *im, k, loss, st);
         net->step += 1;
         const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)net->step));
         const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)net->step));
-        hipLaunchKernelGGL(k_adam, dim3((kMasterSize + 255) / 256), dim3(256), 0, st, layer_table(),
+        const int n = master_size(net->layers);
+        hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, layer_table(net->layers),
                            net->master.as<float>(), net->m.as<float>(), net->v.as<float>(), k.grad, lr, c1, c2,
                            net->wf.as<uint16_t>(), net->wb.as<uint16_t>());
         PBRT_HIP(hipGetLastError());

@@ -38,20 +38,22 @@ def relmse(x, ref, albedo=None):
     return float(np.mean((x - r) ** 2 / (r ** 2 + REL_EPS)))
 
 
-DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "weights" / "kpn_cornell.bin"
+WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
+DEFAULT_WEIGHTS = WEIGHTS_DIR / "kpn_cornell.bin"        # 6 layers (the learned-denoiser configuration)
+TWO_LAYER_WEIGHTS = WEIGHTS_DIR / "kpn2_cornell.bin"     # 2 layers, fused (equal quality to BMFR, faster)
 
 
 class Network:
-    """A pbrt_cnn handle with host parameter I/O (float32, canonical order)."""
+    """A pbrt_cnn handle (2..6 layers) with host parameter I/O (float32, canonical order)."""
 
-    def __init__(self, seed: int = 1):
+    def __init__(self, seed: int = 1, layers: int = 6):
         self.handle = C.c_void_p()
-        capi.check(capi.lib().pbrt_cnn_create(seed, C.byref(self.handle)))
+        self.layers = layers
+        capi.check(capi.lib().pbrt_cnn_create_layers(layers, seed, C.byref(self.handle)))
 
-    @staticmethod
-    def parameter_count() -> int:
+    def parameter_count(self) -> int:
         n = C.c_uint32()
-        capi.check(capi.lib().pbrt_cnn_parameter_count(C.byref(n)))
+        capi.check(capi.lib().pbrt_cnn_layers_parameter_count(self.layers, C.byref(n)))
         return n.value
 
     def get(self) -> np.ndarray:
@@ -254,6 +256,7 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--layers", type=int, default=6, help="convolutions (2..6)")
     ap.add_argument("--out", type=Path, required=True, help="trained parameters (.bin raw float32 or .npy)")
     args = ap.parse_args()
     import torch
@@ -265,7 +268,7 @@ def main():
     tr = render_pairs(torch, train_seeds, args.size, args.size, args.target_spp, progress=32)
     ev = render_pairs(torch, eval_seeds, args.size, args.size, args.target_spp)
     t_render = time.perf_counter() - t0
-    net = Network(args.seed)
+    net = Network(args.seed, args.layers)
     before = evaluate(torch, net, ev)
     print("before training", before, flush=True)
     history, t_train = train(torch, net, tr, args.steps, args.batch, args.lr, seed=args.seed)
@@ -274,6 +277,7 @@ def main():
     net.save(args.out)
     report = dict(train_scenes=args.train_scenes, eval_scenes=args.eval_scenes, size=args.size,
                   target_spp=args.target_spp, steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed,
+                  layers=args.layers,
                   render_s=round(t_render, 1), train_s=round(t_train, 1), loss_history=history,
                   eval_untrained=before, eval_trained=after)
     args.out.with_suffix(".json").write_text(json.dumps(report, indent=1))

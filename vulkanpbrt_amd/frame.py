@@ -48,20 +48,24 @@ class Frame:
         self.trace_ws = t.zeros(tn.value, dtype=t.uint8, device=d)
         self.prev_view = None
         self.graph_nodes, self.graph_instantiations = 0, 0
-        self.net = None
-        if denoiser == "cnn":
-            self.ensure_cnn()
+        self.nets = {}
+        if denoiser in ("cnn", "cnn2"):
+            self.ensure_cnn(2 if denoiser == "cnn2" else 6)
 
-    def ensure_cnn(self):
-        """The learned denoiser with the committed trained parameters."""
-        if self.net is not None:
-            return
-        from vulkanpbrt_amd.denoiser_train import DEFAULT_WEIGHTS, Network
-        self.net = Network()
-        self.net.load(DEFAULT_WEIGHTS)
-        cn = C.c_size_t()
-        self.capi.check(self.capi.lib().pbrt_cnn_workspace_size(self.w, self.h, 1, 0, C.byref(cn)))
-        self.cnn_ws = self.torch.zeros(cn.value, dtype=self.torch.uint8, device="cuda")
+    def ensure_cnn(self, layers=6):
+        """A learned denoiser with its committed trained parameters: the 6-layer network
+        or the fused 2-layer one."""
+        if layers in self.nets:
+            return self.nets[layers]
+        from vulkanpbrt_amd.denoiser_train import DEFAULT_WEIGHTS, TWO_LAYER_WEIGHTS, Network
+        net = Network(1, layers)
+        net.load(TWO_LAYER_WEIGHTS if layers == 2 else DEFAULT_WEIGHTS)
+        self.nets[layers] = net
+        if not hasattr(self, "cnn_ws"):
+            cn = C.c_size_t()
+            self.capi.check(self.capi.lib().pbrt_cnn_workspace_size(self.w, self.h, 1, 0, C.byref(cn)))
+            self.cnn_ws = self.torch.zeros(cn.value, dtype=self.torch.uint8, device="cuda")
+        return net
 
     def stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
@@ -135,18 +139,18 @@ class Frame:
                              capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
         capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), self.stream()))
 
-    def cnn(self, frame):
+    def cnn(self, frame, layers=6):
         capi = self.capi
         cur = frame & 1
         im = capi.CnnImages(self.w, self.h, 1, capi.ptr(self.acc[cur]), capi.ptr(self.img["depth"]),
                             capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]), capi.ptr(self.spp[cur]),
                             capi.ptr(self.motion), None, capi.ptr(self.bmfr_final), None, capi.ptr(self.cnn_ws),
                             self.cnn_ws.numel())
-        capi.check(capi.lib().pbrt_cnn_denoise(self.net.handle, C.byref(im), self.stream()))
+        capi.check(capi.lib().pbrt_cnn_denoise(self.nets[layers].handle, C.byref(im), self.stream()))
 
     def denoise(self, frame):
-        if self.denoiser == "cnn":
-            self.cnn(frame)
+        if self.denoiser in ("cnn", "cnn2"):
+            self.cnn(frame, 2 if self.denoiser == "cnn2" else 6)
         else:
             self.bmfr(frame)
 

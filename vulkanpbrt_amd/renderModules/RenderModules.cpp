@@ -352,16 +352,28 @@ std::string LearnedDenoiser::default_weights_path() {
 
 void LearnedDenoiser::compile(Context&) {
     _final_illumination->compile();
-    uint32_t count = 0;
-    check(pbrt_cnn_parameter_count(&count));
-    std::ifstream f(_weights_path, std::ios::binary);
+    // the parameter file's length names the network: the layer count whose
+    // canonical parameter count it holds
+    std::ifstream f(_weights_path, std::ios::binary | std::ios::ate);
     if (!f) throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: cannot open " + _weights_path);
+    const std::streamsize bytes_in_file = f.tellg();
+    uint32_t layers = 0, count = 0;
+    for (uint32_t l = 2; l <= 6 && !layers; ++l) {
+        uint32_t n = 0;
+        check(pbrt_cnn_layers_parameter_count(l, &n));
+        if (bytes_in_file == (std::streamsize)(n * sizeof(float))) layers = l, count = n;
+    }
+    if (!layers)
+        throw Exception(PBRT_ERR_INVALID_ARG,
+                        "LearnedDenoiser: " + _weights_path + " is not the float32 parameters of a 2..6-layer network");
     std::vector<float> params(count);
+    f.seekg(0);
     f.read(reinterpret_cast<char*>(params.data()), (std::streamsize)(count * sizeof(float)));
-    if (f.gcount() != (std::streamsize)(count * sizeof(float)) || f.peek() != EOF)
-        throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: " + _weights_path + " does not hold " +
-                                                  std::to_string(count) + " float32 parameters");
-    if (!_net) check(pbrt_cnn_create(1, &_net));
+    if (f.gcount() != (std::streamsize)(count * sizeof(float)))
+        throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: cannot read " + _weights_path);
+    if (_net) (void)pbrt_cnn_destroy(_net);
+    _net = nullptr;
+    check(pbrt_cnn_create_layers(layers, 1, &_net));
     check(pbrt_cnn_set_parameters(_net, params.data()));
     size_t bytes = 0;
     check(pbrt_cnn_workspace_size(_width, _height, 1, 0, &bytes));
