@@ -627,12 +627,15 @@ LayerTable layer_table(int layers) {
 // (registers, A operand) by the chunk pixels' shifted inputs (LDS, B operand).
 // Hidden positions outside the image hold zero (the second convolution's zero
 // padding).  Weights, biases and the bf16 rounding points are those of the
-// layer-by-layer path, so both give the same logits.
+// layer-by-layer path, so both give the same logits.  The grid is persistent:
+// each workgroup keeps both layers' weights in registers across its tiles (three
+// waves per SIMD, with a few spilled registers: reloading the second layer's
+// weights per tile instead costs more).
 constexpr int kTwoHalo = kTile + 4, kTwoHid = kTile + 2;
 constexpr int kFeatStride = kFeatCh + 8;  // bf16 per LDS pixel (16 channels)
 constexpr int kHidStride = kCh + 8;       // bf16 per LDS pixel (32 channels)
 
-__global__ void __launch_bounds__(256) k_two_layer(CnnGeom g, pbrt_cnn_images im, const uint16_t* __restrict__ wf,
+__global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images im, const uint16_t* __restrict__ wf,
                                                    const float* __restrict__ master) {
     __shared__ __attribute__((aligned(16))) uint16_t buf_f[kTile * kTile * kHidStride];  // features, then logits
     __shared__ __attribute__((aligned(16))) uint16_t buf_h[kTwoHid * kTwoHid * kHidStride];
@@ -641,158 +644,182 @@ __global__ void __launch_bounds__(256) k_two_layer(CnnGeom g, pbrt_cnn_images im
     const int j = lane & 15, grp = lane >> 4;
     const int tiles_x = (g.width + kTile - 1) / kTile, tiles_y = (g.height + kTile - 1) / kTile;
     const int per_image = tiles_x * tiles_y, ntiles = per_image * g.count;
-    // contiguous tile range per XCD (workgroup b runs on XCD b % 8): neighbouring
-    // tiles, whose halos overlap, share an L2
-    const int chunk = (ntiles + 7) / 8;
-    const int t = (blockIdx.x % 8) * chunk + blockIdx.x / 8;
-    if (t >= ntiles) return;
-    const int img = t / per_image, rem = t % per_image;
-    const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+    // persistent: workgroup b runs on XCD b % 8 and walks that XCD's contiguous
+    // share of the tiles (neighbouring tiles, whose halos overlap, share an L2);
+    // the weights are loaded once per workgroup
+    const int slots = gridDim.x / 8, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int chunk = (ntiles + 7) / 8, begin = xcd * chunk, end = min(ntiles, begin + chunk);
+    if (begin + slot >= end) return;
     const size_t plane = (size_t)g.width * g.height;
 
-    for (int p = tid; p < kTwoHalo * kTwoHalo; p += 256) {
-        const int gx = x0 - 2 + p % kTwoHalo, gy = y0 - 2 + p / kTwoHalo;
-        uint4 lo = make_uint4(0, 0, 0, 0), hi = lo;
-        if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height)
-            pixel_features(im, plane * img + (size_t)gy * g.width + gx, lo, hi);
-        *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride]) = lo;
-        *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride + 8]) = hi;
-        // the clamped noisy taps share the 20x20 footprint
-        const int yy = min(max(gy, 0), g.height - 1), xx = min(max(gx, 0), g.width - 1);
-        const uint2 hv = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
-        ntile[p] = make_float4(h2f(hv.x & 0xFFFFu), h2f(hv.x >> 16), h2f(hv.y & 0xFFFFu), 0.0f);
-    }
+    // A operands (row = output channel ct*16 + j): 16 input channels per lane for the
+    // first layer, 8 of 32 for the second
+    uint2 w0[kTaps][2];
+    uint4 w1[kTaps][2];
+#pragma unroll
+    for (int tp = 0; tp < kTaps; ++tp)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            w0[tp][ct] =
+                *reinterpret_cast<const uint2*>(wf + weight_offset(0) + (tp * kCh + ct * 16 + j) * kFeatCh + 4 * grp);
+            w1[tp][ct] =
+                *reinterpret_cast<const uint4*>(wf + weight_offset(1) + (tp * kCh + ct * 16 + j) * kCh + 8 * grp);
+        }
     const float* b0 = master + layer_offset(0) + layer_weights(0);
     const float* b1 = master + layer_offset(1) + layer_weights(1);
-    {
-        uint2 w0[kTaps][2];
+    float bv0[2][4], bv1[2][4];
 #pragma unroll
-        for (int tp = 0; tp < kTaps; ++tp)
+    for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-                w0[tp][ct] = *reinterpret_cast<const uint2*>(wf + weight_offset(0) + (tp * kCh + ct * 16 + j) * kFeatCh +
-                                                             4 * grp);
-        float bv[2][4];
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) bv[ct][i] = b0[ct * 16 + 4 * grp + i];
+        for (int i = 0; i < 4; ++i) {
+            bv0[ct][i] = b0[ct * 16 + 4 * grp + i];
+            bv1[ct][i] = b1[ct * 16 + 4 * grp + i];
+        }
+
+    for (int t = begin + slot; t < end; t += slots) {
+        const int img = t / per_image, rem = t % per_image;
+        const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+
+        for (int p = tid; p < kTwoHalo * kTwoHalo; p += 256) {
+            const int gx = x0 - 2 + p % kTwoHalo, gy = y0 - 2 + p / kTwoHalo;
+            uint4 lo = make_uint4(0, 0, 0, 0), hi = lo;
+            if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height)
+                pixel_features(im, plane * img + (size_t)gy * g.width + gx, lo, hi);
+            *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride]) = lo;
+            *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride + 8]) = hi;
+            // the clamped noisy taps share the 20x20 footprint
+            const int yy = min(max(gy, 0), g.height - 1), xx = min(max(gx, 0), g.width - 1);
+            const uint2 hv = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
+            ntile[p] = make_float4(h2f(hv.x & 0xFFFFu), h2f(hv.x >> 16), h2f(hv.y & 0xFFFFu), 0.0f);
+        }
         __syncthreads();
-        constexpr int npix = kTwoHid * kTwoHid, nchunks = (npix + 15) / 16;
-        for (int c = wave; c < nchunks; c += 4) {
-            const int idx = c * 16 + j;
-            const int ic = idx < npix ? idx : npix - 1;
-            const int oy = ic / kTwoHid, ox = ic % kTwoHid;
-            const uint16_t* base = buf_f + (oy * kTwoHalo + ox) * kFeatStride + 4 * grp;
-            float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+        {
+            constexpr int npix = kTwoHid * kTwoHid, nchunks = (npix + 15) / 16;
+            for (int c = wave; c < nchunks; c += 4) {
+                const int idx = c * 16 + j;
+                const int ic = idx < npix ? idx : npix - 1;
+                const int oy = ic / kTwoHid, ox = ic % kTwoHid;
+                const uint16_t* base = buf_f + (oy * kTwoHalo + ox) * kFeatStride + 4 * grp;
+                float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
 #pragma unroll
-            for (int tp = 0; tp < kTaps; ++tp) {
-                const short4v bx = __builtin_bit_cast(
-                    short4v, *reinterpret_cast<const uint2*>(base + ((tp / 3) * kTwoHalo + tp % 3) * kFeatStride));
+                for (int tp = 0; tp < kTaps; ++tp) {
+                    const short4v bx = __builtin_bit_cast(
+                        short4v, *reinterpret_cast<const uint2*>(base + ((tp / 3) * kTwoHalo + tp % 3) * kFeatStride));
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, w0[tp][ct]), bx,
-                                                                        acc[ct], 0, 0, 0);
+                    for (int ct = 0; ct < 2; ++ct)
+                        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4v, w0[tp][ct]),
+                                                                            bx, acc[ct], 0, 0, 0);
+                }
+                if (idx < npix) {
+                    const int gx = x0 - 1 + ox, gy = y0 - 1 + oy;
+                    // 1 inside the image, 0 outside (zero padding), without branches
+                    const float inside = (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) ? 1.0f : 0.0f;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        float v[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[ct][i] + bv0[ct][i], 0.0f) * inside;
+                        *reinterpret_cast<uint2*>(&buf_h[idx * kHidStride + ct * 16 + 4 * grp]) =
+                            make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                    }
+                }
             }
-            if (idx < npix) {
-                const int gx = x0 - 1 + ox, gy = y0 - 1 + oy;
-                const bool inside = gx >= 0 && gy >= 0 && gx < g.width && gy < g.height;
+        }
+        __syncthreads();
+        {
+            constexpr int nchunks = kTile * kTile / 16;
+            for (int c = wave; c < nchunks; c += 4) {
+                const int idx = c * 16 + j;
+                const int oy = idx / kTile, ox = idx % kTile;
+                const uint16_t* base = buf_h + (oy * kTwoHid + ox) * kHidStride + 8 * grp;
+                float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+                for (int tp = 0; tp < kTaps; ++tp) {
+                    const bf16x8 bx = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4*>(base + ((tp / 3) * kTwoHid + tp % 3) * kHidStride));
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct)
+                        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w1[tp][ct]), bx,
+                                                                          acc[ct], 0, 0, 0);
+                }
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
                     float v[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[ct][i] + bv[ct][i], 0.0f) : 0.0f;
-                    *reinterpret_cast<uint2*>(&buf_h[idx * kHidStride + ct * 16 + 4 * grp]) =
+                    for (int i = 0; i < 4; ++i) v[i] = acc[ct][i] + bv1[ct][i];
+                    *reinterpret_cast<uint2*>(&buf_f[idx * kHidStride + ct * 16 + 4 * grp]) =
                         make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
                 }
             }
         }
-    }
-    __syncthreads();
-    {
-        uint4 w1[kTaps][2];
+        __syncthreads();
+
+        // softmax over this thread's pixel's 25 logits and the 5x5 weighted sum
+        const int py = tid / kTile, px = tid % kTile;
+        const int gy = y0 + py, gx = x0 + px;
+        if (gx < g.width && gy < g.height) {
+            // the 25 bf16 logits (and 7 padding channels): four 16-byte reads, twice
+            // (maximum, then the weighted sum) to keep few registers live
+            const uint4* lz = reinterpret_cast<const uint4*>(&buf_f[tid * kHidStride]);
+            float zmax = -3.4e38f;
 #pragma unroll
-        for (int tp = 0; tp < kTaps; ++tp)
+            for (int q = 0; q < 4; ++q) {
+                const uint4 v = lz[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
-                w1[tp][ct] = *reinterpret_cast<const uint4*>(wf + weight_offset(1) + (tp * kCh + ct * 16 + j) * kCh +
-                                                             8 * grp);
-        float bv[2][4];
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) bv[ct][i] = b1[ct * 16 + 4 * grp + i];
-        constexpr int nchunks = kTile * kTile / 16;
-        for (int c = wave; c < nchunks; c += 4) {
-            const int idx = c * 16 + j;
-            const int oy = idx / kTile, ox = idx % kTile;
-            const uint16_t* base = buf_h + (oy * kTwoHid + ox) * kHidStride + 8 * grp;
-            float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
-#pragma unroll
-            for (int tp = 0; tp < kTaps; ++tp) {
-                const bf16x8 bx = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(base + ((tp / 3) * kTwoHid + tp % 3) * kHidStride));
-#pragma unroll
-                for (int ct = 0; ct < 2; ++ct)
-                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w1[tp][ct]), bx,
-                                                                      acc[ct], 0, 0, 0);
+                for (int k = 0; k < 8; ++k) {
+                    const int c = 8 * q + k;
+                    const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
+                    if (c < kKernelTaps) zmax = fmaxf(zmax, zc);
+                }
             }
+            // exp(z - max) = exp2(z log2(e) - max log2(e)); the normalisation is applied once
+            const float l2e = 1.4426950408889634f, zm = zmax * l2e;
+            // tap rows / columns in the tile: clamped to the image only on border tiles
+            int ry[5], rx[5];
+            const bool interior = x0 >= 2 && y0 >= 2 && x0 + kTile + 2 <= g.width && y0 + kTile + 2 <= g.height;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-                float v[4];
+            for (int k = 0; k < 5; ++k) {
+                ry[k] = interior ? (py + k) * kTwoHalo
+                                 : (min(max(gy + k - 2, 0), g.height - 1) - (y0 - 2)) * kTwoHalo;
+                rx[k] = interior ? px + k : min(max(gx + k - 2, 0), g.width - 1) - (x0 - 2);
+            }
+            float sum = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = acc[ct][i] + bv[ct][i];
-                *reinterpret_cast<uint2*>(&buf_f[idx * kHidStride + ct * 16 + 4 * grp]) =
-                    make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+            for (int q = 0; q < 4; ++q) {
+                const uint4 v = lz[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int c = 8 * q + k;
+                    if (c < kKernelTaps) {
+                        const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
+                        const float e = __builtin_amdgcn_exp2f(fmaf(zc, l2e, -zm));
+                        const float4 nv = ntile[ry[c / 5] + rx[c % 5]];
+                        sum += e;
+                        ox = fmaf(nv.x, e, ox);
+                        oy = fmaf(nv.y, e, oy);
+                        oz = fmaf(nv.z, e, oz);
+                    }
+                }
+            }
+            const float inv = 1.0f / sum;
+            ox *= inv;
+            oy *= inv;
+            oz *= inv;
+            const size_t i = plane * img + (size_t)gy * g.width + gx;
+            if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(ox, oy, oz, 1.0f);
+            if (im.final_image) {
+                // remodulated, gamma 1/2.2
+                const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
+                const float k = 1.0f / 255.0f;
+                const float tx = fmaxf(((float)ac.x * k + 1e-6f) * ox, 0.0f);
+                const float ty = fmaxf(((float)ac.y * k + 1e-6f) * oy, 0.0f);
+                const float tz = fmaxf(((float)ac.z * k + 1e-6f) * oz, 0.0f);
+                store_bgra8(im.final_image, i, make_vec4(fast_gamma(tx), fast_gamma(ty), fast_gamma(tz), 1.0f));
             }
         }
-    }
-    __syncthreads();
-
-    // softmax over this thread's pixel's 25 logits and the 5x5 weighted sum
-    const int py = tid / kTile, px = tid % kTile;
-    const int gy = y0 + py, gx = x0 + px;
-    if (gx >= g.width || gy >= g.height) return;
-    float z[kKernelTaps];
-    float zmax = -3.4e38f;
-#pragma unroll
-    for (int c = 0; c < kKernelTaps; ++c) {
-        z[c] = bf2f(buf_f[tid * kHidStride + c]);
-        zmax = fmaxf(zmax, z[c]);
-    }
-    float sum = 0.0f;
-#pragma unroll
-    for (int c = 0; c < kKernelTaps; ++c) {
-        z[c] = __expf(z[c] - zmax);
-        sum += z[c];
-    }
-    const float inv = 1.0f / sum;
-    // tile-local rows / columns of the clamped 5x5 neighbourhood
-    int ry[5], rx[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        ry[k] = (min(max(gy + k - 2, 0), g.height - 1) - (y0 - 2)) * kTwoHalo;
-        rx[k] = min(max(gx + k - 2, 0), g.width - 1) - (x0 - 2);
-    }
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-#pragma unroll
-    for (int c = 0; c < kKernelTaps; ++c) {
-        const float4 nv = ntile[ry[c / 5] + rx[c % 5]];
-        const float w = z[c] * inv;
-        ox += nv.x * w;
-        oy += nv.y * w;
-        oz += nv.z * w;
-    }
-    const size_t i = plane * img + (size_t)gy * g.width + gx;
-    if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(ox, oy, oz, 1.0f);
-    if (im.final_image) {
-        // remodulated, gamma 1/2.2
-        const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
-        const float k = 1.0f / 255.0f;
-        const float tx = fmaxf(((float)ac.x * k + 1e-6f) * ox, 0.0f);
-        const float ty = fmaxf(((float)ac.y * k + 1e-6f) * oy, 0.0f);
-        const float tz = fmaxf(((float)ac.z * k + 1e-6f) * oz, 0.0f);
-        store_bgra8(im.final_image, i, make_vec4(fast_gamma(tx), fast_gamma(ty), fast_gamma(tz), 1.0f));
+        __syncthreads();  // the next tile's features overwrite the logits and taps
     }
 }
 
@@ -898,8 +925,8 @@ void forward_two_layer(const pbrt_cnn* net, const pbrt_cnn_images& im, hipStream
     const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
     const size_t tiles = (size_t)((im.width + kTile - 1) / kTile) * ((im.height + kTile - 1) / kTile) * im.count;
     PBRT_REQUIRE(tiles < (1u << 30), "pbrt_cnn_denoise: batch too large");
-    hipLaunchKernelGGL(k_two_layer, dim3((unsigned)(((tiles + 7) / 8) * 8)), dim3(256), 0, st, g, im,
-                       net->wf.as<uint16_t>(), net->master.as<float>());
+    const dim3 grid = persistent_grid(k_two_layer, (int)tiles);
+    hipLaunchKernelGGL(k_two_layer, grid, dim3(256), 0, st, g, im, net->wf.as<uint16_t>(), net->master.as<float>());
 }
 
 // inference: kernel prediction fused into the last layer
