@@ -74,10 +74,17 @@ std::vector<int> canonical_map(int layers) {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float(h << 16); }
 // round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint32_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// two floats to packed bf16 (lo = a), rounded to nearest even in one v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((float2v){a, b}, bf16x2v));
+}
 // min(x^(1/2.2), 1) for x >= 0 as exp2(log2(x) / 2.2) on v_log_f32 / v_exp_f32:
 // the result is rounded to 8 bits, far above their error
 __device__ __forceinline__ float fast_gamma(float x) {
@@ -110,7 +117,8 @@ __device__ __forceinline__ void pixel_features(const pbrt_cnn_images& im, size_t
     const float k255 = 1.0f / 255.0f;
     const vec3 alb = make_vec3((float)ac.x * k255, (float)ac.y * k255, (float)ac.z * k255);
     const float2 sn = reinterpret_cast<const float2*>(im.normal)[i];
-    // fast sine / cosine / log: the features are rounded to bf16 (8 significand bits)
+    // fast sine / cosine / log / reciprocal: the features are rounded to bf16 (8
+    // significand bits)
     float st, ct, sp, cp;
     __sincosf(sn.x, &st, &ct);
     __sincosf(sn.y, &sp, &cp);
@@ -118,13 +126,18 @@ __device__ __forceinline__ void pixel_features(const pbrt_cnn_images& im, size_t
     const float d = im.depth[i];
     const float spp = (float)im.spp[i] * (256.0f / 255.0f);
     const float mvx = h2f(reinterpret_cast<const ushort2*>(im.motion)[i].x);
-    float f[kFeatCh] = {__logf(1.0f + fmaxf(nz.x, 0.0f)), __logf(1.0f + fmaxf(nz.y, 0.0f)),
-                        __logf(1.0f + fmaxf(nz.z, 0.0f)),
-                        alb.x, alb.y, alb.z, nn.x, nn.y, nn.z, __fdividef(d, 1.0f + d), fminf(spp, 16.0f) * 0.0625f,
+    // log(1 + x) as ln2 * v_log_f32 (its argument is >= 1: no denormal scaling) and
+    // d / (1 + d) on v_rcp_f32 (1 ulp)
+    const float ln2 = 0.69314718f;
+    float f[kFeatCh] = {ln2 * __builtin_amdgcn_logf(1.0f + fmaxf(nz.x, 0.0f)),
+                        ln2 * __builtin_amdgcn_logf(1.0f + fmaxf(nz.y, 0.0f)),
+                        ln2 * __builtin_amdgcn_logf(1.0f + fmaxf(nz.z, 0.0f)),
+                        alb.x, alb.y, alb.z, nn.x, nn.y, nn.z, d * __builtin_amdgcn_rcpf(1.0f + d),
+                        fminf(spp, 16.0f) * 0.0625f,
                         mvx >= 0.0f ? 1.0f : 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t w[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) w[k] = f2bf(f[2 * k]) | (f2bf(f[2 * k + 1]) << 16);
+    for (int k = 0; k < 8; ++k) w[k] = pack_bf16(f[2 * k], f[2 * k + 1]);
     lo = make_uint4(w[0], w[1], w[2], w[3]);
     hi = make_uint4(w[4], w[5], w[6], w[7]);
 }
@@ -301,7 +314,7 @@ __global__ void __launch_bounds__(256) k_conv3x3(CnnGeom g, const uint16_t* __re
 #pragma unroll
                     for (int i = 0; i < 4; ++i) v[i] = acc[r][ct][i] + bv[ct][i];
                     *reinterpret_cast<uint2*>(&logit[p * kCh + ct * 16 + 4 * grp]) =
-                        make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                        make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
                 }
             }
             __syncthreads();
@@ -356,7 +369,7 @@ __global__ void __launch_bounds__(256) k_conv3x3(CnnGeom g, const uint16_t* __re
                     if (MODE == kConvRelu) v[i] = fmaxf(v[i], 0.0f);
                 }
                 *reinterpret_cast<uint2*>(o + ct * 16 + 4 * grp) =
-                    make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                    make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
             }
         }
         }
@@ -530,7 +543,7 @@ __global__ void __launch_bounds__(256) k_kpn(CnnGeom g, pbrt_cnn_images im, cons
             for (int k = 0; k < kCh / 2; ++k) {
                 const float a = 2 * k < kKernelTaps ? z[2 * k] * (dw[2 * k] - wdw) : 0.0f;
                 const float b = 2 * k + 1 < kKernelTaps ? z[2 * k + 1] * (dw[2 * k + 1] - wdw) : 0.0f;
-                packed[k] = f2bf(a) | (f2bf(b) << 16);
+                packed[k] = pack_bf16(a, b);
             }
             uint4* d4 = reinterpret_cast<uint4*>(dlogits + i * kCh);
 #pragma unroll
@@ -627,7 +640,8 @@ LayerTable layer_table(int layers) {
 // (registers, A operand) by the chunk pixels' shifted inputs (LDS, B operand).
 // Hidden positions outside the image hold zero (the second convolution's zero
 // padding).  Weights, biases and the bf16 rounding points are those of the
-// layer-by-layer path, so both give the same logits.  The grid is persistent:
+// layer-by-layer path (the biases enter as the accumulators' initial values, so
+// the FP32 sums round in another order).  The grid is persistent:
 // each workgroup keeps both layers' weights in registers across its tiles (three
 // waves per SIMD, with a few spilled registers: reloading the second layer's
 // weights per tile instead costs more).
@@ -639,7 +653,7 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                                                    const float* __restrict__ master) {
     __shared__ __attribute__((aligned(16))) uint16_t buf_f[kTile * kTile * kHidStride];  // features, then logits
     __shared__ __attribute__((aligned(16))) uint16_t buf_h[kTwoHid * kTwoHid * kHidStride];
-    __shared__ __attribute__((aligned(16))) float4 ntile[kTwoHalo * kTwoHalo];  // noisy taps, converted once
+    __shared__ __attribute__((aligned(16))) uint2 ntile[kTwoHalo * kTwoHalo];  // noisy taps, raw FP16
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane & 15, grp = lane >> 4;
     const int tiles_x = (g.width + kTile - 1) / kTile, tiles_y = (g.height + kTile - 1) / kTile;
@@ -667,7 +681,8 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
         }
     const float* b0 = master + layer_offset(0) + layer_weights(0);
     const float* b1 = master + layer_offset(1) + layer_weights(1);
-    float bv0[2][4], bv1[2][4];
+    // the biases of this lane's four output rows: the accumulators' initial values
+    float4v bv0[2], bv1[2];
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
@@ -689,8 +704,7 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
             *reinterpret_cast<uint4*>(&buf_f[p * kFeatStride + 8]) = hi;
             // the clamped noisy taps share the 20x20 footprint
             const int yy = min(max(gy, 0), g.height - 1), xx = min(max(gx, 0), g.width - 1);
-            const uint2 hv = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
-            ntile[p] = make_float4(h2f(hv.x & 0xFFFFu), h2f(hv.x >> 16), h2f(hv.y & 0xFFFFu), 0.0f);
+            ntile[p] = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
         }
         __syncthreads();
         {
@@ -700,7 +714,7 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                 const int ic = idx < npix ? idx : npix - 1;
                 const int oy = ic / kTwoHid, ox = ic % kTwoHid;
                 const uint16_t* base = buf_f + (oy * kTwoHalo + ox) * kFeatStride + 4 * grp;
-                float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+                float4v acc[2] = {bv0[0], bv0[1]};
 #pragma unroll
                 for (int tp = 0; tp < kTaps; ++tp) {
                     const short4v bx = __builtin_bit_cast(
@@ -712,15 +726,16 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                 }
                 if (idx < npix) {
                     const int gx = x0 - 1 + ox, gy = y0 - 1 + oy;
-                    // 1 inside the image, 0 outside (zero padding), without branches
-                    const float inside = (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) ? 1.0f : 0.0f;
+                    // ReLU inside the image, 0 outside (zero padding): median of (x, 0, hi)
+                    // with hi = +inf inside, 0 outside
+                    const float hi = (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height) ? INFINITY : 0.0f;
 #pragma unroll
                     for (int ct = 0; ct < 2; ++ct) {
                         float v[4];
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[ct][i] + bv0[ct][i], 0.0f) * inside;
+                        for (int i = 0; i < 4; ++i) v[i] = __builtin_amdgcn_fmed3f(acc[ct][i], 0.0f, hi);
                         *reinterpret_cast<uint2*>(&buf_h[idx * kHidStride + ct * 16 + 4 * grp]) =
-                            make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                            make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
                     }
                 }
             }
@@ -732,7 +747,7 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                 const int idx = c * 16 + j;
                 const int oy = idx / kTile, ox = idx % kTile;
                 const uint16_t* base = buf_h + (oy * kTwoHid + ox) * kHidStride + 8 * grp;
-                float4v acc[2] = {{0.0f, 0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f, 0.0f}};
+                float4v acc[2] = {bv1[0], bv1[1]};
 #pragma unroll
                 for (int tp = 0; tp < kTaps; ++tp) {
                     const bf16x8 bx = __builtin_bit_cast(
@@ -744,11 +759,8 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                 }
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    float v[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) v[i] = acc[ct][i] + bv1[ct][i];
                     *reinterpret_cast<uint2*>(&buf_f[idx * kHidStride + ct * 16 + 4 * grp]) =
-                        make_uint2(f2bf(v[0]) | (f2bf(v[1]) << 16), f2bf(v[2]) | (f2bf(v[3]) << 16));
+                        make_uint2(pack_bf16(acc[ct][0], acc[ct][1]), pack_bf16(acc[ct][2], acc[ct][3]));
                 }
             }
         }
@@ -775,15 +787,10 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
             }
             // exp(z - max) = exp2(z log2(e) - max log2(e)); the normalisation is applied once
             const float l2e = 1.4426950408889634f, zm = zmax * l2e;
-            // tap rows / columns in the tile: clamped to the image only on border tiles
-            int ry[5], rx[5];
-            const bool interior = x0 >= 2 && y0 >= 2 && x0 + kTile + 2 <= g.width && y0 + kTile + 2 <= g.height;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                ry[k] = interior ? (py + k) * kTwoHalo
-                                 : (min(max(gy + k - 2, 0), g.height - 1) - (y0 - 2)) * kTwoHalo;
-                rx[k] = interior ? px + k : min(max(gx + k - 2, 0), g.width - 1) - (x0 - 2);
-            }
+            // ntile holds the clamped noisy pixel at every halo position, so tap (ky, kx)
+            // of this pixel sits at halo (py + ky, px + kx) also on border tiles: the 25
+            // reads are constant offsets from one address
+            const uint2* taps = &ntile[py * kTwoHalo + px];
             float sum = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -795,15 +802,17 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
                     if (c < kKernelTaps) {
                         const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
                         const float e = __builtin_amdgcn_exp2f(fmaf(zc, l2e, -zm));
-                        const float4 nv = ntile[ry[c / 5] + rx[c % 5]];
+                        // FP16 taps widen exactly inside the multiply-add (v_fma_mix_f32)
+                        const uint2 nv = taps[(c / 5) * kTwoHalo + c % 5];
+                        const half2v rg = __builtin_bit_cast(half2v, nv.x), ba = __builtin_bit_cast(half2v, nv.y);
                         sum += e;
-                        ox = fmaf(nv.x, e, ox);
-                        oy = fmaf(nv.y, e, oy);
-                        oz = fmaf(nv.z, e, oz);
+                        ox = fmaf((float)rg.x, e, ox);
+                        oy = fmaf((float)rg.y, e, oy);
+                        oz = fmaf((float)ba.x, e, oz);
                     }
                 }
             }
-            const float inv = 1.0f / sum;
+            const float inv = __builtin_amdgcn_rcpf(sum);  // 1 ulp
             ox *= inv;
             oy *= inv;
             oz *= inv;
