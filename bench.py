@@ -51,6 +51,9 @@ def parse():
                          "(default for the city; the per-kernel breakdown then comes from an eager pass over "
                          "the same frames after the timed region)")
     ap.add_argument("--eager", dest="graph", action="store_false", help="launch the frame's kernels one by one")
+    ap.add_argument("--scene", default=None, metavar="OBJ",
+                    help="trace a Wavefront OBJ/MTL file (pbrt_load_obj) instead of the generated scene, with the "
+                         "workload's camera path, image size and explicit lights")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -135,13 +138,13 @@ def denoise_comparison(torch, fr, frame, runs, evaluate):
     return out
 
 
-def cpu_baseline(gen, cfg, budget_s, frames):
+def cpu_baseline(gen, desc, cfg, budget_s, frames):
     """The oracle's FP32 restatement on the host cores, over bands of 1/8 of the image
     rows (from the middle, then the rest) of the timed camera frames, until the time
     budget is spent (at least one band)."""
     from oracle import oracle
     from vulkanpbrt_amd import camera
-    osc = oracle.OracleScene(gen.desc)
+    osc = oracle.OracleScene(desc)
     w, h = cfg["width"], cfg["height"]
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     threads = max(1, min(threads, 16))
@@ -190,8 +193,22 @@ def main():
     t_setup = time.perf_counter()
     gen = capi.GeneratedScene(args.config, cfg["seed"])
     bvh = args.bvh or cfg.get("bvh", "sah")
-    gen.desc.bvh_builder = capi.PBRT_BVH_LBVH_GPU if bvh == "lbvh" else capi.PBRT_BVH_SAH_HOST
-    scene = capi.Scene(gen.desc)
+    builder = capi.PBRT_BVH_LBVH_GPU if bvh == "lbvh" else capi.PBRT_BVH_SAH_HOST
+    gen.desc.bvh_builder = builder
+    desc, loaded, load_info = gen.desc, None, None
+    if args.scene:
+        t_load = time.perf_counter()
+        loaded = capi.LoadedScene(args.scene, bvh_builder=builder)
+        st = loaded.stats()
+        desc = loaded.desc
+        # a file carries no light sources but its emissive meshes (the reference does not
+        # visit vsg::Light nodes): the workload's explicit lights (the sun) are added
+        desc.light_count, desc.lights = gen.desc.light_count, gen.desc.lights
+        load_info = {"file": str(args.scene), "meshes": st.mesh_count, "triangles": st.triangle_count,
+                     "textures": st.texture_count, "emissive_meshes": st.emissive_mesh_count,
+                     "file_mb": round(st.file_bytes / 1e6, 1), "parse_ms": round(st.parse_ms, 1),
+                     "load_s": round(time.perf_counter() - t_load, 2)}
+    scene = capi.Scene(desc, keepalive=(loaded, gen))
     info = scene.info()
     fr = Frame(torch, capi, camera, gen, scene, w, h, cfg["depth"], args.tracer, args.bmfr_fit, args.denoiser)
     setup_s = time.perf_counter() - t_setup
@@ -302,7 +319,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seed-generated scene, random-free camera spline)",
+        "data": ("synthetic (seed-generated scene, random-free camera spline)" if not args.scene else
+                 "scene file read by pbrt_load_obj, the workload's camera spline"),
         "config": {"workload": cfg["name"], "width": w, "height": h, "spp": 1, "max_depth": cfg["depth"],
                    "triangles": info.triangle_count, "lights": info.light_count, "tracer": args.tracer,
                    "denoiser": args.denoiser, "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"],
@@ -321,6 +339,9 @@ def main():
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
     }
+    if load_info:
+        out["config"]["workload"] += " (geometry and materials from the scene file)"
+        out["scene_file"] = load_info
     if graph_info:
         out["hipgraph"] = graph_info
     if tuning:
@@ -329,7 +350,7 @@ def main():
     out["denoise_comparison"] = denoise_comparison(torch, fr, args.warmup + K - 1, max(K, 10),
                                                    not args.no_denoise_eval)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(gen, cfg, args.cpu_seconds, range(args.warmup, args.warmup + K))
+        out["cpu_baseline"] = cpu_baseline(gen, desc, cfg, args.cpu_seconds, range(args.warmup, args.warmup + K))
     print(json.dumps(out))
     if graph is not None:
         capi.check(lib.pbrt_frame_graph_destroy(graph))

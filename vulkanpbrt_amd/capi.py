@@ -1,4 +1,5 @@
-"""ctypes view of the C ABI in ``include/pbrt_c.h`` and ``include/pbrt_scenegen.h``.
+"""ctypes view of the C ABI in ``include/pbrt_c.h``, ``include/pbrt_scenegen.h`` and
+``include/pbrt_io.h``.
 
 The library ``libpbrt.so`` is built in-tree by :mod:`vulkanpbrt_amd.build`.
 Loading fails loudly when it is missing: there is no fallback path.
@@ -155,6 +156,34 @@ class CameraPose(C.Structure):
     ]
 
 
+class PhongMaterial(C.Structure):
+    """vsg::PhongMaterial (pbrt_phong_material)."""
+    _fields_ = [("ambient", C.c_float * 4), ("diffuse", C.c_float * 4), ("specular", C.c_float * 4),
+                ("emissive", C.c_float * 4), ("transmissive", C.c_float * 3), ("shininess", C.c_float),
+                ("alpha_mask", C.c_float), ("alpha_mask_cutoff", C.c_float), ("index_of_refraction", C.c_float),
+                ("category_id", C.c_uint32)]
+
+
+class PbrMaterial(C.Structure):
+    """vsg::PbrMaterial (pbrt_pbr_material)."""
+    _fields_ = [("base_color", C.c_float * 4), ("emissive", C.c_float * 4), ("diffuse", C.c_float * 4),
+                ("specular", C.c_float * 4), ("transmission", C.c_float * 3), ("metallic", C.c_float),
+                ("roughness", C.c_float), ("alpha_mask", C.c_float), ("alpha_mask_cutoff", C.c_float),
+                ("index_of_refraction", C.c_float), ("category_id", C.c_uint32)]
+
+
+class LoadOptions(C.Structure):
+    _fields_ = [("instance_count", C.c_uint32), ("bvh_builder", C.c_uint32),
+                ("instance_transforms", C.POINTER(C.c_float))]
+
+
+class LoadedSceneStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("mesh_count", "triangle_count", "vertex_count", "texture_count",
+                                          "emissive_mesh_count", "instance_count")] + [
+        ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("parse_ms", C.c_double),
+        ("file_bytes", C.c_uint64)]
+
+
 class AccumulateParams(C.Structure):
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("frame_number", C.c_uint32), ("reserved", C.c_uint32),
@@ -261,6 +290,16 @@ SIGNATURES = {
     "pbrt_generated_scene_desc": (C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)]),
     "pbrt_generated_scene_camera": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(CameraPose)]),
     "pbrt_generated_scene_destroy": (C.c_int32, [C.c_void_p]),
+    "pbrt_pack_phong_material": (C.c_int32, [C.POINTER(PhongMaterial), C.POINTER(Material), C.POINTER(C.c_uint32)]),
+    "pbrt_pack_pbr_material": (C.c_int32, [C.POINTER(PbrMaterial), C.POINTER(Material), C.POINTER(C.c_uint32)]),
+    "pbrt_load_obj": (C.c_int32, [C.c_char_p, C.POINTER(LoadOptions), C.POINTER(C.c_void_p)]),
+    "pbrt_loaded_scene_desc": (C.c_int32, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "pbrt_loaded_scene_stats_get": (C.c_int32, [C.c_void_p, C.POINTER(LoadedSceneStats)]),
+    "pbrt_loaded_scene_destroy": (C.c_int32, [C.c_void_p]),
+    "pbrt_image_load_rgba8": (C.c_int32, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.POINTER(C.c_uint8))]),
+    "pbrt_image_free": (None, [C.POINTER(C.c_uint8)]),
+    "pbrt_image_write_png": (C.c_int32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
 }
 
 _lib: C.CDLL | None = None
@@ -313,6 +352,32 @@ def _np_from(ptr_value, count, dtype):
     return np.frombuffer(buf, dtype=dtype).copy()
 
 
+def desc_arrays(d: SceneDesc) -> dict:
+    """Host copies of a scene description's arrays (input of the oracle)."""
+    meshes = []
+    for i in range(d.mesh_count):
+        m = d.meshes[i]
+        idt = np.uint16 if m.index_stride == 2 else np.uint32
+        meshes.append(dict(
+            positions=_np_from(m.positions, 3 * m.vertex_count, np.float32),
+            normals=_np_from(m.normals, 3 * m.vertex_count, np.float32),
+            uvs=_np_from(m.uvs, 2 * m.vertex_count, np.float32),
+            indices=_np_from(m.indices, m.index_count, idt),
+            material=np.frombuffer(bytes(m.material), dtype=np.uint8).copy(),
+            textures=np.array([m.diffuse_texture, m.normal_texture, m.specular_texture,
+                               m.emissive_texture], dtype=np.int32),
+        ))
+    instances = [(np.array(d.instances[i].transform, dtype=np.float32), d.instances[i].mesh)
+                 for i in range(d.instance_count)]
+    textures = []
+    for i in range(d.texture_count):
+        t = d.textures[i]
+        textures.append(_np_from(t.rgba8, 4 * t.width * t.height, np.uint8).reshape(t.height, t.width, 4))
+    lights = np.frombuffer(bytes((Light * d.light_count).from_address(C.addressof(d.lights.contents)))
+                           if d.light_count else b"", dtype=np.float32).reshape(-1, 32).copy()
+    return dict(meshes=meshes, instances=instances, textures=textures, lights=lights)
+
+
 class GeneratedScene:
     """A seed-generated workload (pbrt_generate_scene)."""
 
@@ -332,29 +397,7 @@ class GeneratedScene:
 
     def arrays(self) -> dict:
         """Host copies of the scene description (input of the oracle)."""
-        d = self.desc
-        meshes = []
-        for i in range(d.mesh_count):
-            m = d.meshes[i]
-            idt = np.uint16 if m.index_stride == 2 else np.uint32
-            meshes.append(dict(
-                positions=_np_from(m.positions, 3 * m.vertex_count, np.float32),
-                normals=_np_from(m.normals, 3 * m.vertex_count, np.float32),
-                uvs=_np_from(m.uvs, 2 * m.vertex_count, np.float32),
-                indices=_np_from(m.indices, m.index_count, idt),
-                material=np.frombuffer(bytes(m.material), dtype=np.uint8).copy(),
-                textures=np.array([m.diffuse_texture, m.normal_texture, m.specular_texture,
-                                   m.emissive_texture], dtype=np.int32),
-            ))
-        instances = [(np.array(d.instances[i].transform, dtype=np.float32), d.instances[i].mesh)
-                     for i in range(d.instance_count)]
-        textures = []
-        for i in range(d.texture_count):
-            t = d.textures[i]
-            textures.append(_np_from(t.rgba8, 4 * t.width * t.height, np.uint8).reshape(t.height, t.width, 4))
-        lights = np.frombuffer(bytes((Light * d.light_count).from_address(C.addressof(d.lights.contents)))
-                               if d.light_count else b"", dtype=np.float32).reshape(-1, 32).copy()
-        return dict(meshes=meshes, instances=instances, textures=textures, lights=lights)
+        return desc_arrays(self.desc)
 
     def __del__(self):
         try:
@@ -363,6 +406,63 @@ class GeneratedScene:
                 self.handle = None
         except Exception:
             pass
+
+
+class LoadedScene:
+    """A Wavefront OBJ/MTL scene read by pbrt_load_obj (the assimp front-end of the
+    reference), ready for :class:`Scene`."""
+
+    def __init__(self, path, transforms=None, bvh_builder: int = PBRT_BVH_SAH_HOST):
+        opt = LoadOptions()
+        opt.bvh_builder = bvh_builder
+        self._transforms = None
+        if transforms is not None:
+            t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float32).reshape(-1, 16))
+            self._transforms = t
+            opt.instance_count = t.shape[0]
+            opt.instance_transforms = t.ctypes.data_as(C.POINTER(C.c_float))
+        h = C.c_void_p()
+        check(lib().pbrt_load_obj(str(path).encode(), C.byref(opt), C.byref(h)))
+        self.handle = h
+        self.desc = SceneDesc()
+        check(lib().pbrt_loaded_scene_desc(self.handle, C.byref(self.desc)))
+
+    def stats(self) -> LoadedSceneStats:
+        st = LoadedSceneStats()
+        check(lib().pbrt_loaded_scene_stats_get(self.handle, C.byref(st)))
+        return st
+
+    def arrays(self) -> dict:
+        return desc_arrays(self.desc)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().pbrt_loaded_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_image(path) -> np.ndarray:
+    """An image file (PNG, TGA, PPM/PGM, PAM) as an (H, W, 4) uint8 RGBA array."""
+    w, h = C.c_uint32(), C.c_uint32()
+    px = C.POINTER(C.c_uint8)()
+    check(lib().pbrt_image_load_rgba8(str(path).encode(), C.byref(w), C.byref(h), C.byref(px)))
+    try:
+        return np.ctypeslib.as_array(px, shape=(h.value, w.value, 4)).copy()
+    finally:
+        lib().pbrt_image_free(px)
+
+
+def write_png(path, pixels: np.ndarray, bgra: bool = False) -> None:
+    """Write an (H, W, 4) uint8 array as an 8-bit RGBA PNG (`bgra` for B8G8R8A8 pixels)."""
+    a = np.ascontiguousarray(pixels, dtype=np.uint8)
+    assert a.ndim == 3 and a.shape[2] == 4
+    check(lib().pbrt_image_write_png(str(path).encode(), a.shape[1], a.shape[0], a.ctypes.data, int(bgra)))
 
 
 class Scene:
