@@ -110,13 +110,70 @@ int32_t pbrt_loaded_scene_stats_get(const pbrt_loaded_scene* scene, pbrt_loaded_
 int32_t pbrt_loaded_scene_destroy(pbrt_loaded_scene* scene);
 
 /* An image file as RGBA8 (first row at the top); free the pixels with
- * pbrt_image_free. */
+ * pbrt_io_free. */
 int32_t pbrt_image_load_rgba8(const char* path, uint32_t* width, uint32_t* height, uint8_t** pixels);
-void pbrt_image_free(uint8_t* pixels);
+/* Releases memory returned by the readers of this header. */
+void pbrt_io_free(void* memory);
 /* 8-bit RGBA PNG of `pixels`; `bgra` != 0 when the pixels are B8G8R8A8 (the
  * tracer's final image format, BMFR.cpp:81). */
 int32_t pbrt_image_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* pixels,
                              uint32_t bgra);
+
+/* ------------------------------------------------------------------------- */
+/* Offline frames (source/io/RenderIO.cpp): camera matrices, G-buffer and     */
+/* illumination images.  The reference stores images as OpenEXR through      */
+/* vsgXchange; this build uses PFM (portable float map) for the float images. */
+/* ------------------------------------------------------------------------- */
+
+/* CameraMatrices (source/io/RenderIO.hpp:27-33); column-major like vsg::mat4. */
+typedef struct pbrt_camera_matrices {
+    float view[16];     /* without a projection: the combined view-projection */
+    float inv_view[16];
+    float proj[16];     /* valid when has_proj */
+    float inv_proj[16];
+    uint32_t has_proj;  /* 1: "ModelView+Projection", 0: "ModelViewProjection" */
+    uint32_t pad[3];
+} pbrt_camera_matrices;
+
+/* MatrixIO::import_matrices (RenderIO.cpp:593-666): a .json path holds
+ * {"amtOfFrames": n, "matrices": [{"type", "storageType", "view", "invView",
+ * "proj", "invProj"}]} with 16 numbers per matrix in column order; any other path
+ * holds a plain list of 16-number matrices (commas and braces allowed), each a
+ * combined matrix whose inverse is computed.  Free *out with pbrt_io_free. */
+int32_t pbrt_matrices_read(const char* path, pbrt_camera_matrices** out, uint32_t* count);
+/* MatrixIO::export_matrices (RenderIO.cpp:668-716): the JSON form. */
+int32_t pbrt_matrices_write(const char* path, const pbrt_camera_matrices* matrices, uint32_t count);
+
+/* PFM with 1 ("Pf") or 3 ("PF") channels, little-endian.  `pixels` holds
+ * `stride` floats per pixel (>= channels), first row at the top; the file stores
+ * the bottom row first. */
+int32_t pbrt_pfm_write(const char* path, uint32_t width, uint32_t height, uint32_t channels, const float* pixels,
+                       uint32_t stride);
+/* Reads a PFM file into width * height * channels floats, first row at the top;
+ * free with pbrt_io_free. */
+int32_t pbrt_pfm_read(const char* path, uint32_t* width, uint32_t* height, uint32_t* channels, float** pixels);
+
+/* GBufferIO conversions over n pixels (RenderIO.cpp), in the reference's FP32
+ * arithmetic. */
+/* convert_normal_to_spherical (:160-178): (x, y, z, w) -> (acos z, atan2(y, x)) */
+int32_t pbrt_normals_to_spherical(const float* xyzw, uint64_t n, float* theta_phi);
+/* spherical_to_cartesian (:312-329): -> (cos phi sin theta, sin phi sin theta, cos theta, 1) */
+int32_t pbrt_spherical_to_normals(const float* theta_phi, uint64_t n, float* xyzw);
+/* compress_albedo, float input (:183-189): 255 * value, truncated (clamped to 0..255) */
+int32_t pbrt_albedo_from_float(const float* rgba, uint64_t n, uint8_t* rgba8);
+/* compress_albedo, 16-bit float input (:197-209), with its half-to-float bit
+ * conversion (no subnormal or infinity handling) */
+int32_t pbrt_albedo_from_half(const uint16_t* rgba, uint64_t n, uint8_t* rgba8);
+/* unorm_to_float (:331-346): value / 255 */
+int32_t pbrt_unorm_to_float(const uint8_t* rgba8, uint64_t n, float* rgba);
+/* depth_to_position (:348-382): the camera ray of each pixel centre (the tracer's
+ * createRay) scaled by the depth, from the camera position; w = 1.  Needs has_proj. */
+int32_t pbrt_depth_to_position(const float* depth, uint32_t width, uint32_t height,
+                               const pbrt_camera_matrices* matrices, float* xyzw);
+/* import_g_buffer_position (:96-110): depth = |camera - position|.  The camera is
+ * the translation column of inv_view (the reference divides column 2, the view
+ * axis, by its w, which is 0 for a view matrix: this build takes column 3). */
+int32_t pbrt_position_to_depth(const float* xyzw, uint64_t n, const pbrt_camera_matrices* matrices, float* depth);
 
 #ifdef __cplusplus
 }

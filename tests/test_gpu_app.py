@@ -8,6 +8,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from vulkanpbrt_amd import capi
+
 pytestmark = pytest.mark.gpu
 
 APP = Path(__file__).resolve().parents[1] / "vulkanpbrt_amd" / "pbrt_frame"
@@ -36,3 +38,64 @@ def test_app_eager_equals_graph(tmp_path, denoiser, block):
     assert np.all(eager[..., 3] == 255)
     # a lit scene: the final image is not black
     assert eager[..., :3].mean() > 5
+
+
+def _run(cmd, timeout=120):
+    print("running", " ".join(map(str, cmd)), flush=True)
+    r = subprocess.run([str(c) for c in cmd], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_app_exports_then_denoises_the_files(tmp_path):
+    """--export* (VulkanPBRT.cpp:100-120, 457-480) writes each frame's G-buffer,
+    illumination and matrices; a second run with --depths/--normals/... takes them as
+    external buffers (no tracing) and denoises them to the live run's final image,
+    within one code where the normals round through cartesian xyz."""
+    d = tmp_path
+    live = d / "live.bin"
+    _run([APP, "--config", "1", "--frames", "4", "--taa", "--raw", live,
+          "--exportDepth", d / "depth_%d.pfm", "--exportPosition", d / "position_%d.pfm",
+          "--exportNormal", d / "normal_%d.pfm", "--exportAlbedo", d / "albedo_%d.pfm",
+          "--exportMaterial", d / "material_%d.pfm", "--exportIllumination", d / "illum_%d.pfm",
+          "--exportMatrices", d / "matrices.json"])
+    assert (d / "material_3.pfm").exists() and (d / "matrices.json").exists()
+    a = np.fromfile(live, np.uint8).reshape(256, 256, 4)
+    for src in ("--depths", "--positions"):
+        off = d / f"off{src}.bin"
+        pattern = d / ("depth_%d.pfm" if src == "--depths" else "position_%d.pfm")
+        r = _run([APP, "-f", "4", "--taa", "--raw", off, src, pattern, "--normals", d / "normal_%d.pfm",
+                  "--albedos", d / "albedo_%d.pfm", "--illuminations", d / "illum_%d.pfm",
+                  "--matrices", d / "matrices.json"])
+        b = np.fromfile(off, np.uint8).reshape(256, 256, 4)
+        diff = np.abs(a.astype(int) - b.astype(int)).max(-1)
+        print(src, "identical", (diff == 0).mean(), "max", diff.max())
+        assert (diff <= 1).mean() >= 0.995 and (diff == 0).mean() >= 0.9
+        assert "for 0 triangles" in r.stderr  # nothing traced
+
+
+def test_app_png_pfm_and_statistics(tmp_path):
+    raw, png, pfm = tmp_path / "f.bin", tmp_path / "f.png", tmp_path / "f.pfm"
+    r = _run([APP, "--config", "1", "--frames", "3", "--taa", "--raw", raw, "--png", png, "--pfm", pfm])
+    img = np.fromfile(raw, np.uint8).reshape(256, 256, 4)
+    assert np.array_equal(capi.load_image(png), img[..., [2, 1, 0, 3]])
+    from vulkanpbrt_amd import frame_io
+    f = frame_io.read_pfm(pfm)
+    assert np.array_equal(f, img[..., [2, 1, 0]].astype(np.float32) / np.float32(255))
+    # the reference GUI's lines (Gui.hpp:39-44) for 5,144 triangles, 256x256, 2 * depth rays/pixel
+    assert "ms/frame" in r.stderr and "for 5144 triangles" in r.stderr
+    assert "Render size is 256 by 256 with 4 rays/pixel resulting in" in r.stderr
+    assert "Samples per pixel: 1" in r.stderr
+
+
+def test_app_traces_an_obj_file(tmp_path):
+    from vulkanpbrt_amd import scene_export
+    gen = capi.GeneratedScene(1, 1)
+    info = scene_export.export(gen.arrays(), tmp_path, "cornell")
+    a, b = tmp_path / "gen.bin", tmp_path / "obj.bin"
+    _run([APP, "--config", "1", "--frames", "2", "--raw", a])
+    r = _run([APP, "--config", "1", "--frames", "2", "--raw", b, "-i", info["obj"]])
+    assert f"for {info['triangles']} triangles" in r.stderr
+    ia, ib = (np.fromfile(p, np.uint8).reshape(256, 256, 4)[..., :3].astype(float) for p in (a, b))
+    print("final image means", ia.mean(), ib.mean())
+    assert abs(ia.mean() - ib.mean()) <= 0.02 * ia.mean()

@@ -172,6 +172,12 @@ class PbrMaterial(C.Structure):
                 ("index_of_refraction", C.c_float), ("category_id", C.c_uint32)]
 
 
+class CameraMatrices(C.Structure):
+    """CameraMatrices of RenderIO.hpp:27-33 (pbrt_camera_matrices), column-major."""
+    _fields_ = [("view", C.c_float * 16), ("inv_view", C.c_float * 16), ("proj", C.c_float * 16),
+                ("inv_proj", C.c_float * 16), ("has_proj", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
 class LoadOptions(C.Structure):
     _fields_ = [("instance_count", C.c_uint32), ("bvh_builder", C.c_uint32),
                 ("instance_transforms", C.POINTER(C.c_float))]
@@ -298,8 +304,21 @@ SIGNATURES = {
     "pbrt_loaded_scene_destroy": (C.c_int32, [C.c_void_p]),
     "pbrt_image_load_rgba8": (C.c_int32, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.POINTER(C.c_uint8))]),
-    "pbrt_image_free": (None, [C.POINTER(C.c_uint8)]),
+    "pbrt_io_free": (None, [C.c_void_p]),
     "pbrt_image_write_png": (C.c_int32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "pbrt_matrices_read": (C.c_int32, [C.c_char_p, C.POINTER(C.POINTER(CameraMatrices)), C.POINTER(C.c_uint32)]),
+    "pbrt_matrices_write": (C.c_int32, [C.c_char_p, C.POINTER(CameraMatrices), C.c_uint32]),
+    "pbrt_pfm_write": (C.c_int32, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "pbrt_pfm_read": (C.c_int32, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.POINTER(C.c_float))]),
+    "pbrt_normals_to_spherical": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbrt_spherical_to_normals": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbrt_albedo_from_float": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbrt_albedo_from_half": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbrt_unorm_to_float": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbrt_depth_to_position": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(CameraMatrices),
+                                           C.c_void_p]),
+    "pbrt_position_to_depth": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(CameraMatrices), C.c_void_p]),
 }
 
 _lib: C.CDLL | None = None
@@ -455,7 +474,7 @@ def load_image(path) -> np.ndarray:
     try:
         return np.ctypeslib.as_array(px, shape=(h.value, w.value, 4)).copy()
     finally:
-        lib().pbrt_image_free(px)
+        lib().pbrt_io_free(C.cast(px, C.c_void_p))
 
 
 def write_png(path, pixels: np.ndarray, bgra: bool = False) -> None:

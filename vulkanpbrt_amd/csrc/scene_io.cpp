@@ -951,7 +951,7 @@ extern "C" int32_t pbrt_image_load_rgba8(const char* path, uint32_t* w, uint32_t
     });
 }
 
-extern "C" void pbrt_image_free(uint8_t* pixels) { std::free(pixels); }
+extern "C" void pbrt_io_free(void* memory) { std::free(memory); }
 
 extern "C" int32_t pbrt_image_write_png(const char* path, uint32_t w, uint32_t h, const uint8_t* px, uint32_t bgra) {
     return guard([&] {

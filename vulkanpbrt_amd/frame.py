@@ -160,6 +160,21 @@ class Frame:
                             capi.ptr(self.final))
         capi.check(capi.lib().pbrt_taa(self.w, self.h, frame, C.byref(im), self.stream()))
 
+    def run_offline(self, frame, gbuffer, illumination, view, proj):
+        """The reference's offline path (use_external_buffers, VulkanPBRT.cpp:224-246 and
+        OfflineGBuffer::upload_to_g_buffer_command): the frame's G-buffer and illumination
+        come from files (vulkanpbrt_amd.frame_io) instead of the tracer; accumulation,
+        denoising and TAA then run as in run().  `view` / `proj` are the frame's camera
+        matrices (row-major)."""
+        t = self.torch
+        for k in ("depth", "normal", "albedo"):
+            self.img[k].copy_(t.from_numpy(np.ascontiguousarray(gbuffer[k])))
+        self.img["illumination"].copy_(t.from_numpy(np.ascontiguousarray(illumination, dtype=np.float32)))
+        self.accumulate(frame, view, proj)
+        self.denoise(frame)
+        self.taa(frame)
+        self.prev_view = view
+
     def run(self, frame, events=None, graph=None):
         """One frame; with `graph` (a pbrt_frame_graph handle) its launches are captured
         into one hipGraph and launched together (no per-stage events then)."""
