@@ -38,9 +38,9 @@ def parse():
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
     ap.add_argument("--autotune", action="store_true",
                     help="pick the tracer launch shape (pbrt_autotune_trace) on the first frame before timing")
-    ap.add_argument("--bvh", default=None, choices=["sah", "lbvh"],
-                    help="BVH builder: binned SAH on the host or the Morton-code LBVH on the GPU "
-                         "(default: SAH, LBVH for the 5M-triangle city)")
+    ap.add_argument("--bvh", default=None, choices=["sah", "sbvh", "lbvh"],
+                    help="BVH builder: binned SAH on the host, SAH with spatial splits on the host, or the "
+                         "Morton-code LBVH on the GPU (default: spatial splits, LBVH for the 5M-triangle city)")
     ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn", "cnn2"],
                     help="denoiser of the timed frame: BMFR 32x32, the 6-layer learned kernel-predicting network "
                          "or the fused 2-layer one")
@@ -192,8 +192,8 @@ def main():
     w, h = cfg["width"], cfg["height"]
     t_setup = time.perf_counter()
     gen = capi.GeneratedScene(args.config, cfg["seed"])
-    bvh = args.bvh or cfg.get("bvh", "sah")
-    builder = capi.PBRT_BVH_LBVH_GPU if bvh == "lbvh" else capi.PBRT_BVH_SAH_HOST
+    bvh = args.bvh or cfg.get("bvh", "sbvh")
+    builder = {"lbvh": capi.PBRT_BVH_LBVH_GPU, "sbvh": capi.PBRT_BVH_SBVH_HOST}.get(bvh, capi.PBRT_BVH_SAH_HOST)
     gen.desc.bvh_builder = builder
     desc, loaded, load_info = gen.desc, None, None
     if args.scene:

@@ -41,19 +41,25 @@ def scene_rays(gen, w, h, n_random, seed):
     return np.concatenate([cam, rnd]), pc
 
 
+@pytest.mark.parametrize("builder", ["lbvh", "sbvh"])
 @pytest.mark.parametrize("config", [1, 2])
-def test_lbvh_queries_match_sah(cuda, pbrt_lib, config):
+def test_lbvh_queries_match_sah(cuda, pbrt_lib, config, builder):
+    """The same for the spatial-split tree (PBRT_BVH_SBVH_HOST), whose leaves may hold a
+    triangle more than once: hits report the same (instance, primitive)."""
     gen = capi.GeneratedScene(config, config)
     gen.desc.bvh_builder = capi.PBRT_BVH_SAH_HOST
     sah = capi.Scene(gen.desc)
-    gen.desc.bvh_builder = capi.PBRT_BVH_LBVH_GPU
+    kind = capi.PBRT_BVH_LBVH_GPU if builder == "lbvh" else capi.PBRT_BVH_SBVH_HOST
+    gen.desc.bvh_builder = kind
     lb = capi.Scene(gen.desc)
     s, l = sah.info(), lb.info()
     print(f"config {config}: {l.triangle_count} triangles; SAH {s.bvh_node_count} nodes depth {s.bvh_depth} "
-          f"{s.bvh_build_ms:.1f} ms (host) | LBVH {l.bvh_node_count} nodes depth {l.bvh_depth} "
-          f"{l.bvh_build_ms:.2f} ms (device)")
-    assert s.bvh_builder == capi.PBRT_BVH_SAH_HOST and l.bvh_builder == capi.PBRT_BVH_LBVH_GPU
-    assert l.bvh_node_count == l.triangle_count - 1 and 1 < l.bvh_depth <= 32
+          f"{s.bvh_build_ms:.1f} ms (host) | {builder} {l.bvh_node_count} nodes depth {l.bvh_depth} "
+          f"{l.bvh_build_ms:.2f} ms")
+    assert s.bvh_builder == capi.PBRT_BVH_SAH_HOST and l.bvh_builder == kind
+    assert l.triangle_count == s.triangle_count and 1 < l.bvh_depth <= 32
+    if builder == "lbvh":
+        assert l.bvh_node_count == l.triangle_count - 1
     rays, pc = scene_rays(gen, 320, 180, 40000, config)
     hs, os_ = intersect(cuda, sah, rays)
     hl, ol = intersect(cuda, lb, rays)

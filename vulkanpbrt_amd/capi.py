@@ -18,6 +18,7 @@ PBRT_OK = 0
 # enums (pbrt_c.h)
 PBRT_BVH_SAH_HOST = 0
 PBRT_BVH_LBVH_GPU = 1
+PBRT_BVH_SBVH_HOST = 2
 LIGHT_SAMPLE_SURFACE_STRENGTH = 0
 LIGHT_SAMPLE_LIGHT_STRENGTH = 1
 LIGHT_SAMPLE_UNIFORM = 2

@@ -366,7 +366,8 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
     return guard([&] {
         PBRT_REQUIRE(desc != nullptr && out_scene != nullptr, "pbrt_scene_create: NULL argument");
         *out_scene = nullptr;
-        PBRT_REQUIRE(desc->bvh_builder == PBRT_BVH_SAH_HOST || desc->bvh_builder == PBRT_BVH_LBVH_GPU,
+        PBRT_REQUIRE(desc->bvh_builder == PBRT_BVH_SAH_HOST || desc->bvh_builder == PBRT_BVH_LBVH_GPU ||
+                         desc->bvh_builder == PBRT_BVH_SBVH_HOST,
                      "pbrt_scene_create: unknown bvh_builder");
         PBRT_REQUIRE(desc->instance_count > 0, "pbrt_scene_create: scene has no instances");
         auto s = std::make_unique<pbrt_scene>();
@@ -398,12 +399,17 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
             s->info.bvh_build_ms = r.build_ms;
             s->info.bvh_builder = PBRT_BVH_LBVH_GPU;
         } else {
-            s->bvh = build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
+            const bool spatial = desc->bvh_builder == PBRT_BVH_SBVH_HOST;
+            s->bvh = spatial ? build_bvh_sbvh(ps.world_tris.data(), ntri, 4, kTraversalStackSize)
+                             : build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
             PBRT_REQUIRE(s->bvh.depth <= (uint32_t)kTraversalStackSize,
                          "pbrt_scene_create: BVH deeper than the traversal stack");
-            // BVH-ordered triangle records
-            std::vector<float4> tri(3 * (size_t)ntri);
-            for (uint32_t k = 0; k < ntri; ++k) {
+            // BVH-ordered triangle records (one per reference: spatial splits repeat a
+            // triangle in several leaves)
+            PBRT_REQUIRE(s->bvh.tri_order.size() < (1u << 27), "pbrt_scene_create: more than 2^27 triangle references");
+            const uint32_t nref = (uint32_t)s->bvh.tri_order.size();
+            std::vector<float4> tri(3 * (size_t)nref);
+            for (uint32_t k = 0; k < nref; ++k) {
                 uint32_t t = s->bvh.tri_order[k];
                 const float* p = &ps.world_tris[9 * (size_t)t];
                 uint32_t inst = ps.tri_inst[t];
@@ -418,7 +424,7 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
             node_count = (uint32_t)(s->bvh.nodes.size() / 16);
             s->info.bvh_build_ms =
                 (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            s->info.bvh_builder = PBRT_BVH_SAH_HOST;
+            s->info.bvh_builder = spatial ? PBRT_BVH_SBVH_HOST : PBRT_BVH_SAH_HOST;
         }
         quantize_bvh_nodes(s->nodes, node_count);
         s->instances.upload(ps.instances.data(), ps.instances.size() * sizeof(DeviceInstance));

@@ -190,6 +190,15 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
     return pop_node(stack, stride, sp);
 }
 
+// A candidate at the same t as the current closest hit replaces it when its
+// (instance, primitive) is lower; nothing replaces "no hit" at exactly tmax.
+__device__ __forceinline__ bool wins_tie(const float4* __restrict__ tris, uint32_t best, uint32_t inst,
+                                         uint32_t prim) {
+    if (best == kNoHit) return false;
+    const uint32_t bi = __float_as_uint(tris[3 * (size_t)best].w), bp = __float_as_uint(tris[3 * (size_t)best + 1].w);
+    return inst < bi || (inst == bi && prim < bp);
+}
+
 // The triangles of one leaf (Moller-Trumbore, alpha test on alpha-tested
 // instances); closest hit keeps the nearest accepted t, any-hit returns true on
 // the first accepted candidate.
@@ -220,7 +229,15 @@ __device__ __forceinline__ bool test_leaf(const DeviceScene& sc, int node, const
         float v = dot(d, qv) * inv;
         if (!(v >= 0.0f && u + v <= 1.0f)) continue;
         float t = dot(e2, qv) * inv;
-        if (!(t >= r.tmin && t < hit.t)) continue;
+        if constexpr (kAnyHit) {
+            if (!(t >= r.tmin && t < hit.t)) continue;
+        } else {
+            // at exactly the current t the lower (instance, primitive) wins, so the
+            // closest hit does not depend on the order leaves are visited in (BVH
+            // builder, traversal order, triangles repeated by spatial splits)
+            if (!(t >= r.tmin && t <= hit.t)) continue;
+            if (t == hit.t && !wins_tie(tris, hit.tri, __float_as_uint(t0.w), __float_as_uint(t1.w))) continue;
+        }
         uint32_t flags = __float_as_uint(t2.w);
         if (flags & kTriFlagAlphaTested) {
             if (alpha_rejects(sc, __float_as_uint(t0.w), __float_as_uint(t1.w), u, v)) continue;
