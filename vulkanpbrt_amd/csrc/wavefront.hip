@@ -32,11 +32,13 @@ constexpr int kMaxWfBounces = 32;
 constexpr uint32_t kDeadPath = 0xFFFFFFFFu;
 
 struct WfBuffers {
-    uint4* rng;
-    float4* tp;      // throughput
-    float4* col;     // accumulated radiance
-    float4* alb;     // primary albedo (diffuse + specular)
-    int4* state;     // x: loop index i, y: transmissive depth, z: primary hit flag
+    float4* col;         // accumulated radiance, per path
+    float4* alb;         // primary albedo (diffuse + specular), per path
+    // path state carried with its ray through the queues (read and written at the
+    // queue slot, so coherently): RNG state, and throughput with w holding the
+    // loop index i (low 16 bits) and the transmissive depth (high 16 bits)
+    uint4* ray_rng[2];
+    float4* ray_tp[2];
     float4* ray_o[2];
     float4* ray_d[2];
     uint32_t* ray_path[2];
@@ -67,12 +69,11 @@ static size_t wf_layout(uint32_t w, uint32_t h, size_t queue_tiles, char* base, 
         return p;
     };
     WfBuffers t{};
-    t.rng = (uint4*)take(np * 16);
-    t.tp = (float4*)take(np * 16);
     t.col = (float4*)take(np * 16);
     t.alb = (float4*)take(np * 16);
-    t.state = (int4*)take(np * 16);
     for (int q = 0; q < 2; ++q) {
+        t.ray_rng[q] = (uint4*)take(n * 16);
+        t.ray_tp[q] = (float4*)take(n * 16);
         t.ray_o[q] = (float4*)take(n * 16);
         t.ray_d[q] = (float4*)take(n * 16);
         t.ray_path[q] = (uint32_t*)take(n * 4);
@@ -127,10 +128,9 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_generate(TraceConsts c, WfBuf
     b.ray_o[0][q] = make_float4(pos.x, pos.y, pos.z, 0.001f);
     b.ray_d[0][q] = make_float4(dir.x, dir.y, dir.z, 10000.0f);
     b.ray_path[0][q] = path;
-    b.rng[path] = make_uint4(re.x, re.y, re.z, re.w);
-    b.tp[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.ray_rng[0][q] = make_uint4(re.x, re.y, re.z, re.w);
+    b.ray_tp[0][q] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
     b.col[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    b.state[path] = make_int4(0, 0, 0, 0);
 }
 
 template <bool kCount, int kThreads>
@@ -163,7 +163,8 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
     uint32_t path = live ? b.ray_path[qi][q] : kDeadPath;
     live = live && path != kDeadPath;
     bool want_shadow = false, want_ray = false;
-    float4 sh_o{}, sh_d{}, sh_p{}, nr_o{}, nr_d{};
+    float4 sh_o{}, sh_d{}, sh_p{}, nr_o{}, nr_d{}, nr_tp{};
+    uint4 nr_rng{};
     if (live) {
         const float4 ro = b.ray_o[qi][q];
         const float4 rd = b.ray_d[qi][q];
@@ -182,13 +183,11 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
             shade_hit(sc, h, dir, s, pos, category);
         else
             miss_surface(s, pos, category);
-        const uint4 r4 = b.rng[path];
+        const uint4 r4 = b.ray_rng[qi][q];
         Rng re{r4.x, r4.y, r4.z, r4.w};
-        float4 tp4 = b.tp[path];
+        const float4 tp4 = b.ray_tp[qi][q];
         vec3 tp = make_vec3(tp4.x, tp4.y, tp4.z);
-        float4 c4 = b.col[path];
-        vec3 col = make_vec3(c4.x, c4.y, c4.z);
-        int4 st = b.state[path];
+        const uint32_t st = __float_as_uint(tp4.w);
         const vec3 o = bounce == 0 ? -normalize(dir) : -dir;
         if (bounce == 0) {
             vec3 albedo = s.diffuse_color + s.specular_color;
@@ -209,9 +208,11 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
                 reinterpret_cast<uchar4*>(albedo_img)[path] = a;
             }
         }
-        // next-event estimation
+        // next-event estimation; the path's radiance is read and written only when
+        // something is added (sky, emission), in the reference's order of additions
+        vec3 add_sky = make_vec3(0.0f);
         if (is_miss(s)) {
-            col += sky_color(-o) * tp;
+            add_sky = sky_color(-o) * tp;
         } else {
             LightSample ls = sample_light(sc, c.light_sampling, pos, s.normal, re);
             if (ls.valid) {
@@ -224,11 +225,18 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
                 }
             }
         }
-        col += bounce == 0 ? s.emissive_color : s.emissive_color * tp;
+        const vec3 add_emit = bounce == 0 ? s.emissive_color : s.emissive_color * tp;
+        if (add_sky != make_vec3(0.0f) || add_emit != make_vec3(0.0f)) {
+            const float4 c4 = b.col[path];
+            vec3 col = make_vec3(c4.x, c4.y, c4.z);
+            col += add_sky;
+            col += add_emit;
+            b.col[path] = make_float4(col.x, col.y, col.z, 0.0f);
+        }
         // the bounce loop of ptRaygen.rgen:66-75
         if (!is_miss(s)) {
-            int i = bounce == 0 ? 0 : st.x + 1;
-            int trans = st.y;
+            int i = bounce == 0 ? 0 : (int)(st & 0xFFFFu) + 1;
+            int trans = (int)(st >> 16);
             if (i < (int)c.max_depth && trans < 10) {
                 if (s.illumination_type == 7) {
                     --i;
@@ -251,26 +259,22 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
                     }
                     if (alive) {
                         want_ray = true;
-                        tp = ptp;
                         nr_o = make_float4(pos.x, pos.y, pos.z, 0.001f);
                         nr_d = make_float4(l.x, l.y, l.z, 10000.0f);
-                        st.x = i;
-                        st.y = trans;
+                        nr_tp = make_float4(ptp.x, ptp.y, ptp.z, __uint_as_float((uint32_t)i | (uint32_t)trans << 16));
+                        nr_rng = make_uint4(re.x, re.y, re.z, re.w);
                     }
                 }
             }
         }
-        b.rng[path] = make_uint4(re.x, re.y, re.z, re.w);
-        b.tp[path] = make_float4(tp.x, tp.y, tp.z, 0.0f);
-        b.col[path] = make_float4(col.x, col.y, col.z, 0.0f);
-        if (bounce == 0) st.z = isinf(pos.x) ? 0 : 1;
-        b.state[path] = st;
     }
     const uint32_t rs = wave_append(&b.counters[bounce + 1], want_ray);
     if (want_ray) {
         b.ray_o[qn][rs] = nr_o;
         b.ray_d[qn][rs] = nr_d;
         b.ray_path[qn][rs] = path;
+        b.ray_rng[qn][rs] = nr_rng;
+        b.ray_tp[qn][rs] = nr_tp;
     }
     const uint32_t ss = wave_append(&b.counters[kMaxWfBounces + bounce], want_shadow);
     if (want_shadow) {
