@@ -34,10 +34,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 5])
-    ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"])
+    ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"],
+                    help="tracer timed with --no-autotune")
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick the tracer launch shape (pbrt_autotune_trace) on the first frame before timing")
+    ap.add_argument("--autotune", dest="autotune", action="store_true", default=True,
+                    help="(default) pick the tracer and its launch shape (pbrt_autotune_trace) and BMFR's Gram "
+                         "tile shape (pbrt_autotune_bmfr) on the first frame before timing")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time --tracer with the library's default launch shapes")
     ap.add_argument("--bvh", default=None, choices=["sah", "sbvh", "lbvh"],
                     help="BVH builder: binned SAH on the host, SAH with spatial splits on the host, or the "
                          "Morton-code LBVH on the GPU (default: spatial splits, LBVH for the 5M-triangle city)")
@@ -347,6 +351,8 @@ def main():
     if tuning:
         out["autotune"] = tuning
         out["config"]["tracer"] = tuning["best"]["tracer"]
+        if "bmfr" in tuning:
+            out["config"]["bmfr_fit_threads"] = tuning["bmfr"]["best_fit_threads"]
     out["denoise_comparison"] = denoise_comparison(torch, fr, args.warmup + K - 1, max(K, 10),
                                                    not args.no_denoise_eval)
     if not args.no_cpu_baseline:

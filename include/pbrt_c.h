@@ -382,7 +382,10 @@ typedef struct pbrt_bmfr_params {
     uint32_t block_size;     /* 8, 16 or 32 (DenoisingBlockSize X8 / X16 / X32) */
     uint32_t frame_number;   /* push constant frameNumber */
     uint32_t fit;            /* enum pbrt_bmfr_fit */
-    uint32_t reserved[3];
+    uint32_t fit_threads;    /* Gram fits: workgroup size 256, 512 or 1024 (0 = 256), i.e. how a
+                                block's rows are tiled over the waves' MFMA Gram accumulations;
+                                pbrt_autotune_bmfr picks it */
+    uint32_t reserved[2];
 } pbrt_bmfr_params;
 
 typedef struct pbrt_bmfr_images {
@@ -404,11 +407,27 @@ typedef struct pbrt_bmfr_images {
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
+/* Autotuner of the Gram fits' tile shape: runs pbrt_bmfr once per workgroup size
+ * (256, 512, 1024 threads), `repeats` timed runs each after a warm-up, on `stream`,
+ * and returns the fastest as `best` (a copy of `params` with fit_threads set).  The
+ * fit must be PBRT_BMFR_FIT_GRAM_F16 or _F64.  No counterpart in the reference
+ * (BMFR.cpp:155-170 fixes fitting_kernel = 256).  Writes the images pbrt_bmfr writes;
+ * synchronizes the stream. */
+typedef struct pbrt_bmfr_tuning {
+    pbrt_bmfr_params best;
+    uint32_t candidates;
+    uint32_t fit_threads[4];
+    float ms[4];
+} pbrt_bmfr_tuning;
+int32_t pbrt_autotune_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, uint32_t repeats,
+                           pbrt_bmfr_tuning* out, void* stream);
+
 /* Batched BMFR: every block of many frames (any mix of resolutions) fitted in
  * one launch and post-processed in a second, instead of one BMFR dispatch chain
  * per frame (BMFR::add_dispatch_to_command_graph, BMFR.cpp:206-229).  All frames
  * share block_size and fit, and the fit is PBRT_BMFR_FIT_GRAM_F16 or _F64 (the
- * Householder path stages features per frame).  images.workspace of each frame
+ * Householder path stages features per frame); the batch fits with 256-thread
+ * workgroups whatever fit_threads says.  images.workspace of each frame
  * is ignored: the batch workspace holds the frame table and every frame's
  * weights.  The frame table is staged through a pinned host buffer owned by the
  * library; a call waits only for the previous call's table upload. */

@@ -106,8 +106,8 @@ def test_copy_back(cuda):
     assert cuda.equal(d, pd) and cuda.equal(s, ps) and cuda.equal(a, pa)
 
 
-def run_bmfr(cuda, fit, bs, frame, inputs, denoised):
-    p = capi.BmfrParams(W, H, bs, frame, fit)
+def run_bmfr(cuda, fit, bs, frame, inputs, denoised, fit_threads=0, tune=False):
+    p = capi.BmfrParams(W, H, bs, frame, fit, fit_threads)
     nbytes = C.c_size_t()
     capi.check(capi.lib().pbrt_bmfr_workspace_size(C.byref(p), C.byref(nbytes)))
     ws = cuda.zeros(nbytes.value, dtype=cuda.uint8, device="cuda")
@@ -116,6 +116,10 @@ def run_bmfr(cuda, fit, bs, frame, inputs, denoised):
     final = cuda.zeros((H, W, 4), dtype=cuda.uint8, device="cuda")
     im = capi.BmfrImages(*(capi.ptr(x) for x in (t["depth"], t["normal"], t["albedo"], t["motion"], t["spp"],
                                                   t["noisy"], den, final, ws)), nbytes.value)
+    if tune:
+        tun = capi.BmfrTuning()
+        capi.check(capi.lib().pbrt_autotune_bmfr(C.byref(p), C.byref(im), 2, C.byref(tun), helpers.stream(cuda)))
+        return tun
     capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), helpers.stream(cuda)))
     cuda.cuda.synchronize()
     wsn = ws.cpu().numpy()
@@ -180,13 +184,16 @@ def test_bmfr_householder_matches_oracle(cuda, oracle_lib, frames, bs):
     assert d_ok.mean() > 0.999 and f_ok.mean() > 0.999
 
 
+@pytest.mark.parametrize("threads", [256, 512, 1024])
 @pytest.mark.parametrize("fit", ["f64", "f16"])
-def test_bmfr_gram_mfma_vs_fp64_lsq(cuda, oracle_lib, frames, fit):
+def test_bmfr_gram_mfma_vs_fp64_lsq(cuda, oracle_lib, frames, fit, threads):
+    """Every Gram tile shape (workgroup size: the waves sharing a block's row chunks)
+    meets the same bound against FP64 least squares."""
     inp = bmfr_inputs(oracle_lib, frames)
     bs, frame = 32, 1
     denoised = np.zeros((2, H, W, 4), np.uint16)
     mode = capi.BMFR_FIT_GRAM_F64 if fit == "f64" else capi.BMFR_FIT_GRAM_F16
-    _, weights, _, final = run_bmfr(cuda, mode, bs, frame, inp, denoised)
+    _, weights, _, final = run_bmfr(cuda, mode, bs, frame, inp, denoised, threads)
     feats = oracle_lib.bmfr_pre(W, H, bs, frame, inp["depth"], inp["normal"], inp["noisy"])
     W64 = oracle_lib.bmfr_lsq64(W, H, bs, frame, feats)
     _, _, nbx, nby = oracle_lib.bmfr_geometry(W, H, bs)
@@ -209,6 +216,25 @@ def test_bmfr_gram_mfma_vs_fp64_lsq(cuda, oracle_lib, frames, fit):
         # the planar / sky blocks where cond(A^T A) ~ 1e10
         assert np.median(errs) < 1e-4 and errs.max() < 0.5
     assert np.isfinite(final).all()
+
+
+@pytest.mark.parametrize("bs", [32, 8])
+def test_bmfr_autotune_picks_a_measured_tile_shape(cuda, oracle_lib, frames, bs):
+    inp = bmfr_inputs(oracle_lib, frames)
+    tun = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16), tune=True)
+    cands = [(tun.fit_threads[i], tun.ms[i]) for i in range(tun.candidates)]
+    print(f"bs {bs}: gram workgroup sizes and ms {cands}, best {tun.best.fit_threads}")
+    assert [c[0] for c in cands] == [256, 512, 1024] and all(c[1] > 0 for c in cands)
+    assert tun.best.fit_threads == min(cands, key=lambda c: c[1])[0]
+    assert tun.best.fit == capi.BMFR_FIT_GRAM_F64 and tun.best.block_size == bs
+    # the tuned shape runs and matches the default one to FP64 rounding of the fit
+    _, w_best, _, _ = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16),
+                               tun.best.fit_threads)
+    _, w_def, _, _ = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16))
+    assert np.allclose(w_best, w_def, rtol=1e-4, atol=1e-5)
+    p = capi.BmfrParams(W, H, bs, 1, capi.BMFR_FIT_GRAM_F64, 384)
+    n = C.c_size_t()
+    assert capi.lib().pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) == -1  # not a supported size
 
 
 def test_taa_matches_oracle(cuda, oracle_lib):

@@ -215,7 +215,13 @@ BMFR_FIT_GRAM_F64 = 2
 
 class BmfrParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("block_size", C.c_uint32),
-                ("frame_number", C.c_uint32), ("fit", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("frame_number", C.c_uint32), ("fit", C.c_uint32), ("fit_threads", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class BmfrTuning(C.Structure):
+    _fields_ = [("best", BmfrParams), ("candidates", C.c_uint32), ("fit_threads", C.c_uint32 * 4),
+                ("ms", C.c_float * 4)]
 
 
 class BmfrImages(C.Structure):
@@ -272,6 +278,8 @@ SIGNATURES = {
     "pbrt_convert": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_bmfr_workspace_size": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(C.c_size_t)]),
     "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
+    "pbrt_autotune_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_uint32,
+                                        C.POINTER(BmfrTuning), C.c_void_p]),
     "pbrt_bmfr_batched_workspace_size": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.POINTER(C.c_size_t)]),
     "pbrt_bmfr_batched": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbrt_running_mean": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),

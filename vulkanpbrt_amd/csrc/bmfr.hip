@@ -230,7 +230,7 @@ typedef double double4v __attribute__((ext_vector_type(4)));
 constexpr int kGramCols = 16;  // 9 projected features + 3 projected rgb + 4 zero columns
 constexpr int kFitThreads = 256;
 
-template <int kRows, bool kF64>
+template <int kRows, bool kF64, int kWaves>
 struct GramLds {
     // F16: column-major fp16 hi / lo planes; F64: row-major fp32 rows
     union {
@@ -240,15 +240,19 @@ struct GramLds {
         } h;
         float f[kRows][kGramCols];
     } m;
-    double gram[4][kGramCols * kGramCols];
+    double gram[kWaves][kGramCols * kGramCols];  // one partial Gram matrix per wave
 };
 
-template <int kRows, bool kF64>
+// kThreads: workgroup size (256, 512 or 1024); each of its waves accumulates the
+// Gram matrix of every kThreads/64-th row chunk, so the thread count sets how the
+// block's rows are tiled over the matrix cores (pbrt_autotune_bmfr picks it).
+template <int kRows, bool kF64, int kThreads>
 __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
-    constexpr int kSub = (kRows + kFitThreads - 1) / kFitThreads;
+    constexpr int kSub = (kRows + kThreads - 1) / kThreads;
+    constexpr int kWaves = kThreads / 64;
     __shared__ float fscratch[16];
     __shared__ double dscratch[16];
-    __shared__ GramLds<kRows, kF64> L;
+    __shared__ GramLds<kRows, kF64, kWaves> L;
     __shared__ double dots[kAlphaSize];
     __shared__ float scale[12];
     __shared__ double G[12][12];
@@ -265,7 +269,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     PixelIn px[kSub];
     #pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kFitThreads;
+        const int r = tid + s * kThreads;
         if (r < kRows) {
             px[s] = load_pixel(g, im, basex + r / bs, basey + r % bs);
             dmin = fminf(dmin, px[s].depth);
@@ -277,7 +281,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     // 2. fp16-quantised features (the reference's R16F feature buffer) + noise
     #pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kFitThreads;
+        const int r = tid + s * kThreads;
         if (r >= kRows) continue;
         const int lx = r / bs, ly = r % bs;
         float z = (px[s].depth - mn) / (mx - mn + 1e-6f);
@@ -296,7 +300,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     for (int c = 0; c < kAlphaSize; ++c) d_local[c] = 0.0;
     #pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kFitThreads;
+        const int r = tid + s * kThreads;
         if (r >= kRows) continue;
         const double a0 = a[s][0];
         for (int c = 0; c < kAlphaSize; ++c) d_local[c] += a0 * (double)a[s][c];
@@ -314,7 +318,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     for (int c = 0; c < 12; ++c) amax[c] = 0.0f;
     #pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kFitThreads;
+        const int r = tid + s * kThreads;
         if (r >= kRows) continue;
         const double a0 = a[s][0];
         for (int c = 0; c < 12; ++c) {
@@ -334,7 +338,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     // 5. stage the scaled columns in LDS
     #pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kFitThreads;
+        const int r = tid + s * kThreads;
         if (r >= kRows) continue;
         for (int c = 0; c < kGramCols; ++c) {
             float v = c < 12 ? b[s][c] * scale[c] : 0.0f;
@@ -357,7 +361,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
         if (kF64) {
             constexpr int kChunks = kRows / 4;
             double4v d = {0.0, 0.0, 0.0, 0.0};
-            for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
+            for (int ch = wave; ch < kChunks; ch += kWaves) {
                 double x = (double)L.m.f[ch * 4 + grp][col];
                 d = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, d, 0, 0, 0);
             }
@@ -365,7 +369,7 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
             for (int i = 0; i < 4; ++i) acc[i] = d[i];
         } else {
             constexpr int kChunks = kRows / 32;
-            for (int ch = wave; ch < kChunks; ch += kFitThreads / 64) {
+            for (int ch = wave; ch < kChunks; ch += kWaves) {
                 const int row0 = ch * 32 + grp * 8;
                 half8 hi, lo;
 #pragma unroll
@@ -398,12 +402,16 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     //    w1..w9 and w0 = a0.(y - A'w') / a0.a0.
     constexpr int n = kFeatures - 1;  // projected features A1..A9
     constexpr int pw = kGramCols * kGramCols;
-    for (int e = tid; e < 12 * 12; e += kFitThreads) {
+    for (int e = tid; e < 12 * 12; e += kThreads) {
         const int i = e / 12, j = e % 12;
         const int a = i * kGramCols + j, b = j * kGramCols + i;
         const double* gr = &L.gram[0][0];
-        const double gij = gr[a] + gr[pw + a] + gr[2 * pw + a] + gr[3 * pw + a];
-        const double gji = gr[b] + gr[pw + b] + gr[2 * pw + b] + gr[3 * pw + b];
+        double gij = 0.0, gji = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            gij += gr[w * pw + a];
+            gji += gr[w * pw + b];
+        }
         G[i][j] = 0.5 * (gij + gji) / ((double)scale[i] * (double)scale[j]);
     }
     __syncthreads();
@@ -454,9 +462,9 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     }
 }
 
-template <int kRows, bool kF64>
-__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
-    bmfr_fit_gram_block<kRows, kF64>(g, im, blockIdx.x, blockIdx.y);
+template <int kRows, bool kF64, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
+    bmfr_fit_gram_block<kRows, kF64, kThreads>(g, im, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,7 +546,8 @@ __global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram_batched(const Bmf
     const BmfrGeom g = frames[f].g;
     const pbrt_bmfr_images im = frames[f].im;
     const uint32_t local = blockIdx.x - frames[f].first_block;
-    bmfr_fit_gram_block<kRows, kF64>(g, im, (int)(local % (uint32_t)g.blocks_x), (int)(local / (uint32_t)g.blocks_x));
+    bmfr_fit_gram_block<kRows, kF64, kFitThreads>(g, im, (int)(local % (uint32_t)g.blocks_x),
+                                                  (int)(local / (uint32_t)g.blocks_x));
 }
 
 __global__ void __launch_bounds__(1024) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
@@ -563,6 +572,8 @@ static BmfrGeom make_geom(const pbrt_bmfr_params& p) {
     PBRT_REQUIRE(p.fit == PBRT_BMFR_FIT_HOUSEHOLDER || p.fit == PBRT_BMFR_FIT_GRAM_F16 ||
                      p.fit == PBRT_BMFR_FIT_GRAM_F64,
                  "pbrt_bmfr: unknown fit");
+    PBRT_REQUIRE(p.fit_threads == 0 || p.fit_threads == 256 || p.fit_threads == 512 || p.fit_threads == 1024,
+                 "pbrt_bmfr: fit_threads must be 0, 256, 512 or 1024");
     g.width = (int)p.width;
     g.height = (int)p.height;
     g.bs = (int)p.block_size;
@@ -576,6 +587,36 @@ static BmfrGeom make_geom(const pbrt_bmfr_params& p) {
     g.frame = p.frame_number;
     g.fit_threads = g.bs == 8 ? 64 : 256;
     return g;
+}
+
+// The Gram fit for a block size and workgroup size (256, 512 or 1024 threads).
+template <bool kF64, int kThreads>
+static void launch_gram_rows(int bs, dim3 grid, hipStream_t st, const BmfrGeom& g, const pbrt_bmfr_images& im) {
+    if (bs == 32)
+        hipLaunchKernelGGL((k_bmfr_fit_gram<1024, kF64, kThreads>), grid, dim3(kThreads), 0, st, g, im);
+    else if (bs == 16)
+        hipLaunchKernelGGL((k_bmfr_fit_gram<256, kF64, kThreads>), grid, dim3(kThreads), 0, st, g, im);
+    else
+        hipLaunchKernelGGL((k_bmfr_fit_gram<64, kF64, kThreads>), grid, dim3(kThreads), 0, st, g, im);
+}
+
+template <bool kF64>
+static void launch_gram_threads(int bs, uint32_t threads, dim3 grid, hipStream_t st, const BmfrGeom& g,
+                                const pbrt_bmfr_images& im) {
+    if (threads == 1024)
+        launch_gram_rows<kF64, 1024>(bs, grid, st, g, im);
+    else if (threads == 512)
+        launch_gram_rows<kF64, 512>(bs, grid, st, g, im);
+    else
+        launch_gram_rows<kF64, 256>(bs, grid, st, g, im);
+}
+
+static void launch_gram_fit(bool f64, int bs, uint32_t threads, dim3 grid, hipStream_t st, const BmfrGeom& g,
+                            const pbrt_bmfr_images& im) {
+    if (f64)
+        launch_gram_threads<true>(bs, threads, grid, st, g, im);
+    else
+        launch_gram_threads<false>(bs, threads, grid, st, g, im);
 }
 
 static size_t feature_bytes(const BmfrGeom& g) { return (size_t)kAlphaSize * g.wp * g.hp * sizeof(uint16_t); }
@@ -616,22 +657,53 @@ extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* 
                 hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(256), 0, st, g);
             else
                 hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
-        } else if (p->fit == PBRT_BMFR_FIT_GRAM_F64) {
-            if (g.bs == 32)
-                hipLaunchKernelGGL((k_bmfr_fit_gram<1024, true>), grid, dim3(kFitThreads), 0, st, g, *im);
-            else if (g.bs == 16)
-                hipLaunchKernelGGL((k_bmfr_fit_gram<256, true>), grid, dim3(kFitThreads), 0, st, g, *im);
-            else
-                hipLaunchKernelGGL((k_bmfr_fit_gram<64, true>), grid, dim3(kFitThreads), 0, st, g, *im);
         } else {
-            if (g.bs == 32)
-                hipLaunchKernelGGL((k_bmfr_fit_gram<1024, false>), grid, dim3(kFitThreads), 0, st, g, *im);
-            else if (g.bs == 16)
-                hipLaunchKernelGGL((k_bmfr_fit_gram<256, false>), grid, dim3(kFitThreads), 0, st, g, *im);
-            else
-                hipLaunchKernelGGL((k_bmfr_fit_gram<64, false>), grid, dim3(kFitThreads), 0, st, g, *im);
+            launch_gram_fit(p->fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, p->fit_threads ? p->fit_threads : kFitThreads, grid,
+                            st, g, *im);
         }
         hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, *im);
+        PBRT_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int32_t pbrt_autotune_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* im, uint32_t repeats,
+                                      pbrt_bmfr_tuning* out, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(p && im && out, "pbrt_autotune_bmfr: NULL argument");
+        PBRT_REQUIRE(p->fit == PBRT_BMFR_FIT_GRAM_F16 || p->fit == PBRT_BMFR_FIT_GRAM_F64,
+                     "pbrt_autotune_bmfr: the fit must be a Gram fit");
+        PBRT_REQUIRE(repeats >= 1 && repeats <= 100, "pbrt_autotune_bmfr: repeats must be 1..100");
+        hipStream_t st = (hipStream_t)stream;
+        std::memset(out, 0, sizeof(*out));
+        hipEvent_t e0, e1;
+        PBRT_HIP(hipEventCreate(&e0));
+        PBRT_HIP(hipEventCreate(&e1));
+        float best_ms = 3.4e38f;
+        const uint32_t threads[3] = {256, 512, 1024};
+        int32_t status = PBRT_OK;
+        for (uint32_t t : threads) {
+            pbrt_bmfr_params c = *p;
+            c.fit_threads = t;
+            status = pbrt_bmfr(&c, im, stream);  // warm-up (and argument check)
+            if (status != PBRT_OK) break;
+            (void)hipEventRecord(e0, st);
+            for (uint32_t r = 0; r < repeats; ++r) (void)pbrt_bmfr(&c, im, stream);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)repeats;
+            out->fit_threads[out->candidates] = t;
+            out->ms[out->candidates] = ms;
+            ++out->candidates;
+            if (ms < best_ms) {
+                best_ms = ms;
+                out->best = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (status != PBRT_OK) fail(status, std::string("pbrt_autotune_bmfr: ") + pbrt_last_error());
         PBRT_HIP(hipGetLastError());
     });
 }

@@ -19,6 +19,7 @@ class Frame:
         self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
         self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
         self.block_threads, self.queue_paths = 0, 0
+        self.bmfr_threads = 0  # Gram-fit workgroup size (0: the library default)
         self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
         d = "cuda"
         t = torch
@@ -97,11 +98,23 @@ class Frame:
         self.tracer, self.block_threads, self.queue_paths = tun.best.tracer, tun.best.block_threads, \
             tun.best.queue_paths
         names = {capi.TRACER_MEGAKERNEL: "megakernel", capi.TRACER_WAVEFRONT: "wavefront"}
-        return {"best": {"tracer": names[tun.best.tracer], "block_threads": tun.best.block_threads,
-                         "queue_paths": tun.best.queue_paths},
-                "candidates": [{"tracer": names[tun.tracer[i]], "block_threads": tun.block_threads[i],
-                                "queue_paths": tun.queue_paths[i], "ms": round(tun.ms[i], 3)}
-                               for i in range(tun.candidates)]}
+        out = {"best": {"tracer": names[tun.best.tracer], "block_threads": tun.best.block_threads,
+                        "queue_paths": tun.best.queue_paths},
+               "candidates": [{"tracer": names[tun.tracer[i]], "block_threads": tun.block_threads[i],
+                               "queue_paths": tun.queue_paths[i], "ms": round(tun.ms[i], 3)}
+                              for i in range(tun.candidates)]}
+        if self.fit != capi.BMFR_FIT_HOUSEHOLDER:
+            # the Gram fit's tile shape, on a traced and accumulated first frame
+            self.run(0)
+            bt = capi.BmfrTuning()
+            capi.check(capi.lib().pbrt_autotune_bmfr(C.byref(self.bmfr_params(0)), C.byref(self.bmfr_images(0)),
+                                                     repeats, C.byref(bt), self.stream()))
+            self.bmfr_threads = bt.best.fit_threads
+            self.reset_history()
+            out["bmfr"] = {"best_fit_threads": bt.best.fit_threads,
+                           "candidates": [{"fit_threads": bt.fit_threads[i], "ms": round(bt.ms[i], 4)}
+                                          for i in range(bt.candidates)]}
+        return out
 
     def trace(self, pc, counters=None):
         capi = self.capi
@@ -130,14 +143,26 @@ class Frame:
         capi.check(capi.lib().pbrt_copy_back(self.w, self.h, capi.ptr(self.img["depth"]), None, None,
                                              capi.ptr(self.depth_hist[cur]), None, None, self.stream()))
 
-    def bmfr(self, frame):
+    def reset_history(self):
+        """Forget every previous frame (accumulation, BMFR and TAA history)."""
+        for t in (*self.depth_hist, *self.spp, *self.acc, self.denoised, self.taa_hist, self.motion):
+            t.zero_()
+        self.prev_view = None
+
+    def bmfr_params(self, frame):
+        return self.capi.BmfrParams(self.w, self.h, 32, frame, self.fit, self.bmfr_threads)
+
+    def bmfr_images(self, frame):
         capi = self.capi
         cur = frame & 1
-        p = capi.BmfrParams(self.w, self.h, 32, frame, self.fit)
-        im = capi.BmfrImages(capi.ptr(self.img["depth"]), capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]),
-                             capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]),
-                             capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
-        capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), self.stream()))
+        return capi.BmfrImages(capi.ptr(self.img["depth"]), capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]),
+                               capi.ptr(self.motion), capi.ptr(self.spp[cur]), capi.ptr(self.acc[cur]),
+                               capi.ptr(self.denoised), capi.ptr(self.bmfr_final), capi.ptr(self.ws), self.ws_bytes)
+
+    def bmfr(self, frame):
+        capi = self.capi
+        capi.check(capi.lib().pbrt_bmfr(C.byref(self.bmfr_params(frame)), C.byref(self.bmfr_images(frame)),
+                                        self.stream()))
 
     def cnn(self, frame, layers=6):
         capi = self.capi
