@@ -93,7 +93,7 @@ def copy_calibration(torch, mib):
 WORKLOADS = {
     1: dict(name="cornell box + 16 icospheres (5.1k tris), 256x256, 1 spp, 2 bounces, accumulator + BMFR + TAA",
             width=256, height=256, depth=2, seed=1, tag="cornell-256"),
-    2: dict(name="atrium (~267k tris, 8 textures, 3 area quads + sun), 1920x1080, 1 spp, 4 bounces, "
+    2: dict(name="atrium (~262k tris, 8 textures, 3 area quads + sun), 1920x1080, 1 spp, 4 bounces, "
                  "accumulator + BMFR 32x32 + TAA over the 64-frame camera spline",
             width=1920, height=1080, depth=4, seed=2, tag="atrium-1080p"),
     5: dict(name="city stress (~4.9M tris: 4096 towers, 65,536 emissive window quads = 131k triangle lights, "
