@@ -16,6 +16,20 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long-running CPU oracle work")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _gpu_runtime(request):
+    """When the session runs GPU tests, initialise the HIP runtime (through torch,
+    whose runtime libpbrt.so then shares) before any test loads the library or starts
+    a GPU child process."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
+    yield
+
+
 @pytest.fixture(scope="session")
 def pbrt_lib():
     from vulkanpbrt_amd import build, capi

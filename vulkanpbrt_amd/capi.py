@@ -7,6 +7,7 @@ Loading fails loudly when it is missing: there is no fallback path.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -347,6 +348,16 @@ def lib() -> C.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m vulkanpbrt_amd.build` "
                 "(there is no CPU fallback for the HIP path)")
+        # With torch imported, libamdhip64.so.7 is torch's copy and this library binds
+        # to it (one HIP runtime per process, as it must be: torch's device pointers
+        # and streams are passed in).  That runtime finds no device when its first
+        # call comes from here, so torch initialises it first.
+        torch = sys.modules.get("torch")
+        if torch is not None:
+            try:
+                torch.cuda.is_available()
+            except Exception:
+                pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

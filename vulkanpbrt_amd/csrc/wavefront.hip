@@ -36,7 +36,8 @@ struct WfBuffers {
     float4* alb;         // primary albedo (diffuse + specular), per path
     // path state carried with its ray through the queues (read and written at the
     // queue slot, so coherently): RNG state, and throughput with w holding the
-    // loop index i (low 16 bits) and the transmissive depth (high 16 bits)
+    // next bounce's loop index (i + 1 >= 0: a transmissive hit at i = 0 leaves
+    // i = -1; low 16 bits) and the transmissive depth (high 16 bits)
     uint4* ray_rng[2];
     float4* ray_tp[2];
     float4* ray_o[2];
@@ -235,7 +236,7 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
         }
         // the bounce loop of ptRaygen.rgen:66-75
         if (!is_miss(s)) {
-            int i = bounce == 0 ? 0 : (int)(st & 0xFFFFu) + 1;
+            int i = bounce == 0 ? 0 : (int)(st & 0xFFFFu);
             int trans = (int)(st >> 16);
             if (i < (int)c.max_depth && trans < 10) {
                 if (s.illumination_type == 7) {
@@ -261,7 +262,8 @@ __global__ void __launch_bounds__(kWfThreads) k_wf_shade(DeviceScene sc, TraceCo
                         want_ray = true;
                         nr_o = make_float4(pos.x, pos.y, pos.z, 0.001f);
                         nr_d = make_float4(l.x, l.y, l.z, 10000.0f);
-                        nr_tp = make_float4(ptp.x, ptp.y, ptp.z, __uint_as_float((uint32_t)i | (uint32_t)trans << 16));
+                        nr_tp = make_float4(ptp.x, ptp.y, ptp.z,
+                                            __uint_as_float((uint32_t)(i + 1) | (uint32_t)trans << 16));
                         nr_rng = make_uint4(re.x, re.y, re.z, re.w);
                     }
                 }

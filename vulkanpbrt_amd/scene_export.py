@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import io
+import json
 import time
 from pathlib import Path
 
@@ -134,7 +135,7 @@ def main():
     from vulkanpbrt_amd import capi
     gen = capi.GeneratedScene(args.config, args.seed if args.seed is not None else args.config)
     info = export(gen.arrays(), Path(args.out), f"config{args.config}")
-    print(info)
+    print(json.dumps(info))
 
 
 if __name__ == "__main__":
