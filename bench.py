@@ -44,7 +44,8 @@ def parse():
                     help="time --tracer with the library's default launch shapes")
     ap.add_argument("--bvh", default=None, choices=["sah", "sbvh", "lbvh"],
                     help="BVH builder: binned SAH on the host, SAH with spatial splits on the host, or the "
-                         "Morton-code LBVH on the GPU (default: spatial splits, LBVH for the 5M-triangle city)")
+                         "Morton-code LBVH on the GPU (default: spatial splits; binned SAH for the 5M-triangle "
+                         "city, whose small triangles gain nothing from spatial splits)")
     ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn", "cnn2"],
                     help="denoiser of the timed frame: BMFR 32x32, the 6-layer learned kernel-predicting network "
                          "or the fused 2-layer one")
@@ -99,7 +100,7 @@ WORKLOADS = {
     5: dict(name="city stress (~4.9M tris: 4096 towers, 65,536 emissive window quads = 131k triangle lights, "
                  "UNIFORM light sampling), 3840x2160, 1 spp, 8 bounces with Russian roulette, accumulator + "
                  "BMFR + TAA over the 128-frame camera path",
-            width=3840, height=2160, depth=8, seed=5, bvh="lbvh", graph=True, tag="city-2160p"),
+            width=3840, height=2160, depth=8, seed=5, bvh="sah", graph=True, tag="city-2160p"),
 }
 
 

@@ -7,9 +7,11 @@
 #include "bvh_host.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace pbrt {
 
@@ -132,23 +134,17 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
     for (uint32_t i = 0; i < n; ++i)
         for (int a = 0; a < 3; ++a) centroid[3 * i + a] = 0.5f * (tri_lo[3 * i + a] + tri_hi[3 * i + a]);
 
-    std::vector<BuildNode> nodes;
-    nodes.reserve(n > 0 ? 2 * (size_t)n : 1);
     struct Task {
         int32_t node;
         uint32_t begin, end;
         int depth;
     };
-    std::vector<Task> stack;
-    nodes.push_back(BuildNode{});
-    stack.push_back({0, 0, n, 1});
     uint32_t* order = out.tri_order.data();
-    uint32_t max_seen_depth = 1;
-
-    while (!stack.empty()) {
-        Task t = stack.back();
-        stack.pop_back();
-        BuildNode& nd = nodes[t.node];
+    // One node: its bounds, then a leaf or a split of its range (in place in `order`)
+    // into two child nodes appended to `nodes`, whose tasks go to `children`.
+    auto step = [&](const Task& t, std::vector<BuildNode>& nodes, std::vector<Task>& children,
+                    uint32_t& max_depth_seen) {
+        BuildNode& nd = nodes[(size_t)t.node];
         Box box, cbox;
         box.reset();
         cbox.reset();
@@ -162,11 +158,11 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
         }
         nd.box = box;
         uint32_t count = t.end - t.begin;
-        max_seen_depth = std::max<uint32_t>(max_seen_depth, (uint32_t)t.depth);
+        max_depth_seen = std::max<uint32_t>(max_depth_seen, (uint32_t)t.depth);
         if (count <= (uint32_t)max_leaf && count > 0 && (count <= 2 || t.depth >= max_depth)) {
             nd.first = t.begin;
             nd.count = count;
-            continue;
+            return;
         }
         // choose axis of largest centroid extent
         int axis = 0;
@@ -192,10 +188,11 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
             // binned SAH over all three axes; the best (axis, bin boundary) wins
             float best = std::numeric_limits<float>::infinity();
             int best_split = -1, best_axis = axis;
-            for (int ax = 0; ax < 3; ++ax) {
-                if (!(ext[ax] > 0.0f)) continue;
-                Box bin_box[kBins];
-                uint32_t bin_cnt[kBins];
+            Box bins[3][kBins];
+            uint32_t counts[3][kBins];
+            auto bin_axis = [&](int ax) {
+                Box* bin_box = bins[ax];
+                uint32_t* bin_cnt = counts[ax];
                 for (int b = 0; b < kBins; ++b) {
                     bin_box[b].reset();
                     bin_cnt[b] = 0;
@@ -210,6 +207,13 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
                     bin_box[b].grow(tb);
                     bin_cnt[b]++;
                 }
+            };
+            for (int ax = 0; ax < 3; ++ax)
+                if (ext[ax] > 0.0f) bin_axis(ax);
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!(ext[ax] > 0.0f)) continue;
+                const Box* bin_box = bins[ax];
+                const uint32_t* bin_cnt = counts[ax];
                 float right_area[kBins];
                 uint32_t right_cnt[kBins];
                 Box acc;
@@ -257,7 +261,7 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
         if (make_leaf) {
             nd.first = t.begin;
             nd.count = count;
-            continue;
+            return;
         }
         int32_t l = (int32_t)nodes.size();
         nodes.push_back(BuildNode{});
@@ -265,9 +269,77 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
         nodes.push_back(BuildNode{});
         nodes[t.node].left = l;
         nodes[t.node].right = r;
-        // push right first so the left subtree is processed (and laid out) first
-        stack.push_back({r, mid, t.end, t.depth + 1});
-        stack.push_back({l, t.begin, mid, t.depth + 1});
+        // right first: a LIFO consumer builds (and lays out) the left subtree first
+        children.push_back({r, mid, t.end, t.depth + 1});
+        children.push_back({l, t.begin, mid, t.depth + 1});
+    };
+    // Serial from the root until the pending ranges are small, then those subtrees in
+    // parallel, each into its own node array, merged at the end.  Splits depend on the
+    // data only, so the tree is the same for any thread count.
+    std::vector<BuildNode> nodes;
+    nodes.reserve(n > 0 ? 2 * (size_t)n : 1);
+    nodes.push_back(BuildNode{});
+    uint32_t max_seen_depth = 1;
+    const uint32_t small = std::max<uint32_t>(4096, n / 64);
+    std::vector<Task> pending{{0, 0, n, 1}}, subtrees, children;
+    while (!pending.empty()) {
+        const Task t = pending.back();
+        pending.pop_back();
+        if (t.end - t.begin <= small) {
+            subtrees.push_back(t);
+            continue;
+        }
+        children.clear();
+        step(t, nodes, children, max_seen_depth);
+        pending.insert(pending.end(), children.begin(), children.end());
+    }
+    struct Part {
+        std::vector<BuildNode> nodes;
+        uint32_t depth = 1;
+    };
+    std::vector<Part> parts(subtrees.size());
+    auto build_subtree = [&](size_t k) {
+        Part& part = parts[k];
+        part.nodes.push_back(BuildNode{});
+        std::vector<Task> stack{{0, subtrees[k].begin, subtrees[k].end, subtrees[k].depth}}, kids;
+        while (!stack.empty()) {
+            const Task t = stack.back();
+            stack.pop_back();
+            kids.clear();
+            step(t, part.nodes, kids, part.depth);
+            stack.insert(stack.end(), kids.begin(), kids.end());
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nthreads = (unsigned)std::min<size_t>(hw, subtrees.size());
+    if (nthreads <= 1) {
+        for (size_t k = 0; k < subtrees.size(); ++k) build_subtree(k);
+    } else {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        for (unsigned i = 0; i < nthreads; ++i)
+            pool.emplace_back([&] {
+                for (size_t k; (k = next.fetch_add(1)) < subtrees.size();) build_subtree(k);
+            });
+        for (std::thread& th : pool) th.join();
+    }
+    // merge: a part's node 0 is the subtree root already allocated in `nodes`
+    for (size_t k = 0; k < parts.size(); ++k) {
+        const int32_t root = subtrees[k].node;
+        const int32_t base = (int32_t)nodes.size() - 1;
+        auto map = [&](int32_t c) { return c <= 0 ? (c == 0 ? root : c) : c + base; };
+        for (size_t i = 0; i < parts[k].nodes.size(); ++i) {
+            BuildNode b = parts[k].nodes[i];
+            if (b.count == 0) {
+                b.left = map(b.left);
+                b.right = map(b.right);
+            }
+            if (i == 0)
+                nodes[(size_t)root] = b;
+            else
+                nodes.push_back(b);
+        }
+        max_seen_depth = std::max(max_seen_depth, parts[k].depth);
     }
     out.depth = max_seen_depth;
     emit_layout(nodes, n > 0, out);
