@@ -1,0 +1,216 @@
+"""Edge cases of the frame path on the GPU, against the CPU oracle (through the C ABI):
+
+* ragged frames: widths and heights that are not multiples of the tracer's 8x8 tiles
+  or of the BMFR / BFR blocks (the block grid is shifted per frame and mirrored at
+  the borders, bmfrGeneral.comp), down to a single traced pixel; BMFR's smallest
+  frame is two blocks each way, smaller ones are rejected;
+* empty inputs: zero rays to the traversal queries, zero-size images (rejected on the
+  host with an error code, nothing launched).
+Bars are those of the full-size parity tests (tests/test_gpu_trace.py,
+tests/test_gpu_post.py)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from vulkanpbrt_amd import camera, capi
+from tests import helpers
+from tests.test_gpu_post import acc_consts, half, half_key, unhalf
+from tests.test_gpu_trace import _compare_frame
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg1(cuda, pbrt_lib, oracle_lib):
+    gen = capi.GeneratedScene(1, 1)
+    return gen, capi.Scene(gen.desc), oracle_lib.OracleScene(gen.desc)
+
+
+@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("w,h", [(67, 45), (1, 1), (9, 130)])
+def test_tracers_on_ragged_frames(cuda, cfg1, tracer, w, h):
+    gen, scene, osc = cfg1
+    pc, _, _ = helpers.frame_push_constants(gen, w, h, 3)
+    img = helpers.gbuffer(cuda, w, h)
+    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    helpers.trace(cuda, scene, pc, img, w, h, tracer=kind)
+    cuda.cuda.synchronize()
+    gpu = helpers.to_np(img)
+    ref = osc.trace(w, h, np.array(pc.view_inverse[:], np.float32), np.array(pc.proj_inverse[:], np.float32),
+                    frame=3, precision=32)
+    frac = _compare_frame(gpu, ref, f"{tracer} {w}x{h}")
+    if w * h == 1:
+        assert frac["depth"] == 1.0 and frac["albedo"] == 1.0
+    else:
+        assert frac["depth"] >= 0.999 and frac["albedo"] >= 0.999 and frac["radiance"] >= 0.99
+
+
+def _frames(osc, w, h):
+    """Two consecutive frames of the oracle at w x h, the camera moved slightly."""
+    gen = capi.GeneratedScene(1, 1)
+    pose = gen.camera(0)
+    out, views = [], []
+    for k, dx in enumerate([0.0, 0.02]):
+        view = camera.look_at([pose.eye[0] + dx, pose.eye[1], pose.eye[2]], list(pose.center), list(pose.up))
+        proj = camera.perspective(60, w / h, 0.1, 1000)
+        pc = camera.push_constants(view, proj, views[-1] if views else view, k)
+        views.append(view)
+        r = osc.trace(w, h, np.array(pc.view_inverse[:], np.float32), np.array(pc.proj_inverse[:], np.float32),
+                      frame=k)
+        r["view"], r["proj"] = view, proj
+        out.append(r)
+    return out
+
+
+def _dev(cuda, a):
+    return cuda.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _accumulate(cuda, w, h, frame, f, prev, prev_depth, prev_spp, prev_acc):
+    inv_proj, inv_view, prev_view, o = acc_consts(f, prev)
+    p = capi.AccumulateParams(w, h, frame, 0)
+    p.inv_proj[:], p.inv_view[:], p.prev_view[:], p.prev_origin[:] = (
+        inv_proj.tolist(), inv_view.tolist(), prev_view.tolist(), o.tolist())
+    t = [_dev(cuda, a) for a in (f["illumination"], f["depth"], prev_depth, prev_spp, prev_acc)]
+    motion = cuda.zeros((h, w, 2), dtype=cuda.int16, device="cuda")
+    spp = cuda.zeros((h, w), dtype=cuda.uint8, device="cuda")
+    acc = cuda.zeros((h, w, 4), dtype=cuda.int16, device="cuda")
+    im = capi.AccumulateImages(*(capi.ptr(x) for x in t + [motion, spp, acc]))
+    capi.check(capi.lib().pbrt_accumulate(C.byref(p), C.byref(im), helpers.stream(cuda)))
+    cuda.cuda.synchronize()
+    return motion.cpu().numpy().view(np.uint16), spp.cpu().numpy(), acc.cpu().numpy().view(np.uint16)
+
+
+def _bmfr(cuda, w, h, fit, bs, frame, inp, denoised):
+    p = capi.BmfrParams(w, h, bs, frame, fit, 0)
+    nbytes = C.c_size_t()
+    capi.check(capi.lib().pbrt_bmfr_workspace_size(C.byref(p), C.byref(nbytes)))
+    ws = cuda.zeros(nbytes.value, dtype=cuda.uint8, device="cuda")
+    t = {k: _dev(cuda, v) for k, v in inp.items()}
+    den = _dev(cuda, denoised)
+    final = cuda.zeros((h, w, 4), dtype=cuda.uint8, device="cuda")
+    im = capi.BmfrImages(*(capi.ptr(x) for x in (t["depth"], t["normal"], t["albedo"], t["motion"], t["spp"],
+                                                  t["noisy"], den, final, ws)), nbytes.value)
+    capi.check(capi.lib().pbrt_bmfr(C.byref(p), C.byref(im), helpers.stream(cuda)))
+    cuda.cuda.synchronize()
+    wsn = ws.cpu().numpy()
+    wp, hp = (w // bs + 2) * bs, (h // bs + 2) * bs
+    fbytes = (13 * wp * hp * 2 + 255) // 256 * 256
+    feats = wsn[:13 * wp * hp * 2].view(np.uint16).reshape(13, hp, wp)
+    weights = wsn[fbytes:fbytes + 4 * 30 * (hp // bs) * (wp // bs)].view(np.float32).reshape(hp // bs, wp // bs, 30)
+    return feats, weights, den.cpu().numpy().view(np.uint16), final.cpu().numpy()
+
+
+@pytest.mark.parametrize("w,h", [(77, 71), (131, 65)])  # BMFR needs two 32-pixel blocks each way
+def test_post_chain_on_ragged_frames(cuda, cfg1, oracle_lib, w, h):
+    """accumulate -> BMFR (Householder and FP64 Gram fits) -> BFR -> TAA -> convert."""
+    _, _, osc = cfg1
+    f0, f1 = _frames(osc, w, h)
+    zeros = (np.zeros((h, w), np.float32), np.zeros((h, w), np.uint8), np.zeros((h, w, 4), np.uint16))
+    # accumulator: frame 0 bit-exact, frame 1 (reprojection) to the full-size bar
+    g0 = _accumulate(cuda, w, h, 0, f0, f0, *zeros)
+    r0 = oracle_lib.accumulate(w, h, 0, *acc_consts(f0, f0), f0["illumination"], f0["depth"], *zeros)
+    for g, r in zip(g0, r0):
+        assert np.array_equal(g, r)
+    g1 = _accumulate(cuda, w, h, 1, f1, f0, f0["depth"], g0[1], g0[2])
+    motion, spp, acc = oracle_lib.accumulate(w, h, 1, *acc_consts(f1, f0), f1["illumination"], f1["depth"],
+                                             f0["depth"], r0[1], r0[2])
+    acc_ok = np.all(np.isclose(unhalf(g1[2]), unhalf(acc), rtol=2e-3, atol=1e-4), axis=-1)
+    assert acc_ok.mean() > 0.999
+    inp = dict(depth=f1["depth"], normal=f1["normal"], albedo=f1["albedo"], motion=motion, spp=spp, noisy=acc)
+    rng = np.random.default_rng(7)
+    denoised = half(rng.uniform(0, 2, (2, h, w, 4)))
+    layer = 0  # frame 1 writes layer (1 & 1) ^ 1
+    for bs in (32, 8):
+        feats, weights, den, final = _bmfr(cuda, w, h, capi.BMFR_FIT_HOUSEHOLDER, bs, 1, inp, denoised)
+        ofeats = oracle_lib.bmfr_pre(w, h, bs, 1, inp["depth"], inp["normal"], inp["noisy"])
+        ulps = np.abs(half_key(feats) - half_key(ofeats))
+        assert (ulps == 0).mean() > 0.99 and ulps.max() <= 1
+        oden, ofinal = oracle_lib.bmfr_post(w, h, bs, 1, inp["depth"], inp["normal"], inp["albedo"], inp["motion"],
+                                            inp["spp"], inp["noisy"], weights, denoised)
+        d_ok = np.isclose(unhalf(den[layer]), unhalf(oden[layer]), rtol=2e-3, atol=1e-3).all(-1)
+        f_ok = (np.abs(final.astype(int) - ofinal.astype(int)) <= 1).all(-1)
+        print(f"{w}x{h} bmfr bs {bs}: features max ulp {ulps.max()}, post {d_ok.mean():.5f} {f_ok.mean():.5f}")
+        assert d_ok.mean() > 0.999 and f_ok.mean() > 0.999
+        # FP64 Gram fit against FP64 least squares on the oracle's features
+        _, wg, _, _ = _bmfr(cuda, w, h, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, denoised)
+        # and the reference's FP32 Householder fit (the oracle's) on the same blocks
+        W64 = oracle_lib.bmfr_lsq64(w, h, bs, 1, ofeats)
+        W32 = oracle_lib.bmfr_fit(w, h, bs, 1, ofeats, subgroup=64)
+        _, _, nbx, nby = oracle_lib.bmfr_geometry(w, h, bs)
+        worst, worst_ref = 0.0, 0.0
+        for by in range(nby):
+            for bx in range(nbx):
+                A = oracle_lib.bmfr_block_matrix(w, h, bs, 1, ofeats, bx, by).astype(np.float64)[:, :10]
+                f64 = A @ W64[by, bx].reshape(10, 3)
+                scale = max(1e-2, np.abs(f64).max())
+                worst = max(worst, np.abs(A @ wg[by, bx].reshape(10, 3).astype(np.float64) - f64).max() / scale)
+                worst_ref = max(worst_ref, np.abs(A @ W32[by, bx].reshape(10, 3).astype(np.float64) - f64).max() / scale)
+        print(f"{w}x{h} bs {bs}: fitted-value rel error vs FP64 LSQ: gram-f64 {worst:.2e}, "
+              f"oracle fp32 householder {worst_ref:.2e}")
+        # 64-pixel blocks are near-singular (the Gram solve drops a feature whose pivot
+        # falls below 1e-15 of the largest): there the bar is the reference fit's own
+        # error (~9e-4 on these frames), divided by four
+        assert worst < max(5e-5, worst_ref / 4)
+    # BFR (16-pixel blocks) on the same inputs
+    p = capi.BfrParams(w, h, 16, 1)
+    t = {k: _dev(cuda, v) for k, v in inp.items()}
+    den = _dev(cuda, denoised)
+    final = cuda.zeros((h, w, 4), dtype=cuda.uint8, device="cuda")
+    im = capi.BfrImages(*(capi.ptr(x) for x in (t["depth"], t["normal"], t["albedo"], t["motion"], t["spp"],
+                                                 t["noisy"], den, final)), None)
+    capi.check(capi.lib().pbrt_bfr(C.byref(p), C.byref(im), helpers.stream(cuda)))
+    cuda.cuda.synchronize()
+    oden, ofinal, _ = oracle_lib.bfr(w, h, 16, 1, inp["depth"], inp["normal"], inp["albedo"], inp["motion"],
+                                     inp["spp"], inp["noisy"], denoised)
+    f_ok = (np.abs(final.cpu().numpy().astype(int) - ofinal.astype(int)) <= 1).all(-1)
+    print(f"{w}x{h} bfr: final {f_ok.mean():.5f}")
+    assert f_ok.mean() > 0.99
+    # TAA over the BMFR output, then the format converter
+    hist = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    m, d, hh = _dev(cuda, motion), _dev(cuda, ofinal), _dev(cuda, hist)
+    out = cuda.zeros((h, w, 4), dtype=cuda.uint8, device="cuda")
+    capi.check(capi.lib().pbrt_taa(w, h, 1, C.byref(capi.TaaImages(*(capi.ptr(x) for x in (m, d, hh, out)))),
+                                   helpers.stream(cuda)))
+    cuda.cuda.synchronize()
+    rf, _ = oracle_lib.taa(w, h, 1, motion, ofinal, hist)
+    assert ((np.abs(out.cpu().numpy().astype(int) - rf.astype(int)) <= 1).all(-1)).mean() > 0.999
+    src = rng.uniform(-0.5, 1.5, (h, w, 4)).astype(np.float32)
+    s = _dev(cuda, src)
+    out = cuda.zeros((h, w, 4), dtype=cuda.uint8, device="cuda")
+    capi.check(capi.lib().pbrt_convert(w, h, capi.ptr(s), capi.ptr(out), helpers.stream(cuda)))
+    cuda.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), oracle_lib.convert(w, h, src))
+
+
+def test_empty_inputs(cuda, cfg1):
+    _, scene, _ = cfg1
+    lib, st = capi.lib(), helpers.stream(cuda)
+    # zero rays: nothing to do, success, NULL buffers allowed
+    assert lib.pbrt_intersect(scene.handle, None, 0, None, None, st) == 0
+    assert lib.pbrt_occluded(scene.handle, None, 0, None, st) == 0
+    # zero-size images: rejected on the host before any launch
+    buf = cuda.zeros(64, dtype=cuda.uint8, device="cuda")
+    taa = capi.TaaImages(*([capi.ptr(buf)] * 4))
+    assert lib.pbrt_taa(0, 16, 0, C.byref(taa), st) != 0
+    assert b"zero-size" in lib.pbrt_last_error()
+    p = capi.AccumulateParams(16, 0, 0, 0)
+    assert lib.pbrt_accumulate(C.byref(p), C.byref(capi.AccumulateImages()), st) != 0
+    n = C.c_size_t()
+    assert lib.pbrt_trace_workspace_size(0, 8, C.byref(n)) != 0
+    # BMFR's mirror addressing needs two blocks each way: smaller frames are rejected
+    p = capi.BmfrParams(63, 200, 32, 0, capi.BMFR_FIT_GRAM_F64, 0)
+    assert lib.pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) != 0
+    assert b"two blocks" in lib.pbrt_last_error()
+    cuda.cuda.synchronize()
+    # the library still works afterwards
+    rays = np.zeros((1, 8), np.float32)
+    rays[0, :3], rays[0, 3], rays[0, 4:7], rays[0, 7] = [0.0, 0.0, 1.0], 1e-3, [0.0, 0.0, -1.0], 1e4
+    r = _dev(cuda, rays)
+    hits = cuda.zeros((1, 4), dtype=cuda.int32, device="cuda")
+    bary = cuda.zeros((1, 2), dtype=cuda.float32, device="cuda")
+    assert lib.pbrt_intersect(scene.handle, capi.ptr(r), 1, capi.ptr(hits), capi.ptr(bary), st) == 0
+    cuda.cuda.synchronize()
+    hit = hits.cpu().numpy().view(np.uint32)[0]
+    assert hit[1] != 0xFFFFFFFF and 0.0 < hit[:1].view(np.float32)[0] <= 1.0  # the box's floor is at z = 0
