@@ -114,12 +114,24 @@ __device__ __forceinline__ float safe_rcp(float d) {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
 }
 
-// A ray prepared for slab tests: origin, direction, reciprocal direction and
-// origin * reciprocal.
+// A ray prepared for slab tests: origin, direction, reciprocal direction,
+// origin * reciprocal, and the byte selectors (v_perm_b32) that pick each child's
+// near and far planes out of a quantized node for this ray's direction octant.
 struct TraversalRay {
     vec3 o, d;
     float ix, iy, iz, ox, oy, oz, tmin;
+    uint32_t sel_near0, sel_far0, sel_near1, sel_far1;
 };
+
+// Byte positions of the child boxes in a quantized node's second half (x, y, z
+// words): child 0 as bytes 0..5 of (y:x) = lo x y z, hi x y z; child 1 as bytes
+// 2..7 of (z:y).  Selector byte 0x0c yields zero.
+__device__ __forceinline__ uint32_t plane_selector(bool neg_x, bool neg_y, bool neg_z, uint32_t lo, bool far) {
+    const uint32_t hi = lo + 3;
+    const uint32_t sx = (neg_x != far) ? hi : lo, sy = (neg_y != far) ? hi + 1 : lo + 1,
+                   sz = (neg_z != far) ? hi + 2 : lo + 2;
+    return sx | sy << 8 | sz << 16 | 0x0c000000u;
+}
 
 __device__ __forceinline__ TraversalRay make_traversal_ray(vec3 o, vec3 d, float tmin) {
     TraversalRay r;
@@ -132,6 +144,12 @@ __device__ __forceinline__ TraversalRay make_traversal_ray(vec3 o, vec3 d, float
     r.oy = o.y * r.iy;
     r.oz = o.z * r.iz;
     r.tmin = tmin;
+    // safe_rcp never returns 0 or NaN, so the sign decides which plane is near
+    const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
+    r.sel_near0 = plane_selector(nx, ny, nz, 0, false);
+    r.sel_far0 = plane_selector(nx, ny, nz, 0, true);
+    r.sel_near1 = plane_selector(nx, ny, nz, 2, false);
+    r.sel_far1 = plane_selector(nx, ny, nz, 2, true);
     return r;
 }
 
@@ -150,6 +168,14 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
 // (bound = fma(q, 2^e, origin), exactly as the quantization pass checked them)
 // and slab-test them against [tmin, tcur]; returns the nearer hit child (the
 // farther one pushed) or the next node off the stack.
+//   kOctant: only the near and far plane of each axis is dequantized, picked by
+//   the ray's octant (with a positive reciprocal the lower bound gives the smaller
+//   t, fma rounding being monotonic; with a negative one the upper), so the
+//   result equals min / max over both planes with 8 fewer VALU operations per
+//   node, at the cost of four more registers per lane for the selectors.
+//   Otherwise all twelve planes are dequantized and ordered by min / max (the
+//   megakernel, whose occupancy those four registers would lower).
+template <bool kOctant>
 __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
                                            float tcur, int* stack, int stride, int& sp) {
     const uint4 na = nodes[2 * node + 0];
@@ -159,22 +185,35 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
     const float sx = __uint_as_float((na.x & 0xFFu) << 23), sy = __uint_as_float((na.y & 0xFFu) << 23),
                 sz = __uint_as_float((na.z & 0xFFu) << 23);
     auto q = [](uint32_t w, int byte) { return (float)((w >> (8 * byte)) & 0xFFu); };
-    const float l0x = fmaf(q(nb.x, 0), sx, gx), l0y = fmaf(q(nb.x, 1), sy, gy), l0z = fmaf(q(nb.x, 2), sz, gz);
-    const float h0x = fmaf(q(nb.x, 3), sx, gx), h0y = fmaf(q(nb.y, 0), sy, gy), h0z = fmaf(q(nb.y, 1), sz, gz);
-    const float l1x = fmaf(q(nb.y, 2), sx, gx), l1y = fmaf(q(nb.y, 3), sy, gy), l1z = fmaf(q(nb.z, 0), sz, gz);
-    const float h1x = fmaf(q(nb.z, 1), sx, gx), h1y = fmaf(q(nb.z, 2), sy, gy), h1z = fmaf(q(nb.z, 3), sz, gz);
-    // child 0
-    float a0 = l0x * r.ix - r.ox, a1 = h0x * r.ix - r.ox;
-    float b0 = l0y * r.iy - r.oy, b1 = h0y * r.iy - r.oy;
-    float c0 = l0z * r.iz - r.oz, c1 = h0z * r.iz - r.oz;
-    float near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
-    float far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
-    // child 1
-    float d0 = l1x * r.ix - r.ox, d1 = h1x * r.ix - r.ox;
-    float e0 = l1y * r.iy - r.oy, e1 = h1y * r.iy - r.oy;
-    float f0 = l1z * r.iz - r.oz, f1 = h1z * r.iz - r.oz;
-    float near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
-    float far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
+    float near0, far0, near1, far1;
+    if constexpr (kOctant) {
+        const uint32_t n0 = __builtin_amdgcn_perm(nb.y, nb.x, r.sel_near0);
+        const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, r.sel_far0);
+        const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, r.sel_near1);
+        const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, r.sel_far1);
+        auto tx = [&](uint32_t w) { return fmaf(q(w, 0), sx, gx) * r.ix - r.ox; };
+        auto ty = [&](uint32_t w) { return fmaf(q(w, 1), sy, gy) * r.iy - r.oy; };
+        auto tz = [&](uint32_t w) { return fmaf(q(w, 2), sz, gz) * r.iz - r.oz; };
+        near0 = fmaxf(fmaxf(tx(n0), ty(n0)), fmaxf(tz(n0), r.tmin));
+        far0 = fminf(fminf(tx(f0), ty(f0)), fminf(tz(f0), tcur)) * 1.00000024f;
+        near1 = fmaxf(fmaxf(tx(n1), ty(n1)), fmaxf(tz(n1), r.tmin));
+        far1 = fminf(fminf(tx(f1), ty(f1)), fminf(tz(f1), tcur)) * 1.00000024f;
+    } else {
+        const float l0x = fmaf(q(nb.x, 0), sx, gx), l0y = fmaf(q(nb.x, 1), sy, gy), l0z = fmaf(q(nb.x, 2), sz, gz);
+        const float h0x = fmaf(q(nb.x, 3), sx, gx), h0y = fmaf(q(nb.y, 0), sy, gy), h0z = fmaf(q(nb.y, 1), sz, gz);
+        const float l1x = fmaf(q(nb.y, 2), sx, gx), l1y = fmaf(q(nb.y, 3), sy, gy), l1z = fmaf(q(nb.z, 0), sz, gz);
+        const float h1x = fmaf(q(nb.z, 1), sx, gx), h1y = fmaf(q(nb.z, 2), sy, gy), h1z = fmaf(q(nb.z, 3), sz, gz);
+        const float a0 = l0x * r.ix - r.ox, a1 = h0x * r.ix - r.ox;
+        const float b0 = l0y * r.iy - r.oy, b1 = h0y * r.iy - r.oy;
+        const float c0 = l0z * r.iz - r.oz, c1 = h0z * r.iz - r.oz;
+        near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
+        far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
+        const float d0 = l1x * r.ix - r.ox, d1 = h1x * r.ix - r.ox;
+        const float e0 = l1y * r.iy - r.oy, e1 = h1y * r.iy - r.oy;
+        const float f0 = l1z * r.iz - r.oz, f1 = h1z * r.iz - r.oz;
+        near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
+        far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
+    }
     bool h0 = near0 <= far0;
     bool h1 = near1 <= far1;
     int ch0 = (int)na.w;
@@ -258,7 +297,7 @@ __device__ __forceinline__ bool test_leaf(const DeviceScene& sc, int node, const
 // instead of interleaving with node visits.  The triangles tested are those of
 // the plain nearest-child-first walk; only the order of leaves can differ, which
 // changes nothing but the choice among hits at exactly equal t.
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, bool kOctant = true>
 __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax, Hit& hit, int* stack,
                          int stride, TraversalCounters& cnt) {
     const TraversalRay r = make_traversal_ray(o, d, tmin);
@@ -270,7 +309,7 @@ __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, floa
     while (node != kNodeDone || leaf != 0) {
         while (node >= 0) {
             if (kCount) cnt.nodes++;
-            node = visit_inner(sc.nodes, node, r, hit.t, stack, stride, sp);
+            node = visit_inner<kOctant>(sc.nodes, node, r, hit.t, stack, stride, sp);
             if (node < 0 && node != kNodeDone && leaf == 0) {
                 leaf = node;
                 node = pop_node(stack, stride, sp);
