@@ -25,7 +25,7 @@ __device__ __forceinline__ void trace_closest(const DeviceScene& sc, vec3 o, vec
                                               int* stack, SurfaceInfo& s, vec3& position, uint32_t& category,
                                               TraversalCounters& cnt) {
     Hit h;
-    if (traverse<false, kCount, false>(sc, o, d, tmin, tmax, h, stack, kTileThreads, cnt))
+    if (traverse<false, kCount, 2>(sc, o, d, tmin, tmax, h, stack, kTileThreads, cnt))
         shade_hit(sc, h, d, s, position, category);
     else
         miss_surface(s, position, category);
@@ -41,7 +41,7 @@ __device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode,
     if (!ls.valid) return make_vec3(0.0f);
     ++shadow_rays;
     Hit h;
-    bool occluded = traverse<true, kCount, false>(sc, pos, ls.l, 0.001f, ls.tmax, h, stack, kTileThreads, cnt);
+    bool occluded = traverse<true, kCount, 2>(sc, pos, ls.l, 0.001f, ls.tmax, h, stack, kTileThreads, cnt);
     if (occluded) return make_vec3(0.0f);
     return nee_unshadowed(ls, o, s, throughput);
 }

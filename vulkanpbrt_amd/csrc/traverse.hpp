@@ -168,14 +168,16 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
 // (bound = fma(q, 2^e, origin), exactly as the quantization pass checked them)
 // and slab-test them against [tmin, tcur]; returns the nearer hit child (the
 // farther one pushed) or the next node off the stack.
-//   kOctant: only the near and far plane of each axis is dequantized, picked by
-//   the ray's octant (with a positive reciprocal the lower bound gives the smaller
-//   t, fma rounding being monotonic; with a negative one the upper), so the
-//   result equals min / max over both planes with 8 fewer VALU operations per
-//   node, at the cost of four more registers per lane for the selectors.
-//   Otherwise all twelve planes are dequantized and ordered by min / max (the
-//   megakernel, whose occupancy those four registers would lower).
-template <bool kOctant>
+//   kPlanes 1 and 2: only the near and far plane of each axis is dequantized,
+//   picked by the ray's octant (with a positive reciprocal the lower bound gives
+//   the smaller t, fma rounding being monotonic; with a negative one the upper),
+//   so the result equals min / max over both planes with 8 fewer VALU operations
+//   per node.  1 keeps the four selectors in registers; 2 keeps child 0's near
+//   selector only and derives the other three per node (3 operations: the far
+//   byte of an axis is lo + hi - near, child 1's bytes sit two further on), for
+//   the megakernel, whose occupancy three more registers would lower.
+//   kPlanes 0: all twelve planes dequantized and ordered by min / max.
+template <int kPlanes>
 __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
                                            float tcur, int* stack, int stride, int& sp) {
     const uint4 na = nodes[2 * node + 0];
@@ -186,11 +188,19 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
                 sz = __uint_as_float((na.z & 0xFFu) << 23);
     auto q = [](uint32_t w, int byte) { return (float)((w >> (8 * byte)) & 0xFFu); };
     float near0, far0, near1, far1;
-    if constexpr (kOctant) {
-        const uint32_t n0 = __builtin_amdgcn_perm(nb.y, nb.x, r.sel_near0);
-        const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, r.sel_far0);
-        const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, r.sel_near1);
-        const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, r.sel_far1);
+    if constexpr (kPlanes != 0) {
+        uint32_t sn0 = r.sel_near0, sf0 = r.sel_far0, sn1 = r.sel_near1, sf1 = r.sel_far1;
+        if constexpr (kPlanes == 2) {
+            // an opaque copy, so the derived selectors are not hoisted out of the walk
+            asm volatile("" : "+v"(sn0));
+            sf0 = 0x18070503u - sn0;  // per byte: 3 - b, 5 - b, 7 - b; 0x0c stays 0x0c
+            sn1 = sn0 + 0x00020202u;
+            sf1 = sf0 + 0x00020202u;
+        }
+        const uint32_t n0 = __builtin_amdgcn_perm(nb.y, nb.x, sn0);
+        const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, sf0);
+        const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, sn1);
+        const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, sf1);
         auto tx = [&](uint32_t w) { return fmaf(q(w, 0), sx, gx) * r.ix - r.ox; };
         auto ty = [&](uint32_t w) { return fmaf(q(w, 1), sy, gy) * r.iy - r.oy; };
         auto tz = [&](uint32_t w) { return fmaf(q(w, 2), sz, gz) * r.iz - r.oz; };
@@ -297,7 +307,7 @@ __device__ __forceinline__ bool test_leaf(const DeviceScene& sc, int node, const
 // instead of interleaving with node visits.  The triangles tested are those of
 // the plain nearest-child-first walk; only the order of leaves can differ, which
 // changes nothing but the choice among hits at exactly equal t.
-template <bool kAnyHit, bool kCount, bool kOctant = true>
+template <bool kAnyHit, bool kCount, int kPlanes = 1>
 __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax, Hit& hit, int* stack,
                          int stride, TraversalCounters& cnt) {
     const TraversalRay r = make_traversal_ray(o, d, tmin);
@@ -309,7 +319,7 @@ __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, floa
     while (node != kNodeDone || leaf != 0) {
         while (node >= 0) {
             if (kCount) cnt.nodes++;
-            node = visit_inner<kOctant>(sc.nodes, node, r, hit.t, stack, stride, sp);
+            node = visit_inner<kPlanes>(sc.nodes, node, r, hit.t, stack, stride, sp);
             if (node < 0 && node != kNodeDone && leaf == 0) {
                 leaf = node;
                 node = pop_node(stack, stride, sp);
