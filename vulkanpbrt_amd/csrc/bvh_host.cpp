@@ -51,6 +51,9 @@ struct BuildNode {
 };
 
 constexpr int kBins = 32;
+// SAH cost of visiting an inner node (both child boxes tested), in units of one
+// triangle test.
+constexpr float kTraversalCost = 1.5f;
 
 // Emit the two-children layout in depth-first order (leaves folded into their
 // parent's child references).
@@ -240,8 +243,7 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
                 }
             }
             float parent_area = box.area();
-            // SAH with unit traversal and intersection costs
-            float split_cost = parent_area > 0.0f ? 1.0f + best / parent_area : std::numeric_limits<float>::infinity();
+            float split_cost = parent_area > 0.0f ? kTraversalCost + best / parent_area : std::numeric_limits<float>::infinity();
             float leaf_cost = (float)count;
             if (best_split < 0) {
                 if (count <= (uint32_t)max_leaf) make_leaf = true;
@@ -595,8 +597,7 @@ BvhHost build_bvh_sbvh(const float* tris, uint32_t n, int max_leaf, int max_dept
                 sp = best_spatial_split(tris, refs, box);
             const float best = std::min(obj.cost, sp.cost);
             const float parent_area = box.area();
-            // SAH with unit traversal and intersection costs
-            const float split_cost = parent_area > 0.0f ? 1.0f + best / parent_area : std::numeric_limits<float>::infinity();
+            const float split_cost = parent_area > 0.0f ? kTraversalCost + best / parent_area : std::numeric_limits<float>::infinity();
             if (obj.axis < 0 && sp.axis < 0) {
                 if (count <= (uint32_t)max_leaf) {
                     make_leaf();
