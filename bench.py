@@ -56,6 +56,13 @@ def parse():
                          "(default for the city; the per-kernel breakdown then comes from an eager pass over "
                          "the same frames after the timed region)")
     ap.add_argument("--eager", dest="graph", action="store_false", help="launch the frame's kernels one by one")
+    ap.add_argument("--pipeline", dest="pipeline", action="store_true", default=False,
+                    help="trace each frame on one stream while the previous frame's accumulation, denoising and "
+                         "TAA run on a second one (trace outputs double-buffered; eager launches only); the "
+                         "per-stage breakdown then comes from a one-stream pass over the same frames.  Off by "
+                         "default: on the atrium the post chain beside the megakernel costs more than it hides")
+    ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
+                    help="(default) run each frame's stages in order on one stream")
     ap.add_argument("--scene", default=None, metavar="OBJ",
                     help="trace a Wavefront OBJ/MTL file (pbrt_load_obj) instead of the generated scene, with the "
                          "workload's camera path, image size and explicit lights")
@@ -251,7 +258,7 @@ def main():
         st = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)] for k in range(K)])
         return start.elapsed_time(end), st.mean(axis=0), tm
 
-    graph_info = None
+    graph_info = pipe_info = None
     if use_graph:
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -267,6 +274,17 @@ def main():
         eager_ms, stage_ms, timing = eager_pass(args.warmup)
         graph_info = {"frames_per_graph": 1, "kernel_nodes": fr.graph_nodes,
                       "instantiations": inst_timed, "eager_ms_per_step": round(eager_ms / K, 3)}
+    elif args.pipeline:
+        post = torch.cuda.Stream()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        fr.run_pipelined(args.warmup, K, post)
+        end.record()
+        torch.cuda.synchronize()
+        total_ms = start.elapsed_time(end)
+        eager_ms, stage_ms, timing = eager_pass(args.warmup)
+        pipe_info = {"streams": 2, "one_stream_ms_per_step": round(eager_ms / K, 3)}
     else:
         total_ms, stage_ms, timing = eager_pass(args.warmup)
     ms_per_step = total_ms / K
@@ -329,7 +347,8 @@ def main():
         "config": {"workload": cfg["name"], "width": w, "height": h, "spp": 1, "max_depth": cfg["depth"],
                    "triangles": info.triangle_count, "lights": info.light_count, "tracer": args.tracer,
                    "denoiser": args.denoiser, "bmfr_fit": args.bmfr_fit, "seed": cfg["seed"],
-                   "launch": "hipgraph per frame" if use_graph else "eager"},
+                   "launch": "hipgraph per frame" if use_graph else
+                             ("eager, trace and post chain on two streams" if args.pipeline else "eager")},
         "stage_ms": {"trace": round(stage_ms[0], 3), "accumulate": round(stage_ms[1], 3),
                      "denoise": round(stage_ms[2], 3), "taa": round(stage_ms[3], 3)},
         "rays_per_frame": int(rays_per_frame),
@@ -349,6 +368,8 @@ def main():
         out["scene_file"] = load_info
     if graph_info:
         out["hipgraph"] = graph_info
+    if pipe_info:
+        out["pipeline"] = pipe_info
     if tuning:
         out["autotune"] = tuning
         out["config"]["tracer"] = tuning["best"]["tracer"]

@@ -74,3 +74,43 @@ def test_frame_graph_errors(cuda):
     torch.cuda.synchronize()
     assert y.sum().item() == 32
     capi.check(lib.pbrt_frame_graph_destroy(g))
+
+
+def history(fr):
+    out = {k: v.cpu().numpy().copy() for k, v in (("final", fr.final), ("bmfr_final", fr.bmfr_final),
+                                                   ("denoised", fr.denoised), ("taa_hist", fr.taa_hist),
+                                                   ("motion", fr.motion), ("illumination", fr.img["illumination"]),
+                                                   ("depth", fr.img["depth"]))}
+    for i in range(2):
+        out[f"acc{i}"], out[f"spp{i}"] = fr.acc[i].cpu().numpy().copy(), fr.spp[i].cpu().numpy().copy()
+    return out
+
+
+@pytest.mark.parametrize("tracer,denoiser", [("megakernel", "bmfr"), ("wavefront", "bmfr"), ("megakernel", "cnn2")])
+def test_pipelined_frames_equal_one_stream_frames(cuda, tracer, denoiser):
+    """Frame.run_pipelined (trace of frame f+1 beside the post chain of frame f on a
+    second stream, trace outputs double-buffered) leaves the same bytes as run() frame
+    after frame on one stream: the final and denoised images and every history."""
+    torch = cuda
+    gen = capi.GeneratedScene(1, 1)
+    scene = capi.Scene(gen.desc)
+    side, post = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        one = make_frame(torch, tracer, denoiser, gen, scene)
+        two = make_frame(torch, tracer, denoiser, gen, scene)
+        for f in range(6):
+            one.run(f)
+        two.run_pipelined(0, 6, post)
+        torch.cuda.synchronize()
+        want, got = history(one), history(two)
+        for k in want:
+            assert np.array_equal(want[k], got[k]), f"after 6 frames: {k} differs"
+        # a second call continues the sequence (the set written last is reused first)
+        for f in range(6, 9):
+            one.run(f)
+        two.run_pipelined(6, 3, post)
+        torch.cuda.synchronize()
+        want, got = history(one), history(two)
+        for k in want:
+            assert np.array_equal(want[k], got[k]), f"after 9 frames: {k} differs"
+        assert want["final"][..., :3].mean() > 5

@@ -48,6 +48,7 @@ class Frame:
         capi.check(capi.lib().pbrt_trace_workspace_size(w, h, C.byref(tn)))
         self.trace_ws = t.zeros(tn.value, dtype=t.uint8, device=d)
         self.prev_view = None
+        self.img_sets = None  # trace outputs, double-buffered by run_pipelined
         self.graph_nodes, self.graph_instantiations = 0, 0
         self.nets = {}
         if denoiser in ("cnn", "cnn2"):
@@ -199,6 +200,41 @@ class Frame:
         self.denoise(frame)
         self.taa(frame)
         self.prev_view = view
+
+    def run_pipelined(self, first, count, post_stream):
+        """Frames first .. first + count - 1, each frame's accumulation, denoising and TAA
+        on `post_stream` while the next frame is traced on the current stream, so the
+        post chain of one frame runs beside the trace of the next.  The trace outputs
+        are double-buffered (frame f writes set f & 1) and frame f's trace waits until
+        the post chain of frame f - 2 is done with that set; the post chain stays in
+        order on its stream, which orders the accumulation, BMFR and TAA histories.
+        The images equal those of run() frame by frame; on return the current stream
+        is ordered after the last frame's post chain."""
+        t = self.torch
+        trace_stream = t.cuda.current_stream()
+        if self.img_sets is None:
+            self.img_sets = [self.img, {k: t.zeros_like(v) for k, v in self.img.items()}]
+        post_stream.wait_stream(trace_stream)
+        posted = []
+        for k in range(count):
+            f = first + k
+            self.img = self.img_sets[f & 1]
+            if k >= 2:
+                trace_stream.wait_event(posted[k - 2])
+            pc, view, proj = self.push_constants(f)
+            self.trace(pc)
+            traced = t.cuda.Event()
+            traced.record(trace_stream)
+            with t.cuda.stream(post_stream):
+                post_stream.wait_event(traced)
+                self.accumulate(f, view, proj)
+                self.denoise(f)
+                self.taa(f)
+                done = t.cuda.Event()
+                done.record(post_stream)
+            posted.append(done)
+            self.prev_view = view
+        trace_stream.wait_stream(post_stream)
 
     def run(self, frame, events=None, graph=None):
         """One frame; with `graph` (a pbrt_frame_graph handle) its launches are captured
