@@ -426,6 +426,11 @@ PBRT_HD LightSample sample_light_light_strength(const DeviceLight* lights, uint3
 }
 
 // Uniform light selection (lighting.glsl:170-227).
+// The light UNIFORM sampling picks for the random number rnd in [0, 1).
+PBRT_HD int uniform_light_index(float rnd, uint32_t light_count) {
+    return (int)(rnd * (float)light_count) - (int)rnd;
+}
+
 PBRT_HD LightSample sample_light_uniform(const DeviceLight* lights, uint32_t light_count, vec3 pos, vec3 n,
                                          Rng& re) {
     LightSample out;
@@ -434,8 +439,7 @@ PBRT_HD LightSample sample_light_uniform(const DeviceLight* lights, uint32_t lig
     out.tmax = 1000.0f;
     out.valid = false;
     out.radiance = make_vec3(0.0f);
-    float rnd = rng_float(re);
-    int i = (int)(rnd * (float)light_count) - (int)rnd;
+    const int i = uniform_light_index(rng_float(re), light_count);
     const DeviceLight li = lights[i];
     vec3 light_strength = xyz(li.col_ambient) + xyz(li.col_diffuse) + xyz(li.col_specular);
     const float tmin = 0.001f;
