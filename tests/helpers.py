@@ -43,8 +43,9 @@ def trace_workspace(torch, width, height):
 
 def trace(torch, scene, pc, img, width, height, max_depth=2, tracer=capi.TRACER_MEGAKERNEL,
           output=capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, light_sampling=capi.LIGHT_SAMPLE_AUTO, counters=None,
-          block_threads=0, queue_paths=0):
-    p = capi.TraceParams(width, height, max_depth, 0, light_sampling, output, 1, tracer, block_threads, queue_paths)
+          block_threads=0, queue_paths=0, min_depth=0):
+    p = capi.TraceParams(width, height, max_depth, min_depth, light_sampling, output, 1, tracer, block_threads,
+                         queue_paths)
     ws = trace_workspace(torch, width, height) if tracer == capi.TRACER_WAVEFRONT else None
     ti = capi.TraceImages(*(capi.ptr(img[k]) for k in ("depth", "normal", "material", "albedo", "illumination")),
                           capi.ptr(ws), ws.numel() if ws is not None else 0)

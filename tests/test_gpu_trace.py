@@ -243,6 +243,26 @@ def test_atrium_matches_fp32_mirror(cuda, pbrt_lib, oracle_lib):
         assert abs(g - r) <= 5e-3 * r
 
 
+@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+def test_russian_roulette_start_matches_fp32_mirror(cuda, cfg1, tracer):
+    """minRecursionDepth (lighting.glsl:279-284): no Russian roulette up to bounce 2 of
+    5, then termination with probability 0.1 and throughput / 0.9."""
+    gen, scene, osc = cfg1
+    w = h = 96
+    pc, _, _ = helpers.frame_push_constants(gen, w, h, 3)
+    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    img = helpers.gbuffer(cuda, w, h)
+    cnt = cuda.zeros(6, dtype=cuda.int64, device="cuda")
+    helpers.trace(cuda, scene, pc, img, w, h, max_depth=5, min_depth=2, tracer=kind, counters=cnt)
+    cuda.cuda.synchronize()
+    ref = osc.trace(w, h, np.array(pc.view_inverse[:], np.float32), np.array(pc.proj_inverse[:], np.float32),
+                    frame=3, max_depth=5, min_depth=2, precision=32)
+    frac = _compare_frame(helpers.to_np(img), ref, f"{tracer} min depth 2 of 5")
+    assert frac["depth"] >= 0.999 and frac["albedo"] >= 0.999 and frac["radiance"] >= 0.99
+    rays = int(cnt.cpu().numpy()[0])
+    assert abs(rays - ref["rays"]) <= 0.01 * ref["rays"]
+
+
 def test_wavefront_counts_rays(cuda, cfg1):
     gen, scene, osc = cfg1
     pc, _, _ = helpers.frame_push_constants(gen, W, H, 2)
