@@ -46,12 +46,8 @@ __device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode,
     return nee_unshadowed(ls, o, s, throughput);
 }
 
-// Five waves per SIMD (the 32-entry LDS stacks allow five): the compiler then keeps 96
-// VGPRs and spills a few values of the cold paths to scratch, which measures faster than
-// four waves without spills.
 template <bool kCount>
-__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(5)))
-pt_megakernel(DeviceScene sc, TraceConsts c, float* depth_img,
+__global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, TraceConsts c, float* depth_img,
                                                               float* normal_img, uint8_t* material_img,
                                                               uint8_t* albedo_img, float4* illum_img,
                                                               unsigned long long* counters) {
