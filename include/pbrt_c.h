@@ -224,7 +224,11 @@ typedef struct pbrt_trace_images {
  * quarter queues), `repeats` timed runs each after a warm-up, on `stream`, and returns
  * the fastest as `best` (a copy of `params` with tracer, block_threads and
  * queue_paths set).  No counterpart in the reference (the Vulkan driver schedules
- * its ray tracing pipeline).  Synchronizes the stream. */
+ * its ray tracing pipeline).  Synchronizes the stream.  Every candidate writes the
+ * output images as pbrt_trace does: the G-buffer and illumination hold the last
+ * candidate's frame afterwards, and with PBRT_OUTPUT_FINAL_IMAGE and
+ * sample_number > 0 each run blends into the progressive image again, so pass
+ * scratch images (or re-trace the frame) when their contents matter. */
 #define PBRT_TUNING_MAX_CANDIDATES 16
 typedef struct pbrt_trace_tuning {
     pbrt_trace_params best;
@@ -570,6 +574,10 @@ int32_t pbrt_frame_graph_begin(pbrt_frame_graph* graph, void* stream);
 /* node_count (may be NULL): kernel nodes of the captured frame; instantiated (may be
  * NULL): 1 when the executable graph was rebuilt rather than updated. */
 int32_t pbrt_frame_graph_end(pbrt_frame_graph* graph, void* stream, uint32_t* node_count, uint32_t* instantiated);
+/* Ends an open capture without launching it (the frame's work is dropped) and keeps
+ * the handle and its executable graph for the next begin(); a no-op when no capture
+ * is open.  For callers whose frame failed between begin() and end(). */
+int32_t pbrt_frame_graph_abort(pbrt_frame_graph* graph);
 
 #ifdef __cplusplus
 } /* extern "C" */

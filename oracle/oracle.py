@@ -248,7 +248,18 @@ def bmfr_geometry(width, height, bs):
     return wp, hp, wp // bs, hp // bs
 
 
+def _check_mirror(width, height, bs):
+    """mirror() reflects once (bmfrGeneral.comp:93-97): the padded grid shifted by the
+    most negative pixelOffsets must stay within one image size, or the passes would
+    read outside the images (the product rejects such sizes)."""
+    wp, hp, _, _ = bmfr_geometry(width, height, bs)
+    if wp - int(np.float32(bs) * np.float32(-0.06)) > 2 * width or \
+            hp - int(np.float32(bs) * np.float32(-0.08)) > 2 * height:
+        raise ValueError(f"{width}x{height} with {bs}-pixel blocks mirrors outside the image")
+
+
 def bmfr_pre(width, height, bs, frame, depth, normal, noisy):
+    _check_mirror(width, height, bs)
     wp, hp, _, _ = bmfr_geometry(width, height, bs)
     feats = np.zeros((13, hp, wp), np.uint16)
     d, n, z = _c(depth, np.float32), _c(normal, np.float32), _c(noisy, np.uint16)
@@ -281,6 +292,7 @@ def bmfr_block_matrix(width, height, bs, frame, features, bx, by):
 
 def bmfr_post(width, height, bs, frame, depth, normal, albedo, motion, spp, noisy, weights, denoised):
     """bmfrPost; denoised is the (2, H, W, 4) RGBA16F array (updated copy returned)."""
+    _check_mirror(width, height, bs)
     den = _c(denoised, np.uint16).copy()
     final = np.zeros((height, width, 4), np.uint8)
     args = [_c(depth, np.float32), _c(normal, np.float32), _c(albedo, np.uint8), _c(motion, np.uint16),

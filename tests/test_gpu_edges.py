@@ -203,6 +203,13 @@ def test_empty_inputs(cuda, cfg1):
     p = capi.BmfrParams(63, 200, 32, 0, capi.BMFR_FIT_GRAM_F64, 0)
     assert lib.pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) != 0
     assert b"two blocks" in lib.pbrt_last_error()
+    # a side of exactly two 32-pixel blocks: the frame-8 offset (-0.08 * 32 -> -2 rows)
+    # would mirror rows 128 and 129 of the padded grid to -1 and -2
+    p = capi.BmfrParams(96, 64, 32, 0, capi.BMFR_FIT_GRAM_F64, 0)
+    assert lib.pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) != 0
+    assert b"two blocks and one pixel" in lib.pbrt_last_error()
+    p = capi.BmfrParams(96, 65, 32, 8, capi.BMFR_FIT_GRAM_F64, 0)
+    assert lib.pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) == 0
     cuda.cuda.synchronize()
     # the library still works afterwards
     rays = np.zeros((1, 8), np.float32)

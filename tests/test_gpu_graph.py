@@ -76,6 +76,42 @@ def test_frame_graph_errors(cuda):
     capi.check(lib.pbrt_frame_graph_destroy(g))
 
 
+def test_frame_failing_inside_capture_keeps_the_graph(cuda):
+    """A frame whose launches fail on the host while its capture is open (here TAA
+    rejects a zero-size image) raises, closes the capture with pbrt_frame_graph_abort
+    and leaves the handle with the caller: the next frame captures and launches
+    through the same graph, and the caller's single destroy succeeds."""
+    torch = cuda
+    lib = capi.lib()
+    gen = capi.GeneratedScene(1, 1)
+    scene = capi.Scene(gen.desc)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        eager = make_frame(torch, "megakernel", "bmfr", gen, scene)
+        graphed = make_frame(torch, "megakernel", "bmfr", gen, scene)
+        g = C.c_void_p()
+        capi.check(lib.pbrt_frame_graph_create(C.byref(g)))
+        try:
+            good_taa = graphed.taa
+
+            def bad_taa(frame):
+                images = capi.TaaImages(None, None, None, None)
+                capi.check(lib.pbrt_taa(0, 0, frame, C.byref(images), graphed.stream()))
+
+            graphed.taa = bad_taa
+            with pytest.raises(capi.PbrtError):
+                graphed.run(0, graph=g)
+            graphed.taa = good_taa
+            assert lib.pbrt_frame_graph_abort(g) == 0  # nothing open any more: a no-op
+            for f in range(2):
+                eager.run(f)
+                graphed.run(f, graph=g)
+            torch.cuda.synchronize()
+            assert np.array_equal(snapshot(eager)["final"], snapshot(graphed)["final"])
+        finally:
+            assert lib.pbrt_frame_graph_destroy(g) == 0
+
+
 def history(fr):
     out = {k: v.cpu().numpy().copy() for k, v in (("final", fr.final), ("bmfr_final", fr.bmfr_final),
                                                    ("denoised", fr.denoised), ("taa_hist", fr.taa_hist),

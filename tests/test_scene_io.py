@@ -167,6 +167,25 @@ def test_image_errors(tmp_path, pbrt_lib):
         capi.load_image(tmp_path / "missing.png")
 
 
+@pytest.mark.parametrize("header", [
+    # w * h * depth * bytes and w * h * 4 both wrap to 0 in 64 bits
+    b"P7\nWIDTH 2147483648\nHEIGHT 2147483648\nDEPTH 4\nMAXVAL 255\nENDHDR\n",
+    # a width that does not fit 32 bits
+    b"P7\nWIDTH 99999999999\nHEIGHT 1\nDEPTH 4\nMAXVAL 255\nENDHDR\n",
+    b"P6\n99999999999 1\n255\n",
+    # over the 2^28-pixel cap without wrapping
+    b"P5\n65536 65536\n255\n",
+])
+def test_pnm_oversized_header_is_rejected(tmp_path, pbrt_lib, header):
+    """A PNM header whose dimensions overflow (or exceed the 2^28-pixel cap PNG also
+    has) is an invalid argument, never a short allocation written past its end."""
+    f = tmp_path / "big.pam"
+    f.write_bytes(header + b"\0" * 64)
+    with pytest.raises(capi.PbrtError) as e:
+        capi.load_image(f)
+    assert e.value.status == -1
+
+
 # ---------------------------------------------------------------------------
 # materials
 # ---------------------------------------------------------------------------

@@ -248,7 +248,7 @@ int main(int argc, char** argv) {
             if (taa) {
                 taa_module = Taa::create(width, height, 16, 16, g_buffer, accumulator->accumulation_buffer, final_image);
                 taa_module->compile(ctx);
-                taa_module->add_dispatch_to_command_graph(commands, push_constants);
+                taa_module->add_dispatch_to_command_graph(commands);
                 final_image = taa_module->get_final_descriptor_image();
             }
             accumulator->accumulation_buffer->copy_to_back_images(commands, g_buffer,

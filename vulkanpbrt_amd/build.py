@@ -45,9 +45,20 @@ def _sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")) + list(MODULES.glob("*.cpp")))
 
 
+# The tracer's shading arithmetic is evaluated as written, one rounding per
+# operation like the CPU oracle's FP32 mirror (built with -ffp-contract=off):
+# hipcc would otherwise fuse a*b+c into v_fmac_f32, moving per-pixel radiance
+# by an ulp at each fused step.  The traversal's slab tests spell their fused
+# multiply-adds out with fmaf, so they keep them.
+PER_FILE = {
+    "trace.hip": ["-ffp-contract=off"],
+    "wavefront.hip": ["-ffp-contract=off"],
+}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.parent.name + "_" + src.name + ".o")
-    cmd = [HIPCC, "-x", "hip", *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, "-x", "hip", *COMMON, *PER_FILE.get(src.name, []), "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

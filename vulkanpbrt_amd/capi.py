@@ -290,6 +290,7 @@ SIGNATURES = {
     "pbrt_frame_graph_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
     "pbrt_frame_graph_destroy": (C.c_int32, [C.c_void_p]),
     "pbrt_frame_graph_begin": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "pbrt_frame_graph_abort": (C.c_int32, [C.c_void_p]),
     "pbrt_frame_graph_end": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pbrt_cnn_create": (C.c_int32, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "pbrt_cnn_destroy": (C.c_int32, [C.c_void_p]),

@@ -25,6 +25,7 @@
 //   there, the FP64-accumulated one does not.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -579,6 +580,18 @@ static BmfrGeom make_geom(const pbrt_bmfr_params& p) {
     g.bs = (int)p.block_size;
     g.wp = (g.width / g.bs + 2) * g.bs;
     g.hp = (g.height / g.bs + 2) * g.bs;
+    // mirror() reflects once: the padded grid shifted by any frame's pixelOffsets
+    // (the most negative shift is trunc(-0.06 bs) in x, trunc(-0.08 bs) in y) must
+    // land within one image size of [0, size).  A side of exactly two 16- or
+    // 32-pixel blocks would reach 2 * size and read outside the image.
+    int min_offx = 0, min_offy = 0;
+    for (const auto& o : kPixelOffsets) {
+        min_offx = std::min(min_offx, (int)((float)g.bs * o[0]));
+        min_offy = std::min(min_offy, (int)((float)g.bs * o[1]));
+    }
+    PBRT_REQUIRE(g.wp - min_offx <= 2 * g.width && g.hp - min_offy <= 2 * g.height,
+                 "pbrt_bmfr: a side of exactly two blocks is too small for the shifted, mirrored block grid "
+                 "(use at least two blocks and one pixel)");
     g.blocks_x = g.wp / g.bs;
     g.blocks_y = g.hp / g.bs;
     const float* o = kPixelOffsets[p.frame_number % 16];

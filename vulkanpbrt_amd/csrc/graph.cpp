@@ -44,6 +44,13 @@ int32_t pbrt_frame_graph_destroy(pbrt_frame_graph* graph) {
     });
 }
 
+int32_t pbrt_frame_graph_abort(pbrt_frame_graph* graph) {
+    return pbrt::guard([&] {
+        PBRT_REQUIRE(graph, "pbrt_frame_graph_abort: graph is NULL");
+        if (graph->active) abandon_capture(graph);
+    });
+}
+
 int32_t pbrt_frame_graph_begin(pbrt_frame_graph* graph, void* stream) {
     return pbrt::guard([&] {
         PBRT_REQUIRE(graph, "pbrt_frame_graph_begin: graph is NULL");

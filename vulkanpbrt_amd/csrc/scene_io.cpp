@@ -323,7 +323,9 @@ Image decode_pnm(const std::vector<uint8_t>& f, const std::string& name) {
         uint32_t v = 0;
         bool any = false;
         while (pos < f.size() && std::isdigit(f[pos])) {
-            v = v * 10 + (f[pos++] - '0');
+            const uint32_t digit = (uint32_t)(f[pos++] - '0');
+            if (v > (UINT32_MAX - digit) / 10) fail(PBRT_ERR_INVALID_ARG, name + ": PNM header number out of range");
+            v = v * 10 + digit;
             any = true;
         }
         if (!any) fail(PBRT_ERR_INVALID_ARG, name + ": bad PNM header");
@@ -360,6 +362,9 @@ Image decode_pnm(const std::vector<uint8_t>& f, const std::string& name) {
     }
     if (w == 0 || h == 0 || depth < 1 || depth > 4 || maxval < 1 || maxval > 65535)
         fail(PBRT_ERR_INVALID_ARG, name + ": bad PNM header");
+    // the same cap as PNG: w * h * 4 and the raster size then fit size_t with room to spare
+    if ((uint64_t)w * h > (1ull << 28)) fail(PBRT_ERR_INVALID_ARG, name + ": PNM too large");
+    if (pos > f.size()) fail(PBRT_ERR_INVALID_ARG, name + ": truncated PNM header");
     const size_t bps = maxval > 255 ? 2 : 1;
     if (pos + (size_t)w * h * depth * bps > f.size()) fail(PBRT_ERR_INVALID_ARG, name + ": truncated PNM data");
     Image img;

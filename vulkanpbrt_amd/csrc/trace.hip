@@ -309,9 +309,17 @@ extern "C" int32_t pbrt_autotune_trace(const pbrt_scene* scene, const pbrt_trace
                     cands.push_back({PBRT_TRACER_WAVEFRONT, block, div == 1 ? 0u : (pixels + div - 1) / div});
         }
         PBRT_REQUIRE(cands.size() <= PBRT_TUNING_MAX_CANDIDATES, "pbrt_autotune_trace: too many candidates");
-        hipEvent_t e0, e1;
-        PBRT_HIP(hipEventCreate(&e0));
-        PBRT_HIP(hipEventCreate(&e1));
+        // the events are released on every exit, also when a candidate is rejected
+        struct Events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events() {
+                for (hipEvent_t ev : e)
+                    if (ev) (void)hipEventDestroy(ev);
+            }
+        } ev;
+        PBRT_HIP(hipEventCreate(&ev.e[0]));
+        PBRT_HIP(hipEventCreate(&ev.e[1]));
+        hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
         float best = 3.4e38f;
         for (size_t i = 0; i < cands.size(); ++i) {
             pbrt_trace_params q = *p;
@@ -340,7 +348,5 @@ extern "C" int32_t pbrt_autotune_trace(const pbrt_scene* scene, const pbrt_trace
             }
         }
         out->candidates = (uint32_t)cands.size();
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     });
 }

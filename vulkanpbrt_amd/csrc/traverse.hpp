@@ -201,9 +201,9 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
         const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, sf0);
         const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, sn1);
         const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, sf1);
-        auto tx = [&](uint32_t w) { return fmaf(q(w, 0), sx, gx) * r.ix - r.ox; };
-        auto ty = [&](uint32_t w) { return fmaf(q(w, 1), sy, gy) * r.iy - r.oy; };
-        auto tz = [&](uint32_t w) { return fmaf(q(w, 2), sz, gz) * r.iz - r.oz; };
+        auto tx = [&](uint32_t w) { return fmaf(fmaf(q(w, 0), sx, gx), r.ix, -r.ox); };
+        auto ty = [&](uint32_t w) { return fmaf(fmaf(q(w, 1), sy, gy), r.iy, -r.oy); };
+        auto tz = [&](uint32_t w) { return fmaf(fmaf(q(w, 2), sz, gz), r.iz, -r.oz); };
         near0 = fmaxf(fmaxf(tx(n0), ty(n0)), fmaxf(tz(n0), r.tmin));
         far0 = fminf(fminf(tx(f0), ty(f0)), fminf(tz(f0), tcur)) * 1.00000024f;
         near1 = fmaxf(fmaxf(tx(n1), ty(n1)), fmaxf(tz(n1), r.tmin));
@@ -213,14 +213,14 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
         const float h0x = fmaf(q(nb.x, 3), sx, gx), h0y = fmaf(q(nb.y, 0), sy, gy), h0z = fmaf(q(nb.y, 1), sz, gz);
         const float l1x = fmaf(q(nb.y, 2), sx, gx), l1y = fmaf(q(nb.y, 3), sy, gy), l1z = fmaf(q(nb.z, 0), sz, gz);
         const float h1x = fmaf(q(nb.z, 1), sx, gx), h1y = fmaf(q(nb.z, 2), sy, gy), h1z = fmaf(q(nb.z, 3), sz, gz);
-        const float a0 = l0x * r.ix - r.ox, a1 = h0x * r.ix - r.ox;
-        const float b0 = l0y * r.iy - r.oy, b1 = h0y * r.iy - r.oy;
-        const float c0 = l0z * r.iz - r.oz, c1 = h0z * r.iz - r.oz;
+        const float a0 = fmaf(l0x, r.ix, -r.ox), a1 = fmaf(h0x, r.ix, -r.ox);
+        const float b0 = fmaf(l0y, r.iy, -r.oy), b1 = fmaf(h0y, r.iy, -r.oy);
+        const float c0 = fmaf(l0z, r.iz, -r.oz), c1 = fmaf(h0z, r.iz, -r.oz);
         near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
         far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
-        const float d0 = l1x * r.ix - r.ox, d1 = h1x * r.ix - r.ox;
-        const float e0 = l1y * r.iy - r.oy, e1 = h1y * r.iy - r.oy;
-        const float f0 = l1z * r.iz - r.oz, f1 = h1z * r.iz - r.oz;
+        const float d0 = fmaf(l1x, r.ix, -r.ox), d1 = fmaf(h1x, r.ix, -r.ox);
+        const float e0 = fmaf(l1y, r.iy, -r.oy), e1 = fmaf(h1y, r.iy, -r.oy);
+        const float f0 = fmaf(l1z, r.iz, -r.oz), f1 = fmaf(h1z, r.iz, -r.oz);
         near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
         far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
     }

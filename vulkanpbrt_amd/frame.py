@@ -245,7 +245,8 @@ class Frame:
             try:
                 self.run(frame)
             except BaseException:
-                lib.pbrt_frame_graph_destroy(graph)  # closes the open capture
+                # close the open capture; the handle stays the caller's to destroy
+                lib.pbrt_frame_graph_abort(graph)
                 raise
             nodes, inst = C.c_uint32(), C.c_uint32()
             self.capi.check(lib.pbrt_frame_graph_end(graph, self.stream(), C.byref(nodes), C.byref(inst)))

@@ -105,6 +105,12 @@ public:
     void addChild(Op op) { _ops.push_back(std::move(op)); }
     // keeps a module alive as long as the recorded ops that reference it
     void keep(std::shared_ptr<void> owner) { _owned.push_back(std::move(owner)); }
+    // vkCmdPushConstants: records `block` as the push-constant block in force for
+    // the ops recorded after it, as a command buffer keeps the last pushed values
+    // for later dispatches (taa.comp reads frameNumber from the block the denoiser
+    // pushed, Taa.cpp:99-107).  The value is still read when the ops run.
+    void push_constants(std::shared_ptr<void> block) { _pushed = std::move(block); }
+    const std::shared_ptr<void>& pushed_constants() const { return _pushed; }
     size_t size() const { return _ops.size(); }
     void submit(hipStream_t stream) const;
     void launch_graph(hipStream_t stream);
@@ -113,6 +119,7 @@ public:
 private:
     std::vector<Op> _ops;
     std::vector<std::shared_ptr<void>> _owned;
+    std::shared_ptr<void> _pushed;
     hipGraphExec_t _exec = nullptr;
 };
 using CommandsPtr = std::shared_ptr<Commands>;

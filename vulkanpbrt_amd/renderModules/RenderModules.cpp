@@ -107,6 +107,7 @@ void PBRTPipeline::add_trace_rays_to_command_graph(const CommandsPtr& command_gr
                                                    const PushConstantsPtr& push_constants) {
     if (!_tlas) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: set_tlas() was not called");
     if (!_workspace) throw Exception(PBRT_ERR_INVALID_ARG, "PBRTPipeline: compile() was not called");
+    command_graph->push_constants(push_constants);
     command_graph->addChild([this, push_constants](hipStream_t s) {
         pbrt_trace_params p{};
         p.width = _width;
@@ -236,6 +237,7 @@ void BMFR::compile(Context&) {
 
 void BMFR::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
     if (!_workspace) throw Exception(PBRT_ERR_INVALID_ARG, "BMFR: compile() was not called");
+    command_graph->push_constants(push_constants);
     command_graph->addChild([this, push_constants](hipStream_t s) {
         pbrt_bmfr_params p{};
         p.width = _width;
@@ -280,6 +282,7 @@ void BFR::compile(Context&) {
 }
 
 void BFR::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
+    command_graph->push_constants(push_constants);
     command_graph->addChild([this, push_constants](hipStream_t s) {
         pbrt_bfr_params p{};
         p.width = _width;
@@ -416,7 +419,14 @@ void Taa::compile(Context&) {
     _accumulation_image->compile();
 }
 
-void Taa::add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants) {
+void Taa::add_dispatch_to_command_graph(const CommandsPtr& command_graph) {
+    // every block the modules push is a PushConstants<RayTracingPushConstants>
+    PushConstantsPtr push_constants =
+        std::static_pointer_cast<PushConstants<RayTracingPushConstants>>(command_graph->pushed_constants());
+    if (!push_constants)
+        throw Exception(PBRT_ERR_INVALID_ARG,
+                        "Taa: no push constants were recorded before it (taa.comp reads frameNumber from the "
+                        "block the denoiser pushes)");
     command_graph->addChild([this, push_constants](hipStream_t s) {
         pbrt_taa_images im{};
         im.motion = _acc->motion->as<uint16_t>();
@@ -503,6 +513,9 @@ void add_denoiser_to_commands(DenoisingType denoising_type, DenoisingBlockSize d
             auto d = LearnedDenoiser::create(width, height, g_buffer, illumination_buffer, accumulation_buffer);
             d->compile(compile);
             d->update_image_layouts(compile);
+            // the denoiser slot's compute constants stay in force for the TAA after it,
+            // as the BMFR and BFR branches push them
+            commands->push_constants(compute_constants);
             d->add_dispatch_to_command_graph(commands);
             commands->keep(d);
             final_descriptor_image = d->get_final_descriptor_image();

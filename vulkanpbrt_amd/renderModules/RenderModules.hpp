@@ -209,9 +209,10 @@ public:
     }
     void compile(Context& context);
     void update_image_layouts(Context&) {}
-    // taa.comp reads frameNumber from the push constants last pushed by the
-    // denoiser (Taa.cpp:99-107); here that block is passed explicitly.
-    void add_dispatch_to_command_graph(const CommandsPtr& command_graph, const PushConstantsPtr& push_constants);
+    // taa.comp reads frameNumber from the push constants last pushed into the
+    // command list, by the denoiser (Taa.cpp:99-107): the block the modules
+    // recorded with Commands::push_constants before this call.
+    void add_dispatch_to_command_graph(const CommandsPtr& command_graph);
     DeviceImagePtr get_final_descriptor_image() const { return _final_image; }
 
 private:
