@@ -74,17 +74,24 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` (e.g. "atrium-1080p-sah") from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (lines
-    `traffic_bytes_per_launch: <kernel> <workload> <bytes>` under profiles/)."""
+def profile_value(key, kernel, workload):
+    """A figure for `kernel` on `workload` (e.g. "atrium-1080p-sbvh") recorded in a committed
+    rocprofv3 summary under profiles/ as a line `<key>: <kernel> <workload> <value>`;
+    returns (value, file name) or (None, None)."""
     import re
     for path in sorted((ROOT / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
-            m = re.match(r"traffic_bytes_per_launch:\s+(\S+)\s+(\S+)\s+(\d+)", line)
+            m = re.match(rf"{key}:\s+(\S+)\s+(\S+)\s+([0-9.eE+-]+)\s*$", line)
             if m and m.group(1) == kernel and m.group(2) == workload:
-                return int(m.group(3)), path.name
+                return float(m.group(3)), path.name
     return None, None
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (`traffic_bytes_per_launch:` lines)."""
+    v, src = profile_value("traffic_bytes_per_launch", kernel, workload)
+    return (int(v), src) if v is not None else (None, None)
 
 
 def copy_calibration(torch, mib):
@@ -322,13 +329,28 @@ def main():
         dom, dom_bytes, dom_ms = "pt_megakernel", ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
         kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    traffic, traffic_src = measured_traffic(dom, f"{cfg['tag']}-{bvh}")
+    workload = f"{cfg['tag']}-{bvh}"
+    traffic, traffic_src = measured_traffic(dom, workload)
+    valu_busy, valu_src = profile_value("valu_busy_frac", dom, workload)
+    lanes, _ = profile_value("valu_lane_util", dom, workload)
     n_launch = timing.extend_launches if fr.tracer == capi.TRACER_WAVEFRONT else timing.megakernel_launches
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
+    avg_us = dom_ms / max(1, n_launch) * 1e3
+    # `achieved` / `frac`: algorithmic bytes (ray I/O, node and triangle fetches, output) over
+    # the HBM peak.  Most node and triangle fetches hit L1/L2/MALL, so this is gather
+    # throughput through the cache hierarchy; what reaches HBM is `traffic` (rocprofv3
+    # FETCH_SIZE + WRITE_SIZE per launch, from the named profile), and what binds the walk
+    # is VALU issue under divergence (`valu_busy_frac`: SIMD cycles issuing VALU over the
+    # kernel's cycles; `valu_lane_util`: live lanes per VALU instruction).
+    roofline = dict(bound="valu_issue", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), frac_of="algorithmic bytes / HBM peak",
+                    traffic=traffic, kernel=dom,
                     algorithmic_bytes_per_launch=round(dom_bytes / max(1, n_launch)),
-                    avg_launch_us=round(dom_ms / max(1, n_launch) * 1e3, 1),
-                    traffic_source=f"profiles/{traffic_src}" if traffic_src else None)
+                    avg_launch_us=round(avg_us, 1),
+                    hbm_gbs=round(traffic / avg_us / 1e3, 1) if traffic else None,
+                    hbm_frac=round(traffic / avg_us / 1e3 / HBM_PEAK_GBS, 4) if traffic else None,
+                    valu_busy_frac=valu_busy, valu_lane_util=lanes,
+                    traffic_source=f"profiles/{traffic_src}" if traffic_src else None,
+                    valu_source=f"profiles/{valu_src}" if valu_src else None)
 
     out = {
         "metric": "1080p trace+denoise ms & Mrays/s; learned vs BMFR denoise ms at equal relMSE",
