@@ -411,6 +411,20 @@ typedef struct pbrt_bmfr_images {
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
+/* pbrt_bmfr without a workspace and without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * results as pbrt_bmfr; images->workspace and workspace_bytes are ignored.  The
+ * scratch (the fit's weights, plus the fp16 feature array for the Householder fit)
+ * lives in a device array of PBRT_BMFR_NOALLOC_SCRATCH_BYTES that the library holds
+ * on each GPU it runs on until the process ends; a frame needing more is rejected
+ * with PBRT_ERR_INVALID_ARG (Gram fits: 30 floats per block, i.e. up to 3840x2160 at
+ * 16-pixel blocks; Householder: about 26 bytes per padded pixel, i.e. up to ~640x480).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two streams,
+ * two threads) race.  Nothing in the scratch needs clearing between calls. */
+#define PBRT_BMFR_NOALLOC_SCRATCH_BYTES (8u << 20)
+int32_t pbrt_bmfr_noalloc(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
+
 /* Autotuner of the Gram fits' tile shape: runs pbrt_bmfr once per workgroup size
  * (256, 512, 1024 threads), `repeats` timed runs each after a warm-up, on `stream`,
  * and returns the fastest as `best` (a copy of `params` with fit_threads set).  The

@@ -66,3 +66,12 @@ def test_generated_scene_config1_counts(pbrt_lib):
 def test_generated_scene_unknown_config(pbrt_lib):
     h = C.c_void_p()
     assert pbrt_lib.pbrt_generate_scene(99, 1, C.byref(h)) == -2
+
+
+def test_bmfr_noalloc_rejects_oversized_frames_without_gpu(pbrt_lib):
+    """The scratch-size check of pbrt_bmfr_noalloc runs before any device call."""
+    p = capi.BmfrParams(7680, 4320, 32, 0, capi.BMFR_FIT_HOUSEHOLDER)
+    fake = [C.c_void_p(256)] * 8  # never dereferenced: the frame is rejected first
+    im = capi.BmfrImages(*fake, None, 0)
+    assert pbrt_lib.pbrt_bmfr_noalloc(C.byref(p), C.byref(im), None) == capi.PBRT_ERR_INVALID_ARG
+    assert b"scratch" in pbrt_lib.pbrt_last_error()

@@ -15,6 +15,11 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libpbrt.so"
 
 PBRT_OK = 0
+PBRT_ERR_INVALID_ARG = -1
+PBRT_ERR_UNSUPPORTED = -2
+PBRT_ERR_OUT_OF_MEMORY = -3
+PBRT_ERR_HIP = -4
+PBRT_ERR_INTERNAL = -5
 
 # enums (pbrt_c.h)
 PBRT_BVH_SAH_HOST = 0
@@ -212,6 +217,7 @@ class TaaImages(C.Structure):
 BMFR_FIT_HOUSEHOLDER = 0
 BMFR_FIT_GRAM_F16 = 1
 BMFR_FIT_GRAM_F64 = 2
+BMFR_NOALLOC_SCRATCH_BYTES = 8 << 20  # PBRT_BMFR_NOALLOC_SCRATCH_BYTES
 
 
 class BmfrParams(C.Structure):
@@ -279,6 +285,7 @@ SIGNATURES = {
     "pbrt_convert": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbrt_bmfr_workspace_size": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(C.c_size_t)]),
     "pbrt_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
+    "pbrt_bmfr_noalloc": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_void_p]),
     "pbrt_autotune_bmfr": (C.c_int32, [C.POINTER(BmfrParams), C.POINTER(BmfrImages), C.c_uint32,
                                         C.POINTER(BmfrTuning), C.c_void_p]),
     "pbrt_bmfr_batched_workspace_size": (C.c_int32, [C.POINTER(BmfrBatchFrame), C.c_uint32, C.POINTER(C.c_size_t)]),

@@ -635,6 +635,61 @@ static void launch_gram_fit(bool f64, int bs, uint32_t threads, dim3 grid, hipSt
 static size_t feature_bytes(const BmfrGeom& g) { return (size_t)kAlphaSize * g.wp * g.hp * sizeof(uint16_t); }
 static size_t weight_bytes(const BmfrGeom& g) { return (size_t)g.blocks_x * g.blocks_y * 30 * sizeof(float); }
 
+static void require_bmfr_images(const pbrt_bmfr_images* im, const char* who) {
+    PBRT_REQUIRE(im->depth && im->normal && im->albedo && im->motion && im->spp && im->noisy && im->denoised &&
+                     im->final_image,
+                 std::string(who) + ": missing image");
+}
+
+// pre + fit + post of one frame with the feature array (Householder fit only) and the
+// weights at the given device addresses.
+static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, BmfrGeom g, uint16_t* features,
+                        float* weights, hipStream_t st) {
+    g.features = features;
+    g.weights = weights;
+    dim3 grid(g.blocks_x, g.blocks_y);
+    const int threads = g.bs * g.bs;
+    if (p.fit == PBRT_BMFR_FIT_HOUSEHOLDER) {
+        hipLaunchKernelGGL(k_bmfr_pre, grid, dim3(threads), 0, st, g, im);
+        if (g.bs == 32)
+            hipLaunchKernelGGL(k_bmfr_fit_householder<4>, grid, dim3(256), 0, st, g);
+        else if (g.bs == 16)
+            hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(256), 0, st, g);
+        else
+            hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
+    } else {
+        launch_gram_fit(p.fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, p.fit_threads ? p.fit_threads : kFitThreads, grid, st,
+                        g, im);
+    }
+    hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, im);
+    PBRT_HIP(hipGetLastError());
+}
+
+// Scratch of pbrt_bmfr_noalloc: a file-scope device array, one copy per GPU, held
+// until the process ends.  Host code never passes `scratch` itself to a kernel (on
+// the host it names a stand-in, not device memory): it uses the device address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+constexpr size_t kNoallocScratchBytes = PBRT_BMFR_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static char* device_scratch() {
+    int dev = 0;
+    PBRT_HIP(hipGetDevice(&dev));
+    PBRT_REQUIRE(dev >= 0 && dev < kMaxDevices, "pbrt_bmfr_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        PBRT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PBRT_REQUIRE(ptr != nullptr, "pbrt_bmfr_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return static_cast<char*>(g_scratch_ptr[dev]);
+}
+
 }  // namespace pbrt
 
 using namespace pbrt;
@@ -651,31 +706,32 @@ extern "C" int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* p, size_t* o
 extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* im, void* stream) {
     return guard([&] {
         PBRT_REQUIRE(p && im, "pbrt_bmfr: NULL argument");
-        PBRT_REQUIRE(im->depth && im->normal && im->albedo && im->motion && im->spp && im->noisy && im->denoised &&
-                         im->final_image && im->workspace,
-                     "pbrt_bmfr: missing image");
+        require_bmfr_images(im, "pbrt_bmfr");
+        PBRT_REQUIRE(im->workspace, "pbrt_bmfr: missing workspace");
         BmfrGeom g = make_geom(*p);
         size_t fb = (feature_bytes(g) + 255) & ~(size_t)255;
         PBRT_REQUIRE(im->workspace_bytes >= fb + weight_bytes(g), "pbrt_bmfr: workspace too small");
-        g.features = static_cast<uint16_t*>(im->workspace);
-        g.weights = reinterpret_cast<float*>(static_cast<char*>(im->workspace) + fb);
-        hipStream_t st = (hipStream_t)stream;
-        dim3 grid(g.blocks_x, g.blocks_y);
-        const int threads = g.bs * g.bs;
-        if (p->fit == PBRT_BMFR_FIT_HOUSEHOLDER) {
-            hipLaunchKernelGGL(k_bmfr_pre, grid, dim3(threads), 0, st, g, *im);
-            if (g.bs == 32)
-                hipLaunchKernelGGL(k_bmfr_fit_householder<4>, grid, dim3(256), 0, st, g);
-            else if (g.bs == 16)
-                hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(256), 0, st, g);
-            else
-                hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
-        } else {
-            launch_gram_fit(p->fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, p->fit_threads ? p->fit_threads : kFitThreads, grid,
-                            st, g, *im);
-        }
-        hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, *im);
-        PBRT_HIP(hipGetLastError());
+        launch_bmfr(*p, *im, g, static_cast<uint16_t*>(im->workspace),
+                    reinterpret_cast<float*>(static_cast<char*>(im->workspace) + fb), (hipStream_t)stream);
+    });
+}
+
+extern "C" int32_t pbrt_bmfr_noalloc(const pbrt_bmfr_params* p, const pbrt_bmfr_images* im, void* stream) {
+    return guard([&] {
+        PBRT_REQUIRE(p && im, "pbrt_bmfr_noalloc: NULL argument");
+        require_bmfr_images(im, "pbrt_bmfr_noalloc");
+        BmfrGeom g = make_geom(*p);
+        // the Gram fits keep only the weights; the Householder fit also stages the
+        // fp16 feature array.  Every byte used is written before it is read.
+        const size_t fb = p->fit == PBRT_BMFR_FIT_HOUSEHOLDER ? (feature_bytes(g) + 255) & ~(size_t)255 : 0;
+        const size_t need = fb + weight_bytes(g);
+        if (need > kNoallocScratchBytes)
+            fail(PBRT_ERR_INVALID_ARG, "pbrt_bmfr_noalloc: this frame needs " + std::to_string(need) +
+                                           " bytes of scratch, more than the " + std::to_string(kNoallocScratchBytes) +
+                                           " the library holds per device (use pbrt_bmfr with a workspace)");
+        char* base = device_scratch();
+        launch_bmfr(*p, *im, g, reinterpret_cast<uint16_t*>(base), reinterpret_cast<float*>(base + fb),
+                    (hipStream_t)stream);
     });
 }
 
