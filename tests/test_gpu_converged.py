@@ -1,0 +1,99 @@
+"""Converged images: the GPU tracers' 4096-sample mean against the FP64 oracle's
+(tests/golden/converged_4096spp.npz, written by tests/golden/make_converged_golden.py).
+
+Each case holds the camera still and traces sample s as frame s (the seed of
+random.glsl:88-96 takes the frame number, not sampleNumber).  Score: relMSE =
+mean over pixels and RGB of (a - b)^2 / (b^2 + 1e-2) of the demodulated illumination,
+over the pixels whose primary hit is the FP64 oracle's in the GPU's frame and in the
+oracle's FP32 mirror (same albedo, normals within 0.01 rad).  The excluded pixels look
+at an edge or corner where two surfaces meet: the ray lies within rounding of the
+edge, and FP32 and FP64 pick different surfaces (4 of 4096 in the mirror on the
+Cornell box, none on the atrium view); at most 0.5 % of the pixels may be excluded.
+
+* Same frames 0..4095: the GPU follows the FP32 paths, so its converged image differs
+  from the FP64 oracle's only where FP32 and FP64 paths part.  Bound: relMSE <= 0.1x
+  the noise level of two independent 4096-spp estimates (Cornell box: 7.2e-4, the
+  oracle's own FP32 mirror 8e-11; 4-bounce atrium: 3.8e-3, the mirror 9.2e-5).
+* Frames 4096..8191 (independent samples): relMSE must equal the noise level the
+  oracle's per-pixel variance predicts, 2 var / 4096 per pixel (ratio in [0.67, 1.5];
+  the FP32 mirror gives 1.04 on both), and the mean difference over the pixels must be
+  within 5 standard errors of 0 (a 1 % bias gives z = 43 on the Cornell box, 8 on the
+  atrium).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from vulkanpbrt_amd import camera, capi
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+GOLDEN = Path(__file__).resolve().parent / "golden" / "converged_4096spp.npz"
+CASES = {"cornell_64x64": (1, 64, 64, 2), "atrium_64x36": (2, 64, 36, 4)}
+
+
+@pytest.fixture(scope="module")
+def golden():
+    g = np.load(GOLDEN)
+    return g, json.loads(bytes(g["meta"]).decode())
+
+
+def unit(spherical):
+    """G-buffer normal (theta = acos z, phi = atan(y, x), ptRaygen.rgen:51-53) as a unit vector."""
+    t, p = spherical[..., 0].astype(np.float64), spherical[..., 1].astype(np.float64)
+    return np.stack([np.cos(p) * np.sin(t), np.sin(p) * np.sin(t), np.cos(t)], -1)
+
+
+def gpu_mean(cuda, scene, gen, w, h, depth, tracer, first):
+    """Mean demodulated illumination of frames first..first+N-1, and the first frame's
+    G-buffer albedo and normal."""
+    view, proj = camera.pose_matrices(gen.camera(0), w, h)
+    img = helpers.gbuffer(cuda, w, h)
+    acc = cuda.zeros((h, w, 3), dtype=cuda.float64, device="cuda")
+    for f in range(first, first + N):
+        pc = camera.push_constants(view, proj, view, f, 0)
+        helpers.trace(cuda, scene, pc, img, w, h, max_depth=depth, tracer=tracer)
+        acc += img["illumination"][..., :3].double()
+        if f == first:
+            albedo, normal = img["albedo"].cpu().numpy(), img["normal"].cpu().numpy()
+    return (acc / N).cpu().numpy(), albedo, normal
+
+
+def relmse(a, b):
+    return float(np.mean((a - b) ** 2 / (b * b + 1e-2)))
+
+
+@pytest.mark.parametrize("tracer", [capi.TRACER_MEGAKERNEL, capi.TRACER_WAVEFRONT])
+@pytest.mark.parametrize("name", list(CASES))
+def test_converged_against_fp64_oracle(cuda, golden, name, tracer):
+    g, meta = golden
+    cfg, w, h, depth = CASES[name]
+    gen = capi.GeneratedScene(cfg, cfg)
+    scene = capi.Scene(gen.desc)
+    m64 = g[f"{name}_mean64"].astype(np.float64)
+    var = np.maximum(g[f"{name}_m2_64"].astype(np.float64) - m64 * m64, 0.0)
+    same, albedo, normal = gpu_mean(cuda, scene, gen, w, h, depth, tracer, 0)
+    gpu_primary = np.all(albedo == g[f"{name}_albedo"], -1) & (
+        np.sum(unit(normal) * unit(g[f"{name}_normal"]), -1) > np.cos(0.01))
+    keep = g[f"{name}_same_primary"] & gpu_primary
+    edges = np.argwhere(~keep)
+    print(f"{name} tracer {tracer}: {len(edges)} edge pixels excluded {edges[:8].tolist()}")
+    assert len(edges) <= 0.005 * w * h
+    expected = float(np.mean((2 * var / N / (m64 * m64 + 1e-2))[keep]))
+    bound_same = 0.1 * expected
+    rel_same = relmse(same[keep], m64[keep])
+    indep, _, _ = gpu_mean(cuda, scene, gen, w, h, depth, tracer, N)
+    rel_indep = relmse(indep[keep], m64[keep])
+    live = np.broadcast_to(keep[..., None], m64.shape) & (var > 0)
+    z = float((indep - m64)[live].sum() / np.sqrt((2 * var / N)[live].sum()))
+    print(f"{name} tracer {tracer}: relMSE same frames {rel_same:.3e} (bound {bound_same:.1e}, FP32 mirror "
+          f"{meta[name]['relmse_mirror32_vs_fp64']:.3e}); independent frames {rel_indep:.3e} = "
+          f"{rel_indep / expected:.3f} x the predicted {expected:.3e}; mean difference z = {z:.2f}")
+    assert np.isfinite(same).all() and np.isfinite(indep).all()
+    assert rel_same <= bound_same
+    assert 0.67 <= rel_indep / expected <= 1.5
+    assert abs(z) <= 5.0
