@@ -406,8 +406,10 @@ typedef struct pbrt_bmfr_images {
     size_t workspace_bytes;
 } pbrt_bmfr_images;
 
-/* Workspace layout: fp16 feature buffer [13][Hp][Wp] (Householder fit only),
- * then float weights [blocks_y][blocks_x][30] (feature-major, rgb inner). */
+/* Workspace layout (each part 256-byte aligned): fp16 feature buffer [13][Hp][Wp]
+ * (Householder fit only), float weights [blocks_y][blocks_x][30] (feature-major,
+ * rgb inner), then the frame's noise table float [10][block_size^2] (Gram fits:
+ * the addRandom values, computed once per frame). */
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
@@ -455,8 +457,9 @@ typedef struct pbrt_bmfr_batch_frame {
 } pbrt_bmfr_batch_frame;
 
 int32_t pbrt_bmfr_batched_workspace_size(const pbrt_bmfr_batch_frame* frames, uint32_t count, size_t* out_bytes);
-/* Workspace layout: frame table, then per frame (256-byte aligned) float weights
- * [blocks_y][blocks_x][30] in frame order. */
+/* Workspace layout: frame table, then per frame (each 256-byte aligned) float
+ * weights [blocks_y][blocks_x][30] and the frame's noise table [10][block_size^2],
+ * in frame order. */
 int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32_t count, void* workspace,
                           size_t workspace_bytes, void* stream);
 

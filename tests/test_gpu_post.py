@@ -127,7 +127,7 @@ def run_bmfr(cuda, fit, bs, frame, inputs, denoised, fit_threads=0, tune=False):
     hp = (H // bs + 2) * bs
     fbytes = (13 * wp * hp * 2 + 255) // 256 * 256
     feats = wsn[:13 * wp * hp * 2].view(np.uint16).reshape(13, hp, wp)
-    weights = wsn[fbytes:].view(np.float32).reshape(hp // bs, wp // bs, 30)
+    weights = wsn[fbytes:fbytes + 4 * 30 * (hp // bs) * (wp // bs)].view(np.float32).reshape(hp // bs, wp // bs, 30)
     return feats, weights, den.cpu().numpy().view(np.uint16), final.cpu().numpy()
 
 

@@ -95,7 +95,8 @@ def test_atrium_post_chain_matches_oracle(cuda, oracle_lib, atrium_frames):
     cuda.cuda.synchronize()
     wp, hp = (W // BS + 2) * BS, (H // BS + 2) * BS
     fbytes = (13 * wp * hp * 2 + 255) // 256 * 256
-    weights = ws.cpu().numpy()[fbytes:].view(np.float32).reshape(hp // BS, wp // BS, 30)
+    nw = 30 * (hp // BS) * (wp // BS)
+    weights = ws.cpu().numpy()[fbytes:fbytes + 4 * nw].view(np.float32).reshape(hp // BS, wp // BS, 30)
     feats = oracle_lib.bmfr_pre(W, H, BS, frame, f1["depth"], f1["normal"], noisy)
     W64 = oracle_lib.bmfr_lsq64(W, H, BS, frame, feats)
     _, _, nbx, nby = oracle_lib.bmfr_geometry(W, H, BS)

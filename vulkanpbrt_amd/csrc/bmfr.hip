@@ -44,7 +44,41 @@ struct BmfrGeom {
     int fit_threads;
     uint16_t* features;  // [13][hp][wp] fp16
     float* weights;      // [blocks_y][blocks_x][30]
+    float* noise;        // [10][bs*bs]: the noise of feature c for each block pixel (Gram fits), in
+                         // the fit's load order: entry c * bs^2 + t * kSub + s for thread t's s-th pixel
 };
+
+// sin and cos of one argument: Cody-Waite reduction by pi/2 (pi/2 split into three
+// floats, the first two with 12 significant bits so k * part is exact for |k| < 2^12),
+// then polynomials in r^2 on [-pi/4, pi/4] fitted by least squares on Chebyshev
+// nodes (within 1.5 fp32 ulp of the correctly rounded values for |x| <= 4; the
+// G-buffer's angles lie in [-pi, pi]).  Far cheaper than the general sincosf, whose
+// large-argument reduction the straight-line code would always execute.
+__device__ __forceinline__ void sincos_small(float x, float& sn, float& cs) {
+    const float k = rintf(x * 0.636619772f);
+    float r = fmaf(-k, 1.5703125f, x);
+    r = fmaf(-k, 4.83751297e-4f, r);
+    r = fmaf(-k, 7.54979013e-8f, r);
+    const float r2 = r * r;
+    const float ps = fmaf(fmaf(-1.95202127e-4f, r2, 8.33220407e-3f), r2, -0.166666552f);
+    const float s = fmaf(r * r2, ps, r);
+    const float pc = fmaf(fmaf(fmaf(2.43941031e-5f, r2, -1.38868089e-3f), r2, 4.16666232e-2f), r2, -0.5f);
+    const float c = fmaf(r2, pc, 1.0f);
+    const int q = (int)k & 3;
+    sn = (q & 1) ? c : s;
+    cs = (q & 1) ? s : c;
+    if (q == 1 || q == 2) cs = -cs;
+    if (q >= 2) sn = -sn;
+}
+
+// Spherical G-buffer normal (theta, phi) as (cos phi sin theta, sin phi sin theta,
+// cos theta), bmfrPre.comp:26-29.
+__device__ __forceinline__ vec3 bmfr_decode_normal(float theta, float phi) {
+    float st, ct, sp, cp;
+    sincos_small(theta, st, ct);
+    sincos_small(phi, sp, cp);
+    return make_vec3(cp * st, sp * st, ct);
+}
 
 // random(uint) of bmfrGeneral.comp:83-93 (Jenkins' 6-step integer hash)
 __device__ __forceinline__ float bmfr_random(uint32_t a) {
@@ -54,13 +88,40 @@ __device__ __forceinline__ float bmfr_random(uint32_t a) {
     a = (a + 0xd3a2646cu) ^ (a << 9);
     a = (a + 0xfd7046c5u) + (a << 3);
     a = (a ^ 0xb55a4f09u) ^ (a >> 16);
-    return __fdiv_rn((float)a, 4294967296.0f);
+    // float(a) / 2^32: dividing by a power of two is exact, so is this product
+    return __fmul_rn((float)a, 0x1p-32f);
 }
 
 // addRandom (bmfrGeneral.comp:95-97) without contraction
 __device__ __forceinline__ float add_noise(float v, uint32_t key) {
     float r = __fsub_rn(bmfr_random(key), 0.5f);
     return __fadd_rn(v, __fmul_rn(2e-4f, r));
+}
+
+// The noise of the fused Gram fits.  Its key depends on the row within the block,
+// the feature and the frame only, so every block of a frame adds the same values:
+// they are computed once per frame, in the order the fit's threads load the block's
+// pixels.  With `threads` threads, thread t loads pixels q = t + s * threads
+// (s < rows / threads), and entry c * rows + t * (rows / threads) + s holds the
+// noise of feature c for pixel q = block-local (q % bs, q / bs) = fit row
+// (q % bs) * bs + q / bs, so a thread reads its pixels' values as one vector.
+// The fit adds them with __fadd_rn, as add_noise does.
+__device__ __forceinline__ int fit_rows_per_thread(int rows, int threads) { return rows > threads ? rows / threads : 1; }
+
+__device__ __forceinline__ void noise_table_entry(uint32_t frame, int bs, int threads, int e, float* table) {
+    const int rows = bs * bs;
+    const int sub = fit_rows_per_thread(rows, threads);
+    const int c = e / rows, k = e % rows;
+    const int q = (k / sub) + (k % sub) * threads;
+    const uint32_t pb = (uint32_t)rows;
+    const uint32_t r = (uint32_t)((q % bs) * bs + q / bs);
+    const uint32_t key = r + (uint32_t)c * pb * pb + frame * (uint32_t)kAlphaSize * pb * pb;
+    table[e] = __fmul_rn(2e-4f, __fsub_rn(bmfr_random(key), 0.5f));
+}
+
+__global__ void __launch_bounds__(256) k_bmfr_noise(uint32_t frame, int bs, int threads, float* table) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < kFeatures * bs * bs) noise_table_entry(frame, bs, threads, e, table);
 }
 
 // ---------------------------------------------------------------------------
@@ -88,6 +149,42 @@ __device__ T block_reduce(T v, Op op, T* scratch) {
     T r = scratch[0];
     __syncthreads();
     return r;
+}
+
+// Wave64 reduction without LDS: DPP swaps within each row of 16 lanes (xor 1,
+// xor 2, half-row mirror, row mirror leave every lane of a row with the row's
+// result), then the four rows combined from lanes 0, 16, 32, 48.  Every lane
+// returns the same value, combined in the same order.
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, int ctrl) {
+    switch (ctrl) {
+        case 0xB1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+        case 0x4E: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+        case 0x141: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false); // row_half_mirror
+        default: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);   // row_mirror
+    }
+}
+__device__ __forceinline__ float dpp_move(float v, int ctrl) { return __uint_as_float(dpp_u32(__float_as_uint(v), ctrl)); }
+__device__ __forceinline__ double dpp_move(double v, int ctrl) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t lo = dpp_u32((uint32_t)b, ctrl), hi = dpp_u32((uint32_t)(b >> 32), ctrl);
+    return __longlong_as_double((long long)(hi << 32 | lo));
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)((uint64_t)hi << 32 | lo));
+}
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce_dpp(T v, Op op) {
+    v = op(v, dpp_move(v, 0xB1));
+    v = op(v, dpp_move(v, 0x4E));
+    v = op(v, dpp_move(v, 0x141));
+    v = op(v, dpp_move(v, 0x140));
+    return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
 }
 
 struct OpMin {
@@ -118,7 +215,7 @@ __device__ __forceinline__ PixelIn load_pixel(const BmfrGeom& g, const pbrt_bmfr
     size_t pix = (size_t)p.img_y * g.width + p.img_x;
     p.depth = im.depth[pix];
     float2 n = reinterpret_cast<const float2*>(im.normal)[pix];
-    p.normal = decode_normal(n.x, n.y);
+    p.normal = bmfr_decode_normal(n.x, n.y);
     p.noisy = xyz(load_half4(im.noisy, pix));
     return p;
 }
@@ -231,248 +328,340 @@ typedef double double4v __attribute__((ext_vector_type(4)));
 constexpr int kGramCols = 16;  // 9 projected features + 3 projected rgb + 4 zero columns
 constexpr int kFitThreads = 256;
 
-template <int kRows, bool kF64, int kWaves>
-struct GramLds {
-    // F16: column-major fp16 hi / lo planes; F64: row-major fp32 rows
+// Wave-private staging of 64 rows (one per lane) for the Gram products, the
+// per-wave partial Gram matrices, and the per-wave partial sums of the block
+// reductions (each reduction costs one workgroup barrier).
+template <bool kF64, int kWaves>
+struct GramShared {
     union {
+        float f[kWaves][64][kGramCols + 1];  // F64: row-major fp32 rows (odd stride: no bank conflicts)
         struct {
-            _Float16 hi[kGramCols][kRows];
-            _Float16 lo[kGramCols][kRows];
+            _Float16 hi[kWaves][kGramCols][64];  // F16: column-major hi / lo planes
+            _Float16 lo[kWaves][kGramCols][64];
         } h;
-        float f[kRows][kGramCols];
-    } m;
-    double gram[kWaves][kGramCols * kGramCols];  // one partial Gram matrix per wave
+    } st;
+    double gram[kWaves][kGramCols * kGramCols];
+    float depth_min[kWaves], depth_max[kWaves];
+    float coord[32];
+    double dots_part[kWaves][kAlphaSize];
+    float amax_part[kWaves][12];
+    double G[12][12];
+    double Lc[kFeatures - 1][kFeatures - 1];
+    double inv_diag[kFeatures - 1];
+    int keep[kFeatures - 1];
 };
 
-// kThreads: workgroup size (256, 512 or 1024); each of its waves accumulates the
-// Gram matrix of every kThreads/64-th row chunk, so the thread count sets how the
-// block's rows are tiled over the matrix cores (pbrt_autotune_bmfr picks it).
+// LDS written by some lanes of a wave and then read by others of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// kThreads: workgroup size (256, 512 or 1024); its waves split the block's rows,
+// so the thread count sets how the rows are tiled over the matrix cores
+// (pbrt_autotune_bmfr picks it).  Thread t loads pixels t, t + kThreads, ... of
+// the block in row-major order (x fastest: coalesced); the pixel at block-local
+// (x, y) is fit row i = x * bs + y, whose index keys the noise.
 template <int kRows, bool kF64, int kThreads>
 __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
     constexpr int kSub = (kRows + kThreads - 1) / kThreads;
     constexpr int kWaves = kThreads / 64;
-    __shared__ float fscratch[16];
-    __shared__ double dscratch[16];
-    __shared__ GramLds<kRows, kF64, kWaves> L;
-    __shared__ double dots[kAlphaSize];
-    __shared__ float scale[12];
-    __shared__ double G[12][12];
-    __shared__ double Lc[kFeatures - 1][kFeatures - 1];
-    __shared__ int keep[kFeatures - 1];
-    const int tid = threadIdx.x;
+    __shared__ GramShared<kF64, kWaves> S;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int bs = g.bs;
-    const uint32_t pb = (uint32_t)(bs * bs);
     const int basex = bx * bs, basey = by * bs;
 
-    // 1. load rows; block min/max of depth
+    // 1. load pixels (normal and noisy colour straight into their feature slots);
+    //    block min/max of depth
     float a[kSub][kAlphaSize];
+    float depth[kSub];
+    int row[kSub];
     float dmin = 3.4e38f, dmax = -3.4e38f;
-    PixelIn px[kSub];
-    #pragma unroll
+#pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kThreads;
-        if (r < kRows) {
-            px[s] = load_pixel(g, im, basex + r / bs, basey + r % bs);
-            dmin = fminf(dmin, px[s].depth);
-            dmax = fmaxf(dmax, px[s].depth);
+        const int q = tid + s * kThreads;
+        row[s] = -1;
+        if (q < kRows) {
+            const int qx = q % bs, qy = q / bs;
+            row[s] = qx * bs + qy;
+            const PixelIn p = load_pixel(g, im, basex + qx, basey + qy);
+            depth[s] = p.depth;
+            a[s][1] = p.normal.x;
+            a[s][2] = p.normal.y;
+            a[s][3] = p.normal.z;
+            a[s][10] = p.noisy.x;
+            a[s][11] = p.noisy.y;
+            a[s][12] = p.noisy.z;
+            dmin = fminf(dmin, p.depth);
+            dmax = fmaxf(dmax, p.depth);
         }
     }
-    const float mn = block_reduce(dmin, OpMin(), fscratch);
-    const float mx = block_reduce(dmax, OpMax(), fscratch);
-    // 2. fp16-quantised features (the reference's R16F feature buffer) + noise
-    #pragma unroll
+    dmin = wave_reduce_dpp(dmin, OpMin());
+    dmax = wave_reduce_dpp(dmax, OpMax());
+    if (lane == 0) {
+        S.depth_min[wave] = dmin;
+        S.depth_max[wave] = dmax;
+    }
+    // block coordinates i / (bs - 1) (bmfrPre.comp:36-43), one IEEE division each
+    if (tid < bs) S.coord[tid] = (float)tid / (float)(bs - 1);
+    __syncthreads();
+    float mn = S.depth_min[0], mx = S.depth_max[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+        mn = fminf(mn, S.depth_min[w]);
+        mx = fmaxf(mx, S.depth_max[w]);
+    }
+    // this thread's noise values, kSub consecutive floats per feature
+    float noise[kFeatures][kSub];
+#pragma unroll
+    for (int c = 0; c < kFeatures; ++c) {
+        const float* src = g.noise + c * kRows + tid * kSub;
+        if (tid * kSub < kRows) {
+            if constexpr (kSub == 4) {
+                const float4 v = *reinterpret_cast<const float4*>(src);
+                noise[c][0] = v.x;
+                noise[c][1] = v.y;
+                noise[c][2] = v.z;
+                noise[c][3] = v.w;
+            } else {
+#pragma unroll
+                for (int s = 0; s < kSub; ++s) noise[c][s] = src[s];
+            }
+        }
+    }
+    // 2. fp16-quantised features (the reference's R16F feature buffer) + noise;
+    //    FP64 sums a0.a_c with the noised constant column a0
+    double d_local[kAlphaSize];
+#pragma unroll
+    for (int c = 0; c < kAlphaSize; ++c) d_local[c] = 0.0;
+#pragma unroll
     for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kThreads;
-        if (r >= kRows) continue;
+        if (row[s] < 0) continue;
+        const int r = row[s];
         const int lx = r / bs, ly = r % bs;
-        float z = (px[s].depth - mn) / (mx - mn + 1e-6f);
-        float fx = (float)lx / (float)(bs - 1), fy = (float)ly / (float)(bs - 1);
-        float f[kAlphaSize] = {1.0f, px[s].normal.x, px[s].normal.y, px[s].normal.z, fx, fy, z, fx * fx, fy * fy,
-                               z * z, px[s].noisy.x, px[s].noisy.y, px[s].noisy.z};
+        float z = (depth[s] - mn) / (mx - mn + 1e-6f);
+        const float fx = S.coord[lx], fy = S.coord[ly];
+        float f[kAlphaSize] = {1.0f, a[s][1], a[s][2], a[s][3], fx, fy, z, fx * fx, fy * fy,
+                               z * z, a[s][10], a[s][11], a[s][12]};
+#pragma unroll
         for (int c = 0; c < kAlphaSize; ++c) {
             float v = h2f(f2h(f[c]));
-            if (c < kFeatures)
-                v = add_noise(v, (uint32_t)r + (uint32_t)c * pb * pb + g.frame * (uint32_t)kAlphaSize * pb * pb);
+            if (c < kFeatures) v = __fadd_rn(v, noise[c][s]);
             a[s][c] = v;
         }
-    }
-    // 3. FP64 dot products with the constant column: a0.a0, a0.A_j, a0.y_c
-    double d_local[kAlphaSize];
-    for (int c = 0; c < kAlphaSize; ++c) d_local[c] = 0.0;
-    #pragma unroll
-    for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kThreads;
-        if (r >= kRows) continue;
         const double a0 = a[s][0];
+#pragma unroll
         for (int c = 0; c < kAlphaSize; ++c) d_local[c] += a0 * (double)a[s][c];
     }
-    for (int c = 0; c < kAlphaSize; ++c) {
-        double t = block_reduce(d_local[c], OpAdd(), dscratch);
-        if (tid == 0) dots[c] = t;
+#pragma unroll
+    for (int c = 0; c < kAlphaSize; ++c) d_local[c] = wave_reduce_dpp(d_local[c], OpAdd());
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < kAlphaSize; ++c) S.dots_part[wave][c] = d_local[c];
     }
     __syncthreads();
-    // 4. project against a0 (FP64), keep fp32; block max-abs per column
-    double coef[12];
-    for (int c = 0; c < 12; ++c) coef[c] = dots[c + 1] / dots[0];
-    float b[kSub][12];
+    // block sum c, folded in wave order (every thread gets the same value)
+    auto dot = [&](int c) {
+        double t = S.dots_part[0][c];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) t += S.dots_part[w][c];
+        return t;
+    };
+    // 3. project the columns against a0 (FP64), keep fp32; block max-abs per column
+    const double inv_a0 = 1.0 / dot(0);
     float amax[12];
-    for (int c = 0; c < 12; ++c) amax[c] = 0.0f;
-    #pragma unroll
-    for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kThreads;
-        if (r >= kRows) continue;
-        const double a0 = a[s][0];
-        for (int c = 0; c < 12; ++c) {
-            b[s][c] = (float)((double)a[s][c + 1] - coef[c] * a0);
-            amax[c] = fmaxf(amax[c], fabsf(b[s][c]));
-        }
-    }
+#pragma unroll
     for (int c = 0; c < 12; ++c) {
-        float m = block_reduce(amax[c], OpMax(), fscratch);
-        if (tid == 0) {
-            int e = 0;
-            frexpf(m, &e);
-            scale[c] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;  // |scaled| < 1, exact power of two
+        const double coef = dot(c + 1) * inv_a0;
+        amax[c] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < kSub; ++s) {
+            if (row[s] < 0) continue;
+            const float b = (float)((double)a[s][c + 1] - coef * (double)a[s][0]);
+            a[s][c + 1] = b;
+            amax[c] = fmaxf(amax[c], fabsf(b));
         }
     }
-    __syncthreads();
-    // 5. stage the scaled columns in LDS
-    #pragma unroll
-    for (int s = 0; s < kSub; ++s) {
-        const int r = tid + s * kThreads;
-        if (r >= kRows) continue;
-        for (int c = 0; c < kGramCols; ++c) {
-            float v = c < 12 ? b[s][c] * scale[c] : 0.0f;
-            if (kF64) {
-                L.m.f[r][c] = v;
-            } else {
-                _Float16 hi = (_Float16)v;
-                L.m.h.hi[c][r] = hi;
-                L.m.h.lo[c][r] = (_Float16)(v - (float)hi);
-            }
-        }
+#pragma unroll
+    for (int c = 0; c < 12; ++c) amax[c] = wave_reduce_dpp(amax[c], OpMax());
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) S.amax_part[wave][c] = amax[c];
     }
     __syncthreads();
-    // 6. Gram on the matrix cores.  For both instructions the A and B operands of a
-    //    Gram product hold the same values (lane l: column l % 16 of a row group).
+    float scale[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        float m = S.amax_part[0][c];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) m = fmaxf(m, S.amax_part[w][c]);
+        int e = 0;
+        frexpf(m, &e);
+        scale[c] = m > 0.0f ? ldexpf(1.0f, -e) : 1.0f;  // |scaled| < 1, exact power of two
+    }
+    // 4. Gram on the matrix cores, 64 rows at a time through the wave's own LDS
+    //    slice (lane l stages its row of chunk s).  For both instructions the A and
+    //    B operands of a Gram product hold the same values (lane l: column l % 16
+    //    of a row group).
     {
-        const int wave = tid >> 6, lane = tid & 63;
         const int col = lane & 15, grp = lane >> 4;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        if (kF64) {
-            constexpr int kChunks = kRows / 4;
-            double4v d = {0.0, 0.0, 0.0, 0.0};
-            for (int ch = wave; ch < kChunks; ch += kWaves) {
-                double x = (double)L.m.f[ch * 4 + grp][col];
-                d = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, d, 0, 0, 0);
-            }
+        double4v d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] = d[i];
-        } else {
-            constexpr int kChunks = kRows / 32;
-            for (int ch = wave; ch < kChunks; ch += kWaves) {
-                const int row0 = ch * 32 + grp * 8;
-                half8 hi, lo;
+        for (int s = 0; s < kSub; ++s) {
+            if (s > 0) wave_lds_sync();  // the previous chunk has been read
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    hi[j] = L.m.h.hi[col][row0 + j];
-                    lo[j] = L.m.h.lo[col][row0 + j];
+            for (int c = 0; c < kGramCols; ++c) {
+                const float v = (c < 12 && row[s] >= 0) ? a[s][c + 1] * scale[c] : 0.0f;
+                if (kF64) {
+                    S.st.f[wave][lane][c] = v;
+                } else {
+                    const _Float16 hi = (_Float16)v;
+                    S.st.h.hi[wave][c][lane] = hi;
+                    S.st.h.lo[wave][c][lane] = (_Float16)(v - (float)hi);
                 }
-                float4v d = {0.0f, 0.0f, 0.0f, 0.0f};
-                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, d, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, d, 0, 0, 0);
+            }
+            wave_lds_sync();
+            if (kF64) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i] += (double)d[i];
+                for (int ch = 0; ch < 16; ++ch) {
+                    const double x = (double)S.st.f[wave][ch * 4 + grp][col];
+                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, d, 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < 2; ++ch) {
+                    const int row0 = ch * 32 + grp * 8;
+                    half8 hi, lo;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        hi[j] = S.st.h.hi[wave][col][row0 + j];
+                        lo[j] = S.st.h.lo[wave][col][row0 + j];
+                    }
+                    float4v e = {0.0f, 0.0f, 0.0f, 0.0f};
+                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, e, 0, 0, 0);
+                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, e, 0, 0, 0);
+                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, e, 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i] += (double)e[i];
+                }
             }
         }
-        // D layout (both instructions: lane holds column `col`): register i of
-        // v_mfma_f32_16x16x32_f16 is row 4 * grp + i, register i of
-        // v_mfma_f64_16x16x4f64 is row grp + 4 * i
+        if (kF64) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = d[i];
+        }
+        // D layout (lane holds column `col`): register i of v_mfma_f32_16x16x32_f16
+        // is row 4 * grp + i, register i of v_mfma_f64_16x16x4f64 is row grp + 4 * i
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = kF64 ? grp + 4 * i : 4 * grp + i;
-            L.gram[wave][row * kGramCols + col] = acc[i];
+            const int r = kF64 ? grp + 4 * i : 4 * grp + i;
+            S.gram[wave][r * kGramCols + col] = acc[i];
         }
     }
     __syncthreads();
-    // 7. FP64 solve of the projected normal equations: the symmetrised, unscaled
-    //    Gram matrix in LDS; Cholesky column by column (wave 0: every lane forms
-    //    the pivot, lane i > j its row of the column; a pivot below 1e-15 of the
-    //    largest diagonal drops its feature); then one lane per colour solves for
-    //    w1..w9 and w0 = a0.(y - A'w') / a0.a0.
-    constexpr int n = kFeatures - 1;  // projected features A1..A9
+    // 5. the symmetrised, unscaled 12 x 12 Gram matrix (9 projected features, 3 colours)
     constexpr int pw = kGramCols * kGramCols;
     for (int e = tid; e < 12 * 12; e += kThreads) {
         const int i = e / 12, j = e % 12;
-        const int a = i * kGramCols + j, b = j * kGramCols + i;
-        const double* gr = &L.gram[0][0];
+        const int ia = i * kGramCols + j, ib = j * kGramCols + i;
+        const double* gr = &S.gram[0][0];
         double gij = 0.0, gji = 0.0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            gij += gr[w * pw + a];
-            gji += gr[w * pw + b];
+            gij += gr[w * pw + ia];
+            gji += gr[w * pw + ib];
         }
-        G[i][j] = 0.5 * (gij + gji) / ((double)scale[i] * (double)scale[j]);
+        S.G[i][j] = 0.5 * (gij + gji) / ((double)scale[i] * (double)scale[j]);
     }
     __syncthreads();
+    if (wave != 0) return;
+    // 6. FP64 Cholesky of the 9 x 9 feature block in wave 0, lane i holding row i
+    //    (a pivot below 1e-15 of the largest diagonal drops its feature), then one
+    //    lane per colour solves for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+    constexpr int n = kFeatures - 1;
     double maxdiag = 0.0;
-    for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, G[i][i]);
-    for (int j = 0; j < n; ++j) {
-        if (tid < 64) {
-            double d = G[j][j];
-            for (int k = 0; k < j; ++k) d -= Lc[j][k] * Lc[j][k];
-            const bool kp = d > 1e-15 * maxdiag && d > 0.0;
-            const double ljj = kp ? sqrt(d) : 1.0;
-            if (tid == j) {
-                Lc[j][j] = ljj;
-                keep[j] = kp ? 1 : 0;
-            } else if (tid > j && tid < n) {
-                double v = G[tid][j];
-                for (int k = 0; k < j; ++k) v -= Lc[tid][k] * Lc[j][k];
-                Lc[tid][j] = kp ? v / ljj : 0.0;
-            }
-        }
-        __syncthreads();
+#pragma unroll
+    for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, S.G[i][i]);
+    const int li = lane < n ? lane : 0;
+    double grow[n], lrow[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        grow[k] = S.G[li][k];
+        lrow[k] = 0.0;
     }
-    if (tid < 3) {
-        const int c = tid;
+#pragma unroll
+    for (int j = 0; j < n; ++j) {
+        double dj = grow[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= lrow[k] * lrow[k];
+        dj = read_lane(dj, j);
+        const bool kp = dj > 1e-15 * maxdiag && dj > 0.0;
+        const double ljj = kp ? sqrt(dj) : 1.0;
+        const double inv = 1.0 / ljj;
+        double v = grow[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) v -= lrow[k] * read_lane(lrow[k], j);
+        lrow[j] = lane == j ? ljj : (lane > j && kp ? v * inv : 0.0);
+        if (lane == 0) {
+            S.keep[j] = kp ? 1 : 0;
+            S.inv_diag[j] = kp ? inv : 0.0;
+        }
+    }
+    if (lane < n) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) S.Lc[lane][k] = lrow[k];
+    }
+    wave_lds_sync();
+    if (lane < 3) {
+        const int c = lane;
         float* out = g.weights + ((size_t)by * g.blocks_x + bx) * 30;
         double z[n], w[n];
 #pragma unroll
         for (int i = 0; i < n; ++i) {
-            double v = keep[i] ? G[i][9 + c] : 0.0;
+            double v = S.G[i][9 + c];
 #pragma unroll
-            for (int k = 0; k < i; ++k) v -= Lc[i][k] * z[k];
-            z[i] = keep[i] ? v / Lc[i][i] : 0.0;
+            for (int k = 0; k < i; ++k) v -= S.Lc[i][k] * z[k];
+            z[i] = v * S.inv_diag[i];
         }
 #pragma unroll
         for (int i = n - 1; i >= 0; --i) {
             double v = z[i];
 #pragma unroll
-            for (int k = i + 1; k < n; ++k) v -= Lc[k][i] * w[k];
-            w[i] = keep[i] ? v / Lc[i][i] : 0.0;
+            for (int k = i + 1; k < n; ++k) v -= S.Lc[k][i] * w[k];
+            w[i] = v * S.inv_diag[i];
         }
-        double w0 = dots[10 + c];
+        double w0 = dot(10 + c);
 #pragma unroll
-        for (int i = 0; i < n; ++i) w0 -= w[i] * dots[i + 1];
-        w0 /= dots[0];
+        for (int i = 0; i < n; ++i) w0 -= w[i] * dot(i + 1);
+        w0 *= inv_a0;
         out[c] = (float)w0;
 #pragma unroll
         for (int i = 0; i < n; ++i) out[(i + 1) * 3 + c] = (float)w[i];
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs): three blocks of 256 threads per CU left
+// the per-block latency (load, reductions, Cholesky) uncovered.
 template <int kRows, bool kF64, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bmfr_fit_gram(BmfrGeom g,
+                                                                                                  pbrt_bmfr_images im) {
     bmfr_fit_gram_block<kRows, kF64, kThreads>(g, im, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
 // bmfrPost: weighted feature sum, temporal blend, remodulation, gamma
 // ---------------------------------------------------------------------------
+// x^0.454545 for the gamma of bmfrPost.comp:122 as exp2(0.454545 * log2 x) on the
+// hardware log2 / exp2: within a few fp32 ulps of powf, and the result is stored as
+// unorm8.  x = 0 gives exp2(-inf) = 0.
+__device__ __forceinline__ float gamma_454545(float x) {
+    return __builtin_amdgcn_exp2f(0.454545f * __builtin_amdgcn_logf(x));
+}
+
 __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
-    __shared__ float scratch[16];
+    __shared__ float part_min[16], part_max[16];
     __shared__ float wts[30];
     const int lx = threadIdx.x % g.bs, ly = threadIdx.x / g.bs;
     const int gx = bx * g.bs + lx, gy = by * g.bs + ly;
@@ -482,8 +671,20 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
     }
     PixelIn p = load_pixel(g, im, gx, gy);
     const size_t pix = (size_t)p.img_y * g.width + p.img_x;
-    float mn = block_reduce(p.depth, OpMin(), scratch);
-    float mx = block_reduce(p.depth, OpMax(), scratch);
+    // block min / max of depth: one barrier, every thread folds the waves in order
+    {
+        const float wmin = wave_reduce_dpp(p.depth, OpMin()), wmax = wave_reduce_dpp(p.depth, OpMax());
+        if ((threadIdx.x & 63) == 0) {
+            part_min[threadIdx.x >> 6] = wmin;
+            part_max[threadIdx.x >> 6] = wmax;
+        }
+    }
+    __syncthreads();
+    float mn = part_min[0], mx = part_max[0];
+    for (int w = 1, nw = (int)(blockDim.x >> 6); w < nw; ++w) {
+        mn = fminf(mn, part_min[w]);
+        mx = fmaxf(mx, part_max[w]);
+    }
     if (!p.inside) return;
     const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
     float mux = h2f(mv.x), muy = h2f(mv.y);
@@ -511,7 +712,7 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
     vec3 alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
     vec3 t = alb * den;
     t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
-    t = vclamp(vpow(t, 0.454545f), 0.0f, 1.0f);
+    t = vclamp(make_vec3(gamma_454545(t.x), gamma_454545(t.y), gamma_454545(t.z)), 0.0f, 1.0f);
     store_bgra8(im.final_image, pix, make_vec4(t, 1.0f));
 }
 
@@ -542,13 +743,21 @@ __device__ __forceinline__ uint32_t batch_frame_of(const BmfrBatchFrame* frames,
 }
 
 template <int kRows, bool kF64>
-__global__ void __launch_bounds__(kFitThreads) k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
+__global__ void __launch_bounds__(kFitThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
     const uint32_t f = batch_frame_of(frames, n, blockIdx.x);
     const BmfrGeom g = frames[f].g;
     const pbrt_bmfr_images im = frames[f].im;
     const uint32_t local = blockIdx.x - frames[f].first_block;
     bmfr_fit_gram_block<kRows, kF64, kFitThreads>(g, im, (int)(local % (uint32_t)g.blocks_x),
                                                   (int)(local / (uint32_t)g.blocks_x));
+}
+
+// the noise table of every frame of a batch (blockIdx.y: frame)
+__global__ void __launch_bounds__(256) k_bmfr_noise_batched(const BmfrBatchFrame* frames) {
+    const BmfrGeom& g = frames[blockIdx.y].g;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < kFeatures * g.bs * g.bs) noise_table_entry(g.frame, g.bs, kFitThreads, e, g.noise);
 }
 
 __global__ void __launch_bounds__(1024) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
@@ -634,6 +843,8 @@ static void launch_gram_fit(bool f64, int bs, uint32_t threads, dim3 grid, hipSt
 
 static size_t feature_bytes(const BmfrGeom& g) { return (size_t)kAlphaSize * g.wp * g.hp * sizeof(uint16_t); }
 static size_t weight_bytes(const BmfrGeom& g) { return (size_t)g.blocks_x * g.blocks_y * 30 * sizeof(float); }
+static size_t noise_bytes(const BmfrGeom& g) { return (size_t)kFeatures * g.bs * g.bs * sizeof(float); }
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static void require_bmfr_images(const pbrt_bmfr_images* im, const char* who) {
     PBRT_REQUIRE(im->depth && im->normal && im->albedo && im->motion && im->spp && im->noisy && im->denoised &&
@@ -641,12 +852,13 @@ static void require_bmfr_images(const pbrt_bmfr_images* im, const char* who) {
                  std::string(who) + ": missing image");
 }
 
-// pre + fit + post of one frame with the feature array (Householder fit only) and the
-// weights at the given device addresses.
+// pre + fit + post of one frame with the feature array (Householder fit only), the
+// weights and the noise table (Gram fits only) at the given device addresses.
 static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, BmfrGeom g, uint16_t* features,
-                        float* weights, hipStream_t st) {
+                        float* weights, float* noise, hipStream_t st) {
     g.features = features;
     g.weights = weights;
+    g.noise = noise;
     dim3 grid(g.blocks_x, g.blocks_y);
     const int threads = g.bs * g.bs;
     if (p.fit == PBRT_BMFR_FIT_HOUSEHOLDER) {
@@ -658,8 +870,11 @@ static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, B
         else
             hipLaunchKernelGGL(k_bmfr_fit_householder<1>, grid, dim3(64), 0, st, g);
     } else {
-        launch_gram_fit(p.fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, p.fit_threads ? p.fit_threads : kFitThreads, grid, st,
-                        g, im);
+        const int entries = kFeatures * g.bs * g.bs;
+        const int threads = (int)(p.fit_threads ? p.fit_threads : kFitThreads);
+        hipLaunchKernelGGL(k_bmfr_noise, dim3((entries + 255) / 256), dim3(256), 0, st, g.frame, g.bs, threads,
+                           g.noise);
+        launch_gram_fit(p.fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, (uint32_t)threads, grid, st, g, im);
     }
     hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, im);
     PBRT_HIP(hipGetLastError());
@@ -698,8 +913,7 @@ extern "C" int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* p, size_t* o
     return guard([&] {
         PBRT_REQUIRE(p && out, "pbrt_bmfr_workspace_size: NULL argument");
         BmfrGeom g = make_geom(*p);
-        size_t fb = (feature_bytes(g) + 255) & ~(size_t)255;
-        *out = fb + weight_bytes(g);
+        *out = align256(feature_bytes(g)) + align256(weight_bytes(g)) + noise_bytes(g);
     });
 }
 
@@ -709,10 +923,11 @@ extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* 
         require_bmfr_images(im, "pbrt_bmfr");
         PBRT_REQUIRE(im->workspace, "pbrt_bmfr: missing workspace");
         BmfrGeom g = make_geom(*p);
-        size_t fb = (feature_bytes(g) + 255) & ~(size_t)255;
-        PBRT_REQUIRE(im->workspace_bytes >= fb + weight_bytes(g), "pbrt_bmfr: workspace too small");
-        launch_bmfr(*p, *im, g, static_cast<uint16_t*>(im->workspace),
-                    reinterpret_cast<float*>(static_cast<char*>(im->workspace) + fb), (hipStream_t)stream);
+        const size_t fb = align256(feature_bytes(g)), wb = align256(weight_bytes(g));
+        PBRT_REQUIRE(im->workspace_bytes >= fb + wb + noise_bytes(g), "pbrt_bmfr: workspace too small");
+        char* ws = static_cast<char*>(im->workspace);
+        launch_bmfr(*p, *im, g, reinterpret_cast<uint16_t*>(ws), reinterpret_cast<float*>(ws + fb),
+                    reinterpret_cast<float*>(ws + fb + wb), (hipStream_t)stream);
     });
 }
 
@@ -721,17 +936,19 @@ extern "C" int32_t pbrt_bmfr_noalloc(const pbrt_bmfr_params* p, const pbrt_bmfr_
         PBRT_REQUIRE(p && im, "pbrt_bmfr_noalloc: NULL argument");
         require_bmfr_images(im, "pbrt_bmfr_noalloc");
         BmfrGeom g = make_geom(*p);
-        // the Gram fits keep only the weights; the Householder fit also stages the
-        // fp16 feature array.  Every byte used is written before it is read.
-        const size_t fb = p->fit == PBRT_BMFR_FIT_HOUSEHOLDER ? (feature_bytes(g) + 255) & ~(size_t)255 : 0;
-        const size_t need = fb + weight_bytes(g);
+        // the Gram fits keep the weights and the frame's noise table; the Householder
+        // fit the fp16 feature array and the weights.  Every byte used is written
+        // before it is read.
+        const bool hh = p->fit == PBRT_BMFR_FIT_HOUSEHOLDER;
+        const size_t fb = hh ? align256(feature_bytes(g)) : 0, wb = align256(weight_bytes(g));
+        const size_t need = fb + wb + (hh ? 0 : noise_bytes(g));
         if (need > kNoallocScratchBytes)
             fail(PBRT_ERR_INVALID_ARG, "pbrt_bmfr_noalloc: this frame needs " + std::to_string(need) +
                                            " bytes of scratch, more than the " + std::to_string(kNoallocScratchBytes) +
                                            " the library holds per device (use pbrt_bmfr with a workspace)");
         char* base = device_scratch();
         launch_bmfr(*p, *im, g, reinterpret_cast<uint16_t*>(base), reinterpret_cast<float*>(base + fb),
-                    (hipStream_t)stream);
+                    reinterpret_cast<float*>(base + fb + wb), (hipStream_t)stream);
     });
 }
 
@@ -779,9 +996,8 @@ extern "C" int32_t pbrt_autotune_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr
 
 namespace {
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// Frame table of a batch and the workspace offsets of every frame's weights.
+// Frame table of a batch and the workspace offsets of every frame's weights and
+// noise table.
 struct BatchPlan {
     std::vector<BmfrBatchFrame> table;
     size_t table_bytes = 0, total_bytes = 0;
@@ -814,9 +1030,11 @@ BatchPlan plan_batch(const pbrt_bmfr_batch_frame* frames, uint32_t count, char* 
         e.im.workspace_bytes = 0;
         e.g.features = nullptr;
         e.g.weights = workspace ? reinterpret_cast<float*>(workspace + off) : nullptr;
+        off = align256(off + weight_bytes(e.g));
+        e.g.noise = workspace ? reinterpret_cast<float*>(workspace + off) : nullptr;
+        off = align256(off + noise_bytes(e.g));
         e.first_block = (uint32_t)blocks;
         blocks += (uint64_t)e.g.blocks_x * e.g.blocks_y;
-        off = align256(off + weight_bytes(e.g));
     }
     PBRT_REQUIRE(blocks < (1ull << 31), "pbrt_bmfr_batched: too many blocks for one launch");
     plan.blocks = (uint32_t)blocks;
@@ -883,6 +1101,8 @@ extern "C" int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32
         const BmfrBatchFrame* table = static_cast<const BmfrBatchFrame*>(workspace);
         const dim3 grid(plan.blocks);
         const bool f64 = plan.fit == PBRT_BMFR_FIT_GRAM_F64;
+        const int entries = kFeatures * plan.bs * plan.bs;
+        hipLaunchKernelGGL(k_bmfr_noise_batched, dim3((entries + 255) / 256, count), dim3(256), 0, st, table);
         if (plan.bs == 32) {
             if (f64)
                 hipLaunchKernelGGL((k_bmfr_fit_gram_batched<1024, true>), grid, dim3(kFitThreads), 0, st, table, count);

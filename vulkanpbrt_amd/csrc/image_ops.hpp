@@ -87,7 +87,10 @@ __device__ __forceinline__ void store_bgra8(uint8_t* p, size_t i, vec4 v) {
 // Spherical normal encoding of the G-buffer (ptRaygen.rgen:54-56) decoded as in
 // accumulator.comp:40 / bmfrPre.comp:255-257.
 __device__ __forceinline__ vec3 decode_normal(float theta, float phi) {
-    return make_vec3(cosf(phi) * sinf(theta), sinf(phi) * sinf(theta), cosf(theta));
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);  // one range reduction for both
+    sincosf(phi, &sp, &cp);
+    return make_vec3(cp * st, sp * st, ct);
 }
 
 }  // namespace pbrt
