@@ -188,7 +188,8 @@ def cpu_baseline(gen, desc, cfg, budget_s, frames):
             break
     return dict(value=round(total_rays / total_t / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
                 sample=f"{px / (w * h):.2f} frames' worth of {w}x{h} pixels in row bands of camera frames "
-                       f"{min(done)}..{max(done)} of the timed run, FP32 oracle restatement with FP64 BVH queries, "
+                       f"{min(done)}..{max(done)} of the timed run, FP32 oracle restatement (the device's FP32 ray/triangle "
+                       f"test over the oracle's median-split BVH), "
                        f"{total_rays / 1e6:.1f} M rays in {total_t:.1f} s")
 
 

@@ -30,6 +30,7 @@ def lib():
             build()
         L = C.CDLL(str(LIB_PATH))
         L.orc_rng_sequence.argtypes = [C.c_uint32] * 4 + [C.c_void_p]
+        L.orc_fn_probe.argtypes = [C.c_int32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_wang_hash.argtypes = [C.c_uint32]
         L.orc_wang_hash.restype = C.c_uint32
         L.orc_scene_create.argtypes = [C.c_void_p]
@@ -73,6 +74,19 @@ def _p(a):
 def rng_sequence(width: int, height: int, frame: int, count: int) -> np.ndarray:
     out = np.zeros((height, width, count), dtype=np.uint32)
     lib().orc_rng_sequence(width, height, frame, count, _p(out))
+    return out
+
+
+FN_KINDS = {"log2": 0, "exp2": 1, "pow": 2, "pow5": 3, "sin": 4, "cos": 5}
+
+
+def fn_probe(kind: str, x, y=None) -> np.ndarray:
+    """The project's transcendental `kind` (pbrt_fn.hpp, shared by the device and the
+    FP32 mirror) evaluated on float32 arrays."""
+    x = np.ascontiguousarray(x, np.float32)
+    yy = np.ascontiguousarray(y if y is not None else np.zeros_like(x), np.float32)
+    out = np.zeros_like(x)
+    lib().orc_fn_probe(FN_KINDS[kind], x.size, _p(x), _p(yy), _p(out))
     return out
 
 

@@ -47,7 +47,7 @@ template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
     return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
 template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
-template <class T> inline V3<T> normalize(V3<T> a) { return a / std::sqrt(dot(a, a)); }
+template <class T> inline V3<T> normalize(V3<T> a) { return a * (T(1) / std::sqrt(dot(a, a))); }
 // GLSL min/max/clamp: NaN-suppressing (fmin/fmax), as the GPU build uses.
 template <class T> inline T gmin(T a, T b) { return std::fmin(a, b); }
 template <class T> inline T gmax(T a, T b) { return std::fmax(a, b); }

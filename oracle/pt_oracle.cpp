@@ -3,10 +3,15 @@
 // Restates, per pixel, shaders/ptRaygen.rgen:24-104 with ptClosesthit.rchit,
 // ptAlphaHit.rahit, ptMiss.rmiss, shadow.rmiss, lighting.glsl, sampling.glsl,
 // brdf.glsl, color.glsl, geometry.glsl, camera.glsl and random.glsl, plus the
-// scene packing rules of source/scene/RayTracingVisitor.cpp.  Ray/triangle
-// queries are answered in FP64 (Moller-Trumbore on world-space triangles
-// transformed in double), either by brute force over every triangle or through
-// a median-split BVH that returns the same nearest hit.
+// scene packing rules of source/scene/RayTracingVisitor.cpp.  Two variants:
+//   FP64: ray/triangle queries in FP64 (Moller-Trumbore on world-space triangles
+//         transformed in double), transcendentals from libm;
+//   FP32 mirror: the device's operation order -- FP32 Moller-Trumbore on the
+//         device's triangle records (float vertex and edges), and the project's
+//         transcendental polynomials (pbrt_fn.hpp) -- so a correct tracer matches
+//         it bit for bit.
+// Either answers queries by brute force over every triangle or through a
+// median-split BVH that returns the same nearest hit.
 //
 // Semantics fixed where the reference is undefined (shared with the product):
 //   * a path ends when indirectLighting returns without tracing or the traced
@@ -16,6 +21,7 @@
 //     FP32 weights, REPEAT addressing, mip 0.
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -24,6 +30,7 @@
 #include <vector>
 
 #include "../include/pbrt_c.h"
+#include "../vulkanpbrt_amd/csrc/pbrt_fn.hpp"
 #include "oracle_common.hpp"
 
 namespace orc {
@@ -102,6 +109,9 @@ struct SceneO {
     float strength_sum = 0;
     // world triangles (double), instance/primitive per triangle
     std::vector<double> wt;  // 9 per triangle
+    // the device's triangle records: v0 and the edges v1 - v0, v2 - v0 in float,
+    // from the float world-space vertices (9 per triangle), for the FP32 mirror
+    std::vector<float> wtf;
     std::vector<uint32_t> tri_inst, tri_prim;
     // BVH (median split) over world triangles
     struct Node {
@@ -337,6 +347,16 @@ SceneO* pack(const pbrt_scene_desc& d) {
         l.strengths[3] = sum;
     }
     s->strength_sum = sum;
+    s->wtf.resize(s->wt.size());
+    for (size_t k = 0; k < s->wt.size() / 9; ++k) {
+        float v[9];
+        for (int i = 0; i < 9; ++i) v[i] = (float)s->wt[9 * k + i];
+        for (int a = 0; a < 3; ++a) {
+            s->wtf[9 * k + a] = v[a];
+            s->wtf[9 * k + 3 + a] = v[3 + a] - v[a];
+            s->wtf[9 * k + 6 + a] = v[6 + a] - v[a];
+        }
+    }
     build_bvh(*s);
     return s.release();
 }
@@ -439,6 +459,76 @@ static HitO query(const SceneO& s, const double* o, const double* d, double tmin
     return best;
 }
 
+// The FP32 mirror's ray queries: Moller-Trumbore in float on the device's triangle
+// records with the device's operation order (traverse.hpp test_leaf); closest hit
+// = smallest t in [tmin, tmax], the lower triangle index (instance, then
+// primitive) at equal t, and nothing accepted at exactly tmax.  Any-hit: the
+// first accepted t in [tmin, tmax).  The BVH only narrows the candidates (its
+// boxes are widened), so the result equals the brute-force one.
+static bool tri_test_f32(const SceneO& s, uint32_t k, const float* o, const float* d, float& t, float& u, float& v) {
+    const float* p = &s.wtf[9 * (size_t)k];
+    const V3<float> v0(p[0], p[1], p[2]), e1(p[3], p[4], p[5]), e2(p[6], p[7], p[8]);
+    const V3<float> D(d[0], d[1], d[2]), O(o[0], o[1], o[2]);
+    const V3<float> pv = cross(D, e2);
+    const float det = dot(e1, pv);
+    if (det == 0.0f) return false;
+    const float inv = 1.0f / det;
+    const V3<float> tv = O - v0;
+    u = dot(tv, pv) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    const V3<float> qv = cross(tv, e1);
+    v = dot(D, qv) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    t = dot(e2, qv) * inv;
+    return true;
+}
+
+static HitO query_f32(const SceneO& s, const float* o, const float* d, float tmin, float tmax, bool any_hit,
+                      bool brute) {
+    HitO best;
+    float tbest = tmax;
+    auto consider = [&](uint32_t k) {
+        float t, u, v;
+        if (!tri_test_f32(s, k, o, d, t, u, v)) return false;
+        if (any_hit) {
+            if (!(t >= tmin && t < tbest)) return false;
+        } else {
+            if (!(t >= tmin && t <= tbest)) return false;
+            if (t == tbest && (best.tri < 0 || (int32_t)k > best.tri)) return false;
+        }
+        if (alpha_rejects(s, k, u, v)) return false;
+        best.tri = (int32_t)k;
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        tbest = t;
+        return true;
+    };
+    if (brute) {
+        for (uint32_t k = 0; k < s.tri_inst.size(); ++k)
+            if (consider(k) && any_hit) return best;
+        return best;
+    }
+    const double od[3] = {o[0], o[1], o[2]};
+    const double inv[3] = {1.0 / d[0], 1.0 / d[1], 1.0 / d[2]};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const SceneO::Node& n = s.nodes[stack[--sp]];
+        // boxes widened well beyond float rounding of the hits they may hold
+        if (!box_test(n, od, inv, (double)tmin - 1e-3, (double)tbest * (1 + 1e-4) + 1e-4)) continue;
+        if (n.left < 0) {
+            for (uint32_t i = n.first; i < n.first + n.count; ++i)
+                if (consider(s.order[i]) && any_hit) return best;
+        } else {
+            stack[sp++] = (uint32_t)n.right;
+            stack[sp++] = (uint32_t)n.left;
+        }
+    }
+    return best;
+}
+
 // ---------------------------------------------------------------------------
 // Textures (sampler LINEAR / REPEAT / mip 0)
 // ---------------------------------------------------------------------------
@@ -493,7 +583,27 @@ template <class T> struct K {
 };
 
 template <class T> inline T luminance(V3<T> c) { return dot(c, V3<T>(T(0.2126), T(0.7152), T(0.0722))); }
-template <class T> inline V3<T> srgb2lin(V3<T> c) { return vpow(c, T(2.2)); }
+// Transcendentals: the FP32 mirror evaluates the project's fixed polynomials
+// (pbrt_fn.hpp, the ones the device evaluates), the FP64 variant libm.
+template <class T> inline T o_pow(T x, T y) {
+    if constexpr (std::is_same_v<T, float>) return pbrt_fn::pow(x, y); else return std::pow(x, y);
+}
+template <class T> inline T o_exp2(T x) {
+    if constexpr (std::is_same_v<T, float>) return pbrt_fn::exp2(x); else return std::exp2(x);
+}
+template <class T> inline T o_pow5(T x) {
+    if constexpr (std::is_same_v<T, float>) return pbrt_fn::pow5(x); else return std::pow(x, T(5));
+}
+template <class T> inline void o_sincos(T x, T& s, T& c) {
+    if constexpr (std::is_same_v<T, float>) {
+        pbrt_fn::sincos(x, s, c);
+    } else {
+        s = std::sin(x);
+        c = std::cos(x);
+    }
+}
+template <class T> inline V3<T> vpow_o(V3<T> v, T e) { return {o_pow(v.x, e), o_pow(v.y, e), o_pow(v.z, e)}; }
+template <class T> inline V3<T> srgb2lin(V3<T> c) { return vpow_o(c, T(2.2)); }
 
 // convertMetallic (color.glsl:30-47)
 template <class T>
@@ -651,7 +761,7 @@ V3<T> brdf(V3<T> v, V3<T> l, V3<T> h, const Surface<T>& s) {
     T NdotV = gclamp(std::fabs(dot(n, v)), T(0.001), T(1));
     T NdotH = gclamp(dot(n, h), T(0), T(1));
     T VdotH = gclamp(dot(v, h), T(0), T(1));
-    V3<T> F = s.r0 + (s.r90 - s.r90 * s.r0) * std::exp2((T(-5.55473) * VdotH - T(6.98316)) * VdotH);
+    V3<T> F = s.r0 + (s.r90 - s.r90 * s.r0) * o_exp2((T(-5.55473) * VdotH - T(6.98316)) * VdotH);
     T r = s.alpha_roughness * s.alpha_roughness;
     T aL = T(2) * NdotL / (NdotL + std::sqrt(r + (T(1) - r) * (NdotL * NdotL)));
     T aV = T(2) * NdotV / (NdotV + std::sqrt(r + (T(1) - r) * (NdotV * NdotV)));
@@ -660,7 +770,7 @@ V3<T> brdf(V3<T> v, V3<T> l, V3<T> h, const Surface<T>& s) {
     T D = r / (K<T>::pi * f * f);
     T Fd90 = T(0.5) + T(2) * s.perceptual_roughness * VdotH * VdotH;
     const T dim = T(1) - T(0.1);  // min over (1 - f0) with f0 = vec3(0.1)
-    T res = ((T(1) + (Fd90 - T(1)) * std::pow(T(1) - NdotL, T(5))) * (T(1) + (Fd90 - T(1)) * std::pow(T(1) - NdotV, T(5)))) *
+    T res = ((T(1) + (Fd90 - T(1)) * o_pow5(T(1) - NdotL)) * (T(1) + (Fd90 - T(1)) * o_pow5(T(1) - NdotV))) *
             K<T>::rpi * dim;
     V3<T> diff = (T(1) - F) * (s.diffuse * res);
     V3<T> spec = F * G * D / (T(4) * NdotL * NdotV);
@@ -683,19 +793,23 @@ V3<T> sample_brdf(const Surface<T>& s, RandomEngine re, V3<T> v, V3<T>& l, T& pd
         T phi = T(2) * K<T>::pi * ux;
         T cosH = std::sqrt((T(1) - uy) / (T(1) + (a2 - T(1)) * uy));
         T sinH = std::sqrt(T(1) - cosH * cosH);
-        h = mul(V3<T>(sinH * std::cos(phi), sinH * std::sin(phi), cosH));
+        T sp, cp;
+        o_sincos(phi, sp, cp);
+        h = mul(V3<T>(sinH * cp, sinH * sp, cosH));
         l = -reflect(v, h);
     } else {
         T t = T(2) * K<T>::pi * uy;
         T sr = std::sqrt(ux);
-        T dx = sr * std::cos(t), dy = sr * std::sin(t);
+        T st, ct;
+        o_sincos(t, st, ct);
+        T dx = sr * ct, dy = sr * st;
         l = mul(V3<T>(dx, dy, std::sqrt(gmax(T(0), T(1) - dx * dx - dy * dy))));
         h = normalize(v + l);
     }
     pdf = pdf_brdf(s, l, h);
     V3<T> b = brdf(v, l, h, s);
     if (s.illum == 7) {
-        T f = T(1) + (T(0) - T(1)) * std::pow(gclamp(T(1) - dot(v, h), T(0), T(1)), T(5));
+        T f = T(1) + (T(0) - T(1)) * o_pow5(gclamp(T(1) - dot(v, h), T(0), T(1)));
         if ((T)random_float(re) < f || !entering) {
             T t = !entering ? s.ior : T(1) / s.ior;
             T cosI = dot(basis[2], l);
@@ -863,8 +977,13 @@ struct Stats {
 
 template <class T>
 HitO trace_ray(const SceneO& sc, V3<T> o, V3<T> d, T tmin, T tmax, bool any, bool brute) {
-    double od[3] = {(double)o.x, (double)o.y, (double)o.z}, dd[3] = {(double)d.x, (double)d.y, (double)d.z};
-    return query(sc, od, dd, (double)tmin, (double)tmax, any, brute);
+    if constexpr (std::is_same_v<T, float>) {
+        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        return query_f32(sc, of, df, tmin, tmax, any, brute);
+    } else {
+        double od[3] = {(double)o.x, (double)o.y, (double)o.z}, dd[3] = {(double)d.x, (double)d.y, (double)d.z};
+        return query(sc, od, dd, (double)tmin, (double)tmax, any, brute);
+    }
 }
 
 // nextEventEsitmation (lighting.glsl:246-261)
@@ -988,7 +1107,7 @@ void trace_pixel(const SceneO& sc, const Params& p, uint32_t px, uint32_t py, bo
             T alpha = T(1) / T(p.sample + 1);
             fc = vmix(srgb2lin(V3<T>(T(pr[0]), T(pr[1]), T(pr[2]))), fc, alpha);
         }
-        fc = vclamp(vpow(fc, T(1) / T(2.2)), T(0), T(1));
+        fc = vclamp(vpow_o(fc, T(1) / T(2.2)), T(0), T(1));
         out.illum[0] = (float)fc.x;
         out.illum[1] = (float)fc.y;
         out.illum[2] = (float)fc.z;
@@ -1012,6 +1131,22 @@ void orc_rng_sequence(uint32_t w, uint32_t h, uint32_t frame, uint32_t count, ui
 }
 
 uint32_t orc_wang_hash(uint32_t s) { return wang_hash(s); }
+
+// The project's transcendentals (pbrt_fn.hpp) on arrays, for their pins:
+// kind 0 log2(x), 1 exp2(x), 2 pow(x, y), 3 pow5(x), 4 sin(x), 5 cos(x).
+void orc_fn_probe(int32_t kind, uint32_t n, const float* x, const float* y, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float s, c;
+        switch (kind) {
+            case 0: out[i] = pbrt_fn::log2(x[i]); break;
+            case 1: out[i] = pbrt_fn::exp2(x[i]); break;
+            case 2: out[i] = pbrt_fn::pow(x[i], y[i]); break;
+            case 3: out[i] = pbrt_fn::pow5(x[i]); break;
+            case 4: pbrt_fn::sincos(x[i], s, c); out[i] = s; break;
+            default: pbrt_fn::sincos(x[i], s, c); out[i] = c; break;
+        }
+    }
+}
 
 void* orc_scene_create(const pbrt_scene_desc* d) { return pack(*d); }
 void orc_scene_destroy(void* s) { delete (SceneO*)s; }

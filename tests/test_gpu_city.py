@@ -49,12 +49,14 @@ def test_city_matches_fp32_mirror(cuda, city, tracer):
     gi, ri = gpu["illumination"][..., :3], ref["illumination"][..., :3]
     rel = np.abs(gi - ri) / np.maximum(np.maximum(np.abs(gi), np.abs(ri)), 1e-3)
     rad_ok = np.all(rel <= 1e-4, axis=-1)
+    exact = np.all(gi == ri, axis=-1)
     c = cnt.cpu().numpy()
-    print(f"city tracer {tracer}: depth {d_ok.mean():.4f} albedo {alb_ok.mean():.4f} radiance {rad_ok.mean():.4f}; "
+    print(f"city tracer {tracer}: depth {d_ok.mean():.4f} albedo {alb_ok.mean():.4f} radiance {rad_ok.mean():.4f} "
+          f"(bit-identical {exact.mean():.4f}); "
           f"rays gpu {c[0]} + {c[1]} shadow, oracle {ref['rays']} + {ref['shadow_rays']}; "
           f"mean radiance gpu {gi[ri < 999].mean():.5f} oracle {ri[ri < 999].mean():.5f}")
     assert d_ok.mean() >= 0.999 and alb_ok.mean() >= 0.999
-    assert rad_ok.mean() >= 0.99
+    assert rad_ok.mean() >= 0.999 and exact.mean() >= 0.999
     assert abs(int(c[0]) - ref["rays"]) <= 0.01 * ref["rays"]
     assert abs(int(c[1]) - ref["shadow_rays"]) <= 0.01 * ref["shadow_rays"]
 

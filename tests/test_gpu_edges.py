@@ -16,7 +16,7 @@ import pytest
 from vulkanpbrt_amd import camera, capi
 from tests import helpers
 from tests.test_gpu_post import acc_consts, half, half_key, unhalf
-from tests.test_gpu_trace import _compare_frame
+from tests.test_gpu_trace import _assert_mirror_exact, _compare_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -43,7 +43,7 @@ def test_tracers_on_ragged_frames(cuda, cfg1, tracer, w, h):
     if w * h == 1:
         assert frac["depth"] == 1.0 and frac["albedo"] == 1.0
     else:
-        assert frac["depth"] >= 0.999 and frac["albedo"] >= 0.999 and frac["radiance"] >= 0.99
+        _assert_mirror_exact(frac)
 
 
 def _frames(osc, w, h):

@@ -31,6 +31,7 @@
 
 #include "capi_common.hpp"
 #include "image_ops.hpp"
+#include "pbrt_fn.hpp"
 
 namespace pbrt {
 
@@ -48,35 +49,12 @@ struct BmfrGeom {
                          // the fit's load order: entry c * bs^2 + t * kSub + s for thread t's s-th pixel
 };
 
-// sin and cos of one argument: Cody-Waite reduction by pi/2 (pi/2 split into three
-// floats, the first two with 12 significant bits so k * part is exact for |k| < 2^12),
-// then polynomials in r^2 on [-pi/4, pi/4] fitted by least squares on Chebyshev
-// nodes (within 1.5 fp32 ulp of the correctly rounded values for |x| <= 4; the
-// G-buffer's angles lie in [-pi, pi]).  Far cheaper than the general sincosf, whose
-// large-argument reduction the straight-line code would always execute.
-__device__ __forceinline__ void sincos_small(float x, float& sn, float& cs) {
-    const float k = rintf(x * 0.636619772f);
-    float r = fmaf(-k, 1.5703125f, x);
-    r = fmaf(-k, 4.83751297e-4f, r);
-    r = fmaf(-k, 7.54979013e-8f, r);
-    const float r2 = r * r;
-    const float ps = fmaf(fmaf(-1.95202127e-4f, r2, 8.33220407e-3f), r2, -0.166666552f);
-    const float s = fmaf(r * r2, ps, r);
-    const float pc = fmaf(fmaf(fmaf(2.43941031e-5f, r2, -1.38868089e-3f), r2, 4.16666232e-2f), r2, -0.5f);
-    const float c = fmaf(r2, pc, 1.0f);
-    const int q = (int)k & 3;
-    sn = (q & 1) ? c : s;
-    cs = (q & 1) ? s : c;
-    if (q == 1 || q == 2) cs = -cs;
-    if (q >= 2) sn = -sn;
-}
-
 // Spherical G-buffer normal (theta, phi) as (cos phi sin theta, sin phi sin theta,
 // cos theta), bmfrPre.comp:26-29.
 __device__ __forceinline__ vec3 bmfr_decode_normal(float theta, float phi) {
     float st, ct, sp, cp;
-    sincos_small(theta, st, ct);
-    sincos_small(phi, sp, cp);
+    pbrt_fn::sincos(theta, st, ct);  // bounded-argument polynomial: the angles lie in [-pi, pi]
+    pbrt_fn::sincos(phi, sp, cp);
     return make_vec3(cp * st, sp * st, ct);
 }
 

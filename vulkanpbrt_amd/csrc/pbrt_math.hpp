@@ -59,7 +59,7 @@ PBRT_HD vec3 cross(vec3 a, vec3 b) {
     return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
 PBRT_HD float length(vec3 a) { return sqrtf(dot(a, a)); }
-PBRT_HD vec3 normalize(vec3 a) { return a / sqrtf(dot(a, a)); }
+PBRT_HD vec3 normalize(vec3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
 PBRT_HD float distance(vec3 a, vec3 b) { return length(a - b); }
 PBRT_HD float gmin(float a, float b) { return fminf(a, b); }
 PBRT_HD float gmax(float a, float b) { return fmaxf(a, b); }

@@ -4,6 +4,7 @@
 // arithmetic order is kept where the sample stream or parity depends on it.
 #pragma once
 
+#include "pbrt_fn.hpp"
 #include "pbrt_math.hpp"
 #include "scene_types.hpp"
 
@@ -76,7 +77,14 @@ PBRT_HD vec3 rng_vec3(Rng& r) {
 // Colour helpers (color.glsl:4-47)
 // ---------------------------------------------------------------------------
 PBRT_HD float luminance(vec3 c) { return dot(c, make_vec3(0.2126f, 0.7152f, 0.0722f)); }
-PBRT_HD vec3 srgb_to_linear(vec3 c) { return vpow(c, 2.2f); }
+PBRT_HD vec3 srgb_to_linear(vec3 c) {
+    return make_vec3(pbrt_fn::pow(c.x, 2.2f), pbrt_fn::pow(c.y, 2.2f), pbrt_fn::pow(c.z, 2.2f));
+}
+// pow(c, 1/2.2) of the final-image output (ptRaygen.rgen:100)
+PBRT_HD vec3 linear_to_srgb(vec3 c) {
+    const float e = 1.0f / 2.2f;
+    return make_vec3(pbrt_fn::pow(c.x, e), pbrt_fn::pow(c.y, e), pbrt_fn::pow(c.z, e));
+}
 
 PBRT_HD float convert_metallic(vec3 diffuse, vec3 specular, float max_specular) {
     float perceived_diffuse =
@@ -161,7 +169,7 @@ PBRT_HD vec3 brdf_eval(vec3 v, vec3 l, vec3 h, const SurfaceInfo& s) {
     float ndoth = gclamp(dot(n, h), 0.0f, 1.0f);
     float vdoth = gclamp(dot(v, h), 0.0f, 1.0f);
     // specularReflection(PBRInfo): Schlick with the exp2 spherical-gaussian fit
-    float fexp = exp2f((-5.55473f * vdoth - 6.98316f) * vdoth);
+    float fexp = pbrt_fn::exp2((-5.55473f * vdoth - 6.98316f) * vdoth);
     vec3 f = s.reflectance0 + (s.reflectance90 - s.reflectance90 * s.reflectance0) * fexp;
     // geometricOcclusion: r = alphaRoughness^2
     float r = s.alpha_roughness * s.alpha_roughness;
@@ -175,8 +183,8 @@ PBRT_HD vec3 brdf_eval(vec3 v, vec3 l, vec3 h, const SurfaceInfo& s) {
     // BRDF_Diffuse_Disney; f0 = 0.1 => dim = 0.9
     float fd90 = 0.5f + 2.0f * s.perceptual_roughness * vdoth * vdoth;
     const float dim = 0.9f;
-    float result = ((1.0f + (fd90 - 1.0f) * powf(1.0f - ndotl, 5.0f)) *
-                    (1.0f + (fd90 - 1.0f) * powf(1.0f - ndotv, 5.0f))) *
+    float result = ((1.0f + (fd90 - 1.0f) * pbrt_fn::pow5(1.0f - ndotl)) *
+                    (1.0f + (fd90 - 1.0f) * pbrt_fn::pow5(1.0f - ndotv))) *
                    kReciprocalPi * dim;
     vec3 diffuse_contrib = (1.0f - f) * (s.diffuse_color * result);
     vec3 spec_contrib = f * g * d / (4.0f * ndotl * ndotv);
@@ -187,7 +195,7 @@ PBRT_HD vec3 brdf_eval(vec3 v, vec3 l, vec3 h, const SurfaceInfo& s) {
 // specularReflection(vec3, vec3, float) (brdf.glsl:94-98)
 // with reflectance0 = 1, reflectance90 = 0 as sampleBRDF calls it.
 PBRT_HD float fresnel_schlick_scalar(float vdoth) {
-    return 1.0f + (0.0f - 1.0f) * powf(gclamp(1.0f - vdoth, 0.0f, 1.0f), 5.0f);
+    return 1.0f + (0.0f - 1.0f) * pbrt_fn::pow5(gclamp(1.0f - vdoth, 0.0f, 1.0f));
 }
 
 // ---------------------------------------------------------------------------
@@ -197,14 +205,18 @@ PBRT_HD vec3 sample_ggx(vec2 r, float alpha2) {
     float phi = 2.0f * kPi * r.x;
     float cos_h = sqrtf((1.0f - r.y) / (1.0f + (alpha2 - 1.0f) * r.y));
     float sin_h = sqrtf(1.0f - cos_h * cos_h);
-    return {sin_h * cosf(phi), sin_h * sinf(phi), cos_h};
+    float sp, cp;
+    pbrt_fn::sincos(phi, sp, cp);
+    return {sin_h * cp, sin_h * sp, cos_h};
 }
 
 PBRT_HD vec3 sample_hemisphere(vec2 r) {
     float t = 2.0f * kPi * r.y;
     float sr = sqrtf(r.x);
-    float dx = sr * cosf(t);
-    float dy = sr * sinf(t);
+    float st, ct;
+    pbrt_fn::sincos(t, st, ct);
+    float dx = sr * ct;
+    float dy = sr * st;
     return {dx, dy, sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy))};
 }
 

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
             vec3 pl = srgb_to_linear(make_vec3(prev.x, prev.y, prev.z));
             fc = vmix(pl, fc, alpha);
         }
-        fc = vpow(fc, 1.0f / 2.2f);
+        fc = linear_to_srgb(fc);
         fc = vclamp(fc, 0.0f, 1.0f);
         illum_img[pix] = make_float4(fc.x, fc.y, fc.z, 1.0f);
     }

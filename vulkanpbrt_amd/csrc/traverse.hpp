@@ -32,9 +32,18 @@ struct TraversalCounters {
 // Texture fetch: RGBA8 UNORM, LINEAR filter, REPEAT addressing, mip 0.
 // 1x1 textures return their texel exactly (all four taps coincide).
 // ---------------------------------------------------------------------------
+// b / 255 correctly rounded for a byte b: q = b * fl(1/255) corrected once by the
+// exact residual b - 255 q (one fma) equals the IEEE quotient for all 256 bytes
+// (checked exhaustively), for three operations instead of a full division.
+__device__ __forceinline__ float unorm8_value(uint32_t b) {
+    const float c = (float)b, r = 1.0f / 255.0f;
+    const float q = c * r;
+    return fmaf(fmaf(-q, 255.0f, c), r, q);
+}
+
 __device__ __forceinline__ vec4 unpack_rgba8(uint32_t c) {
-    return make_vec4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
-                     (float)((c >> 16) & 0xFFu) / 255.0f, (float)(c >> 24) / 255.0f);
+    return make_vec4(unorm8_value(c & 0xFFu), unorm8_value((c >> 8) & 0xFFu), unorm8_value((c >> 16) & 0xFFu),
+                     unorm8_value(c >> 24));
 }
 
 __device__ __forceinline__ int wrap_repeat(int i, int n) {

@@ -329,7 +329,7 @@ __global__ void k_wf_finalize(TraceConsts c, WfBuffers b, float4* illum_img) {
             float alpha = 1.0f / (float)(c.sample_number + 1);
             fc = vmix(srgb_to_linear(make_vec3(prev.x, prev.y, prev.z)), fc, alpha);
         }
-        fc = vclamp(vpow(fc, 1.0f / 2.2f), 0.0f, 1.0f);
+        fc = vclamp(linear_to_srgb(fc), 0.0f, 1.0f);
         illum_img[path] = make_float4(fc.x, fc.y, fc.z, 1.0f);
     }
 }
