@@ -11,14 +11,14 @@
 // Fits:
 //   k_bmfr_fit_householder   the reference's FP32 Householder QR over the fp16
 //                            feature buffer, subgroup reductions on wave64
-//   k_bmfr_fit_gram<F64>     pre + fit fused: features stay in registers; the
-//                            columns are projected against the noised constant
-//                            column a0 in FP64 and scaled by powers of two; their
-//                            Gram matrix is accumulated on the matrix cores, by
-//                            v_mfma_f64_16x16x4f64 (F64) or by fp16 hi + lo splits
-//                            through v_mfma_f32_16x16x32_f16 (F16, FP32 inside a
-//                            32-row chunk, FP64 across chunks); FP64 Cholesky solves
-//                            for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+//   k_bmfr_fit_gram<F64>     pre + fit fused, features in registers: the Gram matrix
+//                            of the raw columns on v_mfma_f64_16x16x4f64, the
+//                            projection against the noised constant column a0 as a
+//                            rank-1 downdate in FP64, equilibrated FP64 Cholesky
+//   k_bmfr_fit_gram<F16>     the columns projected against a0 in FP64 per row, scaled
+//                            by powers of two, split into fp16 hi + lo parts whose
+//                            Gram v_mfma_f32_16x16x32_f16 accumulates (FP32 inside a
+//                            32-row chunk, FP64 across chunks); FP64 Cholesky
 //   The BMFR features are nearly collinear on planar blocks (depth is almost a
 //   linear function of the block coordinates; only the 1e-4 noise separates
 //   them), so cond(A^T A) reaches ~1e10: an FP32-accumulated Gram loses the fit
@@ -297,27 +297,25 @@ __global__ void __launch_bounds__(256) k_bmfr_fit_householder(BmfrGeom g) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused pre + fit: projected features -> MFMA Gram -> FP64 Cholesky solve
+// Fused pre + fit on the matrix cores: FP64 (raw Gram, below) and FP16 (projected
+// features split into fp16 hi + lo parts)
 // ---------------------------------------------------------------------------
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 typedef double double4v __attribute__((ext_vector_type(4)));
 
-constexpr int kGramCols = 16;  // 9 projected features + 3 projected rgb + 4 zero columns
+constexpr int kGramCols = 16;  // 13 columns (FP64: a0, 9 features, rgb; FP16: 9 projected features, rgb) + zeros
 constexpr int kFitThreads = 256;
 
-// Wave-private staging of 64 rows (one per lane) for the Gram products, the
-// per-wave partial Gram matrices, and the per-wave partial sums of the block
-// reductions (each reduction costs one workgroup barrier).
-template <bool kF64, int kWaves>
+// FP16 fit: wave-private staging of 64 rows (one per lane) as column-major fp16 hi / lo
+// planes, the per-wave partial Gram matrices, and the per-wave partial sums of the
+// block reductions (each reduction costs one workgroup barrier).
+template <int kWaves>
 struct GramShared {
-    union {
-        float f[kWaves][64][kGramCols + 1];  // F64: row-major fp32 rows (odd stride: no bank conflicts)
-        struct {
-            _Float16 hi[kWaves][kGramCols][64];  // F16: column-major hi / lo planes
-            _Float16 lo[kWaves][kGramCols][64];
-        } h;
-    } st;
+    struct {
+        _Float16 hi[kWaves][kGramCols][64];
+        _Float16 lo[kWaves][kGramCols][64];
+    } h;
     double gram[kWaves][kGramCols * kGramCols];
     float depth_min[kWaves], depth_max[kWaves];
     float coord[32];
@@ -341,11 +339,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // (pbrt_autotune_bmfr picks it).  Thread t loads pixels t, t + kThreads, ... of
 // the block in row-major order (x fastest: coalesced); the pixel at block-local
 // (x, y) is fit row i = x * bs + y, whose index keys the noise.
-template <int kRows, bool kF64, int kThreads>
-__device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
+// The FP16 fit: the columns are projected against the noised constant column a0 in
+// FP64, rounded to fp32, scaled by powers of two and split into fp16 hi + lo parts
+// whose Gram matrix v_mfma_f32_16x16x32_f16 accumulates (FP32 inside a 32-row chunk,
+// FP64 across chunks); FP64 Cholesky then solves for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+template <int kRows, int kThreads>
+__device__ __forceinline__ void bmfr_fit_gram_f16_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx,
+                                                        int by) {
     constexpr int kSub = (kRows + kThreads - 1) / kThreads;
     constexpr int kWaves = kThreads / 64;
-    __shared__ GramShared<kF64, kWaves> S;
+    __shared__ GramShared<kWaves> S;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int bs = g.bs;
     const int basex = bx * bs, basey = by * bs;
@@ -485,58 +488,37 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
     {
         const int col = lane & 15, grp = lane >> 4;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        double4v d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < kSub; ++s) {
             if (s > 0) wave_lds_sync();  // the previous chunk has been read
 #pragma unroll
             for (int c = 0; c < kGramCols; ++c) {
                 const float v = (c < 12 && row[s] >= 0) ? a[s][c + 1] * scale[c] : 0.0f;
-                if (kF64) {
-                    S.st.f[wave][lane][c] = v;
-                } else {
-                    const _Float16 hi = (_Float16)v;
-                    S.st.h.hi[wave][c][lane] = hi;
-                    S.st.h.lo[wave][c][lane] = (_Float16)(v - (float)hi);
-                }
+                const _Float16 hi = (_Float16)v;
+                S.h.hi[wave][c][lane] = hi;
+                S.h.lo[wave][c][lane] = (_Float16)(v - (float)hi);
             }
             wave_lds_sync();
-            if (kF64) {
 #pragma unroll
-                for (int ch = 0; ch < 16; ++ch) {
-                    const double x = (double)S.st.f[wave][ch * 4 + grp][col];
-                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, d, 0, 0, 0);
+            for (int ch = 0; ch < 2; ++ch) {
+                const int row0 = ch * 32 + grp * 8;
+                half8 hi, lo;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    hi[j] = S.h.hi[wave][col][row0 + j];
+                    lo[j] = S.h.lo[wave][col][row0 + j];
                 }
-            } else {
+                float4v e = {0.0f, 0.0f, 0.0f, 0.0f};
+                e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, e, 0, 0, 0);
+                e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, e, 0, 0, 0);
+                e = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, e, 0, 0, 0);
 #pragma unroll
-                for (int ch = 0; ch < 2; ++ch) {
-                    const int row0 = ch * 32 + grp * 8;
-                    half8 hi, lo;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        hi[j] = S.st.h.hi[wave][col][row0 + j];
-                        lo[j] = S.st.h.lo[wave][col][row0 + j];
-                    }
-                    float4v e = {0.0f, 0.0f, 0.0f, 0.0f};
-                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, e, 0, 0, 0);
-                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, e, 0, 0, 0);
-                    e = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, e, 0, 0, 0);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[i] += (double)e[i];
-                }
+                for (int i = 0; i < 4; ++i) acc[i] += (double)e[i];
             }
         }
-        if (kF64) {
+        // register i of v_mfma_f32_16x16x32_f16's result is row 4 * grp + i, column `col`
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i] = d[i];
-        }
-        // D layout (lane holds column `col`): register i of v_mfma_f32_16x16x32_f16
-        // is row 4 * grp + i, register i of v_mfma_f64_16x16x4f64 is row grp + 4 * i
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = kF64 ? grp + 4 * i : 4 * grp + i;
-            S.gram[wave][r * kGramCols + col] = acc[i];
-        }
+        for (int i = 0; i < 4; ++i) S.gram[wave][(4 * grp + i) * kGramCols + col] = acc[i];
     }
     __syncthreads();
     // 5. the symmetrised, unscaled 12 x 12 Gram matrix (9 projected features, 3 colours)
@@ -622,10 +604,240 @@ __device__ __forceinline__ void bmfr_fit_gram_block(const BmfrGeom& g, const pbr
 
 // Four waves per SIMD (at most 128 VGPRs): three blocks of 256 threads per CU left
 // the per-block latency (load, reductions, Cholesky) uncovered.
+// The FP64 fit.  The Gram matrix R of the raw columns [a0 | a1..a9 | r g b] (fp32
+// values: their products are exact in FP64) is accumulated on v_mfma_f64_16x16x4f64;
+// the normal equations of the features projected against the noised constant column
+// a0 follow in FP64 by the rank-1 downdate G = R[1:,1:] - R[0,1:] R[0,1:]^T / R[0,0].
+// The 9 x 9 feature block of G is equilibrated by its diagonal, factored by Cholesky
+// (a pivot below 1e-15 drops its feature) and solved for the three colours;
+// w0 = (R[0,10+c] - sum_i w_i R[0,1+i]) / R[0,0].  Against FP64 least squares on the
+// same noised blocks this is more accurate than projecting each row first and
+// rounding the projected columns to fp32 (fitted values within ~1e-8 instead of
+// ~5e-6 on near-collinear planar blocks), and it needs no per-row FP64 work and no
+// block reductions beyond the depth range.
+template <int kWaves>
+struct RawGramShared {
+    float st[kWaves][64][kGramCols + 1];  // wave-private rows (odd stride: no bank conflicts)
+    double gram[kWaves][kGramCols * kGramCols];
+    float depth_min[kWaves], depth_max[kWaves];
+    float coord[32];
+    double r0[kAlphaSize];  // row 0 of R: a0 . a_c
+    double G[12][12];       // projected normal equations (9 features | 3 colours)
+    double Lc[kFeatures - 1][kFeatures - 1];
+    double inv_diag[kFeatures - 1];
+    double eq[kFeatures - 1];  // equilibration 1 / sqrt(G_ii)
+};
+
+template <int kRows, int kThreads>
+__device__ __forceinline__ void bmfr_fit_raw_gram_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx,
+                                                        int by) {
+    constexpr int kSub = (kRows + kThreads - 1) / kThreads;
+    constexpr int kWaves = kThreads / 64;
+    __shared__ RawGramShared<kWaves> S;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bs = g.bs;
+    const int basex = bx * bs, basey = by * bs;
+
+    // 1. load pixels; block min/max of depth
+    float a[kSub][kAlphaSize];
+    float depth[kSub];
+    int row[kSub];
+    float dmin = 3.4e38f, dmax = -3.4e38f;
+#pragma unroll
+    for (int s = 0; s < kSub; ++s) {
+        const int q = tid + s * kThreads;
+        row[s] = -1;
+        if (q < kRows) {
+            const int qx = q % bs, qy = q / bs;
+            row[s] = qx * bs + qy;
+            const PixelIn p = load_pixel(g, im, basex + qx, basey + qy);
+            depth[s] = p.depth;
+            a[s][1] = p.normal.x;
+            a[s][2] = p.normal.y;
+            a[s][3] = p.normal.z;
+            a[s][10] = p.noisy.x;
+            a[s][11] = p.noisy.y;
+            a[s][12] = p.noisy.z;
+            dmin = fminf(dmin, p.depth);
+            dmax = fmaxf(dmax, p.depth);
+        }
+    }
+    dmin = wave_reduce_dpp(dmin, OpMin());
+    dmax = wave_reduce_dpp(dmax, OpMax());
+    if (lane == 0) {
+        S.depth_min[wave] = dmin;
+        S.depth_max[wave] = dmax;
+    }
+    if (tid < bs) S.coord[tid] = (float)tid / (float)(bs - 1);  // bmfrPre.comp:36-43
+    __syncthreads();
+    float mn = S.depth_min[0], mx = S.depth_max[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+        mn = fminf(mn, S.depth_min[w]);
+        mx = fmaxf(mx, S.depth_max[w]);
+    }
+    // 2. features, fp16-quantised (the reference's R16F feature buffer), + noise; the
+    //    noise of feature c for this thread's pixels is one vector load
+#pragma unroll
+    for (int s = 0; s < kSub; ++s) {
+        const int r = row[s] < 0 ? 0 : row[s];
+        const float z = (depth[s] - mn) / (mx - mn + 1e-6f);
+        const float fx = S.coord[r / bs], fy = S.coord[r % bs];
+        a[s][0] = 1.0f;
+        a[s][4] = fx;
+        a[s][5] = fy;
+        a[s][6] = z;
+        a[s][7] = fx * fx;
+        a[s][8] = fy * fy;
+        a[s][9] = z * z;
+    }
+#pragma unroll
+    for (int c = 0; c < kAlphaSize; ++c) {
+        float nz[kSub];
+        if (c < kFeatures && tid * kSub < kRows) {
+            const float* src = g.noise + c * kRows + tid * kSub;
+            if constexpr (kSub == 4) {
+                const float4 v = *reinterpret_cast<const float4*>(src);
+                nz[0] = v.x;
+                nz[1] = v.y;
+                nz[2] = v.z;
+                nz[3] = v.w;
+            } else {
+#pragma unroll
+                for (int s = 0; s < kSub; ++s) nz[s] = src[s];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kSub; ++s) {
+            float v = h2f(f2h(a[s][c]));
+            if (c < kFeatures) v = __fadd_rn(v, nz[s]);
+            a[s][c] = v;
+        }
+    }
+    // 3. R = A^T A on the matrix cores, 64 rows at a time through the wave's own LDS
+    //    slice (lane l stages its row of chunk s); both operands of the product hold
+    //    the same values (lane l: column l % 16 of a group of four rows)
+    {
+        const int col = lane & 15, grp = lane >> 4;
+        // two accumulators: even and odd row groups form independent MFMA chains
+        double4v d = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < kSub; ++s) {
+            if (s > 0) wave_lds_sync();  // the previous chunk has been read
+#pragma unroll
+            for (int c = 0; c < kGramCols; ++c) S.st[wave][lane][c] = (c < kAlphaSize && row[s] >= 0) ? a[s][c] : 0.0f;
+            wave_lds_sync();
+#pragma unroll
+            for (int ch = 0; ch < 16; ch += 2) {
+                const double x0 = (double)S.st[wave][ch * 4 + grp][col];
+                const double x1 = (double)S.st[wave][ch * 4 + 4 + grp][col];
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, x0, d, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, x1, d1, 0, 0, 0);
+            }
+        }
+        d += d1;
+        // register i of v_mfma_f64_16x16x4f64's result is row grp + 4 * i, column `col`
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S.gram[wave][(grp + 4 * i) * kGramCols + col] = d[i];
+    }
+    __syncthreads();
+    // 4. R summed over the waves and symmetrised; G by the rank-1 downdate
+    constexpr int pw = kGramCols * kGramCols;
+    const double* gr = &S.gram[0][0];
+    auto rsum = [&](int i, int j) {
+        double a_ij = 0.0, a_ji = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            a_ij += gr[w * pw + i * kGramCols + j];
+            a_ji += gr[w * pw + j * kGramCols + i];
+        }
+        return 0.5 * (a_ij + a_ji);
+    };
+    const double r00 = rsum(0, 0), inv_r00 = 1.0 / r00;
+    for (int e = tid; e < 12 * 12 + kAlphaSize; e += kThreads) {
+        if (e < 12 * 12) {
+            const int i = e / 12 + 1, j = e % 12 + 1;
+            S.G[i - 1][j - 1] = rsum(i, j) - rsum(0, i) * rsum(0, j) * inv_r00;
+        } else {
+            S.r0[e - 12 * 12] = rsum(0, e - 12 * 12);
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    // 5. Cholesky of the equilibrated 9 x 9 feature block in wave 0, lane i holding row i
+    constexpr int n = kFeatures - 1;
+    const int li = lane < n ? lane : 0;
+    double eqi = S.G[li][li] > 0.0 ? 1.0 / sqrt(S.G[li][li]) : 0.0;
+    if (lane < n) S.eq[lane] = eqi;
+    wave_lds_sync();
+    double grow[n], lrow[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+        grow[k] = S.G[li][k] * eqi * S.eq[k];
+        lrow[k] = 0.0;
+    }
+    double maxdiag = 0.0;
+#pragma unroll
+    for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, S.G[i][i] * S.eq[i] * S.eq[i]);
+#pragma unroll
+    for (int j = 0; j < n; ++j) {
+        double dj = grow[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= lrow[k] * lrow[k];
+        dj = read_lane(dj, j);
+        const bool kp = dj > 1e-15 * maxdiag && dj > 0.0;
+        const double ljj = kp ? sqrt(dj) : 1.0;
+        const double inv = 1.0 / ljj;
+        double v = grow[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) v -= lrow[k] * read_lane(lrow[k], j);
+        lrow[j] = lane == j ? ljj : (lane > j && kp ? v * inv : 0.0);
+        if (lane == 0) S.inv_diag[j] = kp ? inv : 0.0;
+    }
+    if (lane < n) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) S.Lc[lane][k] = lrow[k];
+    }
+    wave_lds_sync();
+    // 6. one lane per colour: forward and back substitution, unequilibrate, w0
+    if (lane < 3) {
+        const int c = lane;
+        float* out = g.weights + ((size_t)by * g.blocks_x + bx) * 30;
+        double z[n], w[n];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            double v = S.G[i][9 + c] * S.eq[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) v -= S.Lc[i][k] * z[k];
+            z[i] = v * S.inv_diag[i];
+        }
+#pragma unroll
+        for (int i = n - 1; i >= 0; --i) {
+            double v = z[i];
+#pragma unroll
+            for (int k = i + 1; k < n; ++k) v -= S.Lc[k][i] * w[k];
+            w[i] = v * S.inv_diag[i];
+        }
+        double w0 = S.r0[10 + c];
+#pragma unroll
+        for (int i = 0; i < n; ++i) {
+            w[i] *= S.eq[i];
+            w0 -= w[i] * S.r0[i + 1];
+        }
+        w0 /= r00;
+        out[c] = (float)w0;
+#pragma unroll
+        for (int i = 0; i < n; ++i) out[(i + 1) * 3 + c] = (float)w[i];
+    }
+}
+
 template <int kRows, bool kF64, int kThreads>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bmfr_fit_gram(BmfrGeom g,
                                                                                                   pbrt_bmfr_images im) {
-    bmfr_fit_gram_block<kRows, kF64, kThreads>(g, im, blockIdx.x, blockIdx.y);
+    if constexpr (kF64)
+        bmfr_fit_raw_gram_block<kRows, kThreads>(g, im, blockIdx.x, blockIdx.y);
+    else
+        bmfr_fit_gram_f16_block<kRows, kThreads>(g, im, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -727,8 +939,11 @@ k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
     const BmfrGeom g = frames[f].g;
     const pbrt_bmfr_images im = frames[f].im;
     const uint32_t local = blockIdx.x - frames[f].first_block;
-    bmfr_fit_gram_block<kRows, kF64, kFitThreads>(g, im, (int)(local % (uint32_t)g.blocks_x),
-                                                  (int)(local / (uint32_t)g.blocks_x));
+    const int bx = (int)(local % (uint32_t)g.blocks_x), by = (int)(local / (uint32_t)g.blocks_x);
+    if constexpr (kF64)
+        bmfr_fit_raw_gram_block<kRows, kFitThreads>(g, im, bx, by);
+    else
+        bmfr_fit_gram_f16_block<kRows, kFitThreads>(g, im, bx, by);
 }
 
 // the noise table of every frame of a batch (blockIdx.y: frame)
