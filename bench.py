@@ -51,6 +51,13 @@ def parse():
                          "or the fused 2-layer one")
     ap.add_argument("--no-denoise-eval", action="store_true",
                     help="skip the learned-vs-BMFR relMSE evaluation on held-out Cornell-box scenes")
+    ap.add_argument("--denoise-quality", action="store_true",
+                    help="instead of timing: relMSE of BMFR (with its temporal history) and of the learned "
+                         "denoisers over the workload's camera sequence at its full size, against references "
+                         "of --quality-spp samples per pixel; prints one JSON line")
+    ap.add_argument("--quality-frames", type=int, default=16, help="frames of the --denoise-quality sequence")
+    ap.add_argument("--quality-score", default="0,15", help="frames scored by --denoise-quality (comma list)")
+    ap.add_argument("--quality-spp", type=int, default=4096, help="samples per pixel of the references")
     ap.add_argument("--graph", dest="graph", action="store_true", default=None,
                     help="capture each frame (trace, accumulate, denoise, TAA) into one hipGraph and launch it "
                          "(default for the city; the per-kernel breakdown then comes from an eager pass over "
@@ -234,6 +241,21 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     tuning = fr.autotune() if args.autotune else None
+    if args.denoise_quality:
+        from vulkanpbrt_amd import denoiser_train as dt
+        score = sorted({int(s) for s in args.quality_score.split(",")})
+        t0 = time.perf_counter()
+        q = dt.sequence_quality(torch, fr, args.quality_frames, score, args.quality_spp, progress=True)
+        print(json.dumps({"denoise_quality": {
+            "workload": cfg["name"], "frames": args.quality_frames, "scored_frames": score,
+            "reference_spp": args.quality_spp, "metric": "relMSE of the remodulated radiance "
+            "(albedo x demodulated), mean over pixels and rgb of (x - ref)^2 / (ref^2 + 0.01); "
+            "*_demodulated: of the demodulated radiance",
+            "networks": "6-layer and fused 2-layer kernel-predicting networks, committed parameters "
+                        "trained on Cornell-box scenes", "seconds": round(time.perf_counter() - t0, 1),
+            "mean": {n: round(v, 5) for n, v in q["mean"].items()},
+            "per_frame": {str(f): {n: round(v, 5) for n, v in row.items()} for f, row in q["frames"].items()}}}))
+        return
     use_graph = cfg.get("graph", False) if args.graph is None else args.graph
     graph = None
     if use_graph:

@@ -47,6 +47,15 @@ def unit(spherical):
     return np.stack([np.cos(p) * np.sin(t), np.sin(p) * np.sin(t), np.cos(t)], -1)
 
 
+def same_surface(r64, material, albedo, normal):
+    """Pixels whose primary hit lies on the FP64 oracle's surface: same material, normal
+    within 0.01 rad, G-buffer albedo within 4 of 255 (a texel-filtered albedo rounds
+    differently when the hit's texture coordinates differ in the last bits)."""
+    da = np.abs(r64["albedo"].astype(np.int32) - np.asarray(albedo).astype(np.int32)).max(-1)
+    return (np.all(r64["material"] == material, -1) & (da <= 4) &
+            (np.sum(unit(r64["normal"]) * unit(normal), -1) > np.cos(0.01)))
+
+
 def relmse(a, b, eps=1e-2):
     return float(np.mean((a - b) ** 2 / (b * b + eps)))
 
@@ -71,11 +80,10 @@ def main():
             q64 += c64 * c64
             s32 += r32["illumination"][..., :3]
             if f == 0:
-                same = np.all(r64["albedo"] == r32["albedo"], -1) & (
-                    np.sum(unit(r64["normal"]) * unit(r32["normal"]), -1) > 0.9999)
                 out[f"{name}_albedo"] = r64["albedo"]
                 out[f"{name}_normal"] = r64["normal"]
-                out[f"{name}_same_primary"] = same
+                out[f"{name}_material"] = r64["material"]
+                out[f"{name}_same_primary"] = same_surface(r64, r32["material"], r32["albedo"], r32["normal"])
         m64, m32 = s64 / SAMPLES, s32 / SAMPLES
         out[f"{name}_mean64"] = m64.astype(np.float32)
         out[f"{name}_m2_64"] = (q64 / SAMPLES).astype(np.float32)

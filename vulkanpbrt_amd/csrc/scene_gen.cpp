@@ -401,6 +401,11 @@ struct pbrt_generated_scene {
 
 namespace {
 
+float catmull_rom(float p0, float p1, float p2, float p3, float t) {
+    float t2 = t * t, t3 = t2 * t;
+    return 0.5f * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t2 + (-p0 + 3 * p1 - 3 * p2 + p3) * t3);
+}
+
 void add_instance(pbrt_generated_scene& g, uint32_t mesh, const float* t) {
     pbrt_instance_desc in{};
     std::memcpy(in.transform, t, sizeof(in.transform));
@@ -495,6 +500,27 @@ void build_cornell(pbrt_generated_scene& g, uint64_t seed) {
         add_instance(g, (uint32_t)g.meshes.size() - 1, t);
     }
     g.camera = pbrt_camera_pose{{0.0f, -3.0f, 1.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 0.0f, 1.0f}, 60.0f, 0.1f, 1000.0f};
+}
+
+// the atrium's camera path: a closed Catmull-Rom loop through 8 keys (eye, look-at)
+// at eye height in the gallery between the colonnade and the walls
+constexpr float kAtriumKeys[8][6] = {{-14, -14, 1.7f, 0, 0, 3}, {-4, -16, 2.0f, 2, 0, 4}, {10, -13, 1.8f, 0, 2, 3},
+                                     {15, -2, 2.5f, -2, 2, 5},  {12, 12, 1.7f, 0, 0, 4},  {0, 15, 2.2f, 0, -2, 3},
+                                     {-13, 10, 1.9f, 2, -2, 6}, {-16, -4, 1.7f, 2, 0, 3}};
+
+// the eye's ground-plane track, `n` points per key interval
+std::vector<std::pair<float, float>> atrium_eye_track(int n) {
+    std::vector<std::pair<float, float>> out;
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < n; ++j) {
+            const float* k0 = kAtriumKeys[(i + 7) % 8];
+            const float* k1 = kAtriumKeys[i];
+            const float* k2 = kAtriumKeys[(i + 1) % 8];
+            const float* k3 = kAtriumKeys[(i + 2) % 8];
+            const float t = (float)j / (float)n;
+            out.push_back({catmull_rom(k0[0], k1[0], k2[0], k3[0], t), catmull_rom(k0[1], k1[1], k2[1], k3[1], t)});
+        }
+    return out;
 }
 
 void build_atrium(pbrt_generated_scene& g, uint64_t seed) {
@@ -623,6 +649,7 @@ void build_atrium(pbrt_generated_scene& g, uint64_t seed) {
     }
     // 200 boxes with log-uniform scale in [0.1, 10]
     {
+        const auto track = atrium_eye_track(64);
         uint32_t box_mesh[4];
         for (int k = 0; k < 4; ++k) {
             Mesh m;
@@ -641,9 +668,26 @@ void build_atrium(pbrt_generated_scene& g, uint64_t seed) {
             sx = std::fmin(sx, 6.0f);
             sy = std::fmin(sy, 6.0f);
             sz = std::fmin(sz, 4.0f);
-            float x = rng.uniform(-18.0f, 18.0f), y = rng.uniform(-18.0f, 18.0f);
+            // positions whose bounding circle comes within 1.5 of the camera's track are
+            // drawn again (a box without such a position in 256 draws is left out), so the
+            // camera never enters or grazes a box
+            const float clear = 0.5f * std::sqrt(sx * sx + sy * sy) + 1.5f;
+            float x = 0.0f, y = 0.0f;
+            bool ok = false;
+            for (int tries = 0; tries < 256 && !ok; ++tries) {
+                x = rng.uniform(-18.0f, 18.0f);
+                y = rng.uniform(-18.0f, 18.0f);
+                ok = true;
+                for (const auto& e : track)
+                    if ((e.first - x) * (e.first - x) + (e.second - y) * (e.second - y) < clear * clear) {
+                        ok = false;
+                        break;
+                    }
+            }
+            const float angle = rng.uniform(0.0f, 6.2831853f);
+            if (!ok) continue;
             float t[16];
-            trs(t, x, y, 0.5f * sz, rng.uniform(0.0f, 6.2831853f), sx, sy, sz);
+            trs(t, x, y, 0.5f * sz, angle, sx, sy, sz);
             add_instance(g, box_mesh[i % 4], t);
         }
     }
@@ -717,11 +761,7 @@ void build_atrium(pbrt_generated_scene& g, uint64_t seed) {
         l.strengths[0] = 0.5f;
         g.lights.push_back(l);
     }
-    // 64-frame Catmull-Rom camera path through 8 keys at eye height
-    const float keys[8][6] = {{-14, -14, 1.7f, 0, 0, 3},   {-4, -16, 2.0f, 2, 0, 4},   {10, -13, 1.8f, 0, 2, 3},
-                              {15, -2, 2.5f, -2, 2, 5},    {12, 12, 1.7f, 0, 0, 4},    {0, 15, 2.2f, 0, -2, 3},
-                              {-13, 10, 1.9f, 2, -2, 6},   {-16, -4, 1.7f, 2, 0, 3}};
-    for (auto& k : keys)
+    for (auto& k : kAtriumKeys)
         g.camera_keys.push_back(pbrt_camera_pose{{k[0], k[1], k[2]}, {k[3], k[4], k[5]}, {0, 0, 1}, 60.0f, 0.1f, 1000.0f});
     g.camera = g.camera_keys[0];
 }
@@ -875,11 +915,6 @@ void finalize(pbrt_generated_scene& g) {
     }
     g.tex_descs.clear();
     for (Texture& t : g.textures) g.tex_descs.push_back(pbrt_texture_desc{t.w, t.h, t.rgba.data()});
-}
-
-float catmull_rom(float p0, float p1, float p2, float p3, float t) {
-    float t2 = t * t, t3 = t2 * t;
-    return 0.5f * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t2 + (-p0 + 3 * p1 - 3 * p2 + p3) * t3);
 }
 
 }  // namespace

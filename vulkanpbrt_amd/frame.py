@@ -165,13 +165,15 @@ class Frame:
         capi.check(capi.lib().pbrt_bmfr(C.byref(self.bmfr_params(frame)), C.byref(self.bmfr_images(frame)),
                                         self.stream()))
 
-    def cnn(self, frame, layers=6):
+    def cnn(self, frame, layers=6, denoised=None):
+        """The learned denoiser on this frame's accumulated illumination; the final image
+        goes where BMFR's does, the demodulated radiance (RGBA32F) into `denoised` if given."""
         capi = self.capi
         cur = frame & 1
         im = capi.CnnImages(self.w, self.h, 1, capi.ptr(self.acc[cur]), capi.ptr(self.img["depth"]),
                             capi.ptr(self.img["normal"]), capi.ptr(self.img["albedo"]), capi.ptr(self.spp[cur]),
-                            capi.ptr(self.motion), None, capi.ptr(self.bmfr_final), None, capi.ptr(self.cnn_ws),
-                            self.cnn_ws.numel())
+                            capi.ptr(self.motion), capi.ptr(denoised), capi.ptr(self.bmfr_final), None,
+                            capi.ptr(self.cnn_ws), self.cnn_ws.numel())
         capi.check(capi.lib().pbrt_cnn_denoise(self.nets[layers].handle, C.byref(im), self.stream()))
 
     def denoise(self, frame):

@@ -350,7 +350,7 @@ std::string LearnedDenoiser::default_weights_path() {
         throw Exception(PBRT_ERR_INTERNAL, "LearnedDenoiser: cannot locate libpbrt.so");
     std::string lib = info.dli_fname;
     const size_t slash = lib.find_last_of('/');
-    return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/weights/kpn_cornell.bin";
+    return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/weights/kpn6.bin";
 }
 
 void LearnedDenoiser::compile(Context&) {

@@ -180,8 +180,8 @@ public:
                                                    AccumulationBufferPtr a) {
         return std::make_shared<LearnedDenoiser>(w, h, g, i, a);
     }
-    // weights/kpn_cornell.bin beside libpbrt.so (6 layers); a file of 2-layer
-    // parameters (weights/kpn2_cornell.bin) selects the fused two-layer network
+    // weights/kpn6.bin beside libpbrt.so (6 layers); a file of 2-layer
+    // parameters (weights/kpn2.bin) selects the fused two-layer network
     static std::string default_weights_path();
     void compile(Context& context);
     void update_image_layouts(Context&) {}
