@@ -7,7 +7,7 @@ temporal accumulation, BMFR (32x32 blocks) and TAA, with the camera moving along
 the scene's 64-frame spline.  Inputs (scene, BVH, textures, images) are resident
 in HBM before the timed region.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--config 2] [--tracer megakernel|wavefront]
+    python bench.py [--steps K] [--warmup W] [--config 2] [--tracer megakernel|persistent|wavefront]
 """
 from __future__ import annotations
 
@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 5])
-    ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "wavefront"],
+    ap.add_argument("--tracer", default="wavefront", choices=["megakernel", "persistent", "wavefront"],
                     help="tracer timed with --no-autotune")
     ap.add_argument("--bmfr-fit", default="gram", choices=["gram", "householder"])
     ap.add_argument("--autotune", dest="autotune", action="store_true", default=True,
@@ -345,8 +345,9 @@ def main():
         kernels["other"] = dict(ms_per_frame=round(timing.other_ms / K, 3))
         dom, dom_bytes, dom_ms = "k_wf_extend", ext_bytes, timing.extend_ms
     else:
-        dom, dom_bytes, dom_ms = "pt_megakernel", ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
-        kernels["pt_megakernel"] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
+        dom = "pt_persistent" if fr.tracer == capi.TRACER_PERSISTENT else "pt_megakernel"
+        dom_bytes, dom_ms = ext_bytes + sh_bytes + 36.0 * w * h * K, timing.megakernel_ms
+        kernels[dom] = dict(ms_per_frame=round(dom_ms / K, 3), launches=int(timing.megakernel_launches))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     workload = f"{cfg['tag']}-{bvh}"
     traffic, traffic_src = measured_traffic(dom, workload)

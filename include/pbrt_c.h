@@ -188,7 +188,10 @@ enum pbrt_illumination_output {
 
 enum pbrt_tracer_kind {
     PBRT_TRACER_MEGAKERNEL = 0, /* one thread per path, whole path in one launch */
-    PBRT_TRACER_WAVEFRONT = 1   /* ray queues: extend / shade / shadow launches per bounce */
+    PBRT_TRACER_WAVEFRONT = 1,  /* ray queues: extend / shade / shadow launches per bounce */
+    PBRT_TRACER_PERSISTENT = 2  /* megakernel with path regeneration: resident waves draw pixels
+                                   from a frame counter, a lane whose path ends starts the next
+                                   pixel (same per-pixel results as the megakernel) */
 };
 
 typedef struct pbrt_trace_params {
@@ -220,7 +223,7 @@ typedef struct pbrt_trace_images {
 } pbrt_trace_images;
 
 /* Autotuner: traces the frame once per candidate launch configuration (megakernel;
- * wavefront with 64 / 128 / 256-thread traversal workgroups x whole-frame, half and
+ * persistent megakernel; wavefront with 64 / 128 / 256-thread traversal workgroups x whole-frame, half and
  * quarter queues), `repeats` timed runs each after a warm-up, on `stream`, and returns
  * the fastest as `best` (a copy of `params` with tracer, block_threads and
  * queue_paths set).  No counterpart in the reference (the Vulkan driver schedules

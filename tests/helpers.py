@@ -41,6 +41,15 @@ def trace_workspace(torch, width, height):
     return _workspaces[key]
 
 
+TRACERS = ("megakernel", "persistent", "wavefront")
+
+
+def tracer_kind(name):
+    """C ABI tracer enum of a tracer name used in test parameters."""
+    return {"megakernel": capi.TRACER_MEGAKERNEL, "persistent": capi.TRACER_PERSISTENT,
+            "wavefront": capi.TRACER_WAVEFRONT}[name]
+
+
 def trace(torch, scene, pc, img, width, height, max_depth=2, tracer=capi.TRACER_MEGAKERNEL,
           output=capi.OUTPUT_DEMOD_ILLUMINATION_FLOAT, light_sampling=capi.LIGHT_SAMPLE_AUTO, counters=None,
           block_threads=0, queue_paths=0, min_depth=0):

@@ -32,7 +32,7 @@ def city(oracle_lib, pbrt_lib):
     return gen, scene, osc
 
 
-@pytest.mark.parametrize("tracer", [capi.TRACER_MEGAKERNEL, capi.TRACER_WAVEFRONT])
+@pytest.mark.parametrize("tracer", [capi.TRACER_MEGAKERNEL, capi.TRACER_PERSISTENT, capi.TRACER_WAVEFRONT])
 def test_city_matches_fp32_mirror(cuda, city, tracer):
     gen, scene, osc = city
     w, h, frame = 96, 54, 3

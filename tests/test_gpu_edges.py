@@ -27,13 +27,13 @@ def cfg1(cuda, pbrt_lib, oracle_lib):
     return gen, capi.Scene(gen.desc), oracle_lib.OracleScene(gen.desc)
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 @pytest.mark.parametrize("w,h", [(67, 45), (1, 1), (9, 130)])
 def test_tracers_on_ragged_frames(cuda, cfg1, tracer, w, h):
     gen, scene, osc = cfg1
     pc, _, _ = helpers.frame_push_constants(gen, w, h, 3)
     img = helpers.gbuffer(cuda, w, h)
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     helpers.trace(cuda, scene, pc, img, w, h, tracer=kind)
     cuda.cuda.synchronize()
     gpu = helpers.to_np(img)

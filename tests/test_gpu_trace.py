@@ -155,13 +155,13 @@ def _assert_mirror_exact(frac):
     assert frac["exact"] >= 0.999 and frac["radiance"] >= 0.999
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 @pytest.mark.parametrize("frame", [0, 5])
 def test_tracer_matches_fp32_mirror(cuda, cfg1, frame, tracer):
     gen, scene, osc = cfg1
     pc, _, _ = helpers.frame_push_constants(gen, W, H, frame)
     img = helpers.gbuffer(cuda, W, H)
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     helpers.trace(cuda, scene, pc, img, W, H, tracer=kind)
     cuda.cuda.synchronize()
     gpu = helpers.to_np(img)
@@ -171,7 +171,7 @@ def test_tracer_matches_fp32_mirror(cuda, cfg1, frame, tracer):
     _assert_mirror_exact(frac)
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 @pytest.mark.parametrize("mode", [capi.LIGHT_SAMPLE_LIGHT_STRENGTH, capi.LIGHT_SAMPLE_UNIFORM])
 def test_light_sampling_modes_match_fp32_mirror(cuda, cfg1, tracer, mode):
     """The other two light samplers (lighting.glsl:98-168 by light strength, :170-227
@@ -181,7 +181,7 @@ def test_light_sampling_modes_match_fp32_mirror(cuda, cfg1, tracer, mode):
     w = h = 128
     pc, _, _ = helpers.frame_push_constants(gen, w, h, 2)
     img = helpers.gbuffer(cuda, w, h)
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     helpers.trace(cuda, scene, pc, img, w, h, tracer=kind, light_sampling=mode)
     cuda.cuda.synchronize()
     gpu = helpers.to_np(img)
@@ -196,7 +196,7 @@ def test_light_sampling_modes_match_fp32_mirror(cuda, cfg1, tracer, mode):
     assert not np.array_equal(helpers.to_np(other)["illumination"], gpu["illumination"])
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 def test_final_image_output_matches_fp32_mirror(cuda, cfg1, tracer):
     """The tracer's own progressive output (PBRT_OUTPUT_FINAL_IMAGE, ptRaygen.rgen:84-103):
     sample 0 writes the gamma-encoded radiance; later samples jitter the camera ray
@@ -204,7 +204,7 @@ def test_final_image_output_matches_fp32_mirror(cuda, cfg1, tracer):
     the oracle given the same previous image."""
     gen, scene, osc = cfg1
     w = h = 96
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     img = helpers.gbuffer(cuda, w, h)
     prev = np.zeros((h, w, 4), np.float32)
     for sample in range(3):
@@ -238,7 +238,7 @@ def test_atrium_matches_fp32_mirror(cuda, pbrt_lib, oracle_lib):
     ref = osc.trace(w, h, iv, ip, frame=7, max_depth=4, precision=32)
     ref64 = osc.trace(w, h, iv, ip, frame=7, max_depth=4, precision=64)
     _compare_frame(ref64, ref, "atrium oracle fp64 vs fp32 mirror")
-    for tracer in (capi.TRACER_MEGAKERNEL, capi.TRACER_WAVEFRONT):
+    for tracer in (capi.TRACER_MEGAKERNEL, capi.TRACER_PERSISTENT, capi.TRACER_WAVEFRONT):
         img = helpers.gbuffer(cuda, w, h)
         helpers.trace(cuda, scene, pc, img, w, h, max_depth=4, tracer=tracer)
         cuda.cuda.synchronize()
@@ -247,14 +247,14 @@ def test_atrium_matches_fp32_mirror(cuda, pbrt_lib, oracle_lib):
         _assert_mirror_exact(frac)
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 def test_russian_roulette_start_matches_fp32_mirror(cuda, cfg1, tracer):
     """minRecursionDepth (lighting.glsl:279-284): no Russian roulette up to bounce 2 of
     5, then termination with probability 0.1 and throughput / 0.9."""
     gen, scene, osc = cfg1
     w = h = 96
     pc, _, _ = helpers.frame_push_constants(gen, w, h, 3)
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     img = helpers.gbuffer(cuda, w, h)
     cnt = cuda.zeros(6, dtype=cuda.int64, device="cuda")
     helpers.trace(cuda, scene, pc, img, w, h, max_depth=5, min_depth=2, tracer=kind, counters=cnt)
@@ -271,7 +271,7 @@ def test_wavefront_counts_rays(cuda, cfg1):
     gen, scene, osc = cfg1
     pc, _, _ = helpers.frame_push_constants(gen, W, H, 2)
     counts = {}
-    for kind in (capi.TRACER_MEGAKERNEL, capi.TRACER_WAVEFRONT):
+    for kind in (capi.TRACER_MEGAKERNEL, capi.TRACER_PERSISTENT, capi.TRACER_WAVEFRONT):
         img = helpers.gbuffer(cuda, W, H)
         cnt = cuda.zeros(6, dtype=cuda.int64, device="cuda")
         helpers.trace(cuda, scene, pc, img, W, H, tracer=kind, counters=cnt)
@@ -286,6 +286,30 @@ def test_wavefront_counts_rays(cuda, cfg1):
     assert abs(int(wf[0]) - ref["rays"]) <= 0.01 * ref["rays"]
     # the wavefront skips shadow rays whose contribution is exactly zero
     assert wf[1] <= mk[1] and abs(int(mk[1]) - ref["shadow_rays"]) <= 0.01 * ref["shadow_rays"]
+    # path regeneration changes which lane traces a path, not the path: rays, node
+    # visits and triangle tests all equal the megakernel's
+    assert np.array_equal(counts[capi.TRACER_PERSISTENT], mk)
+
+
+@pytest.mark.parametrize("w,h", [(W, H), (61, 37), (1, 1), (200, 9)])
+def test_persistent_equals_megakernel(cuda, cfg1, w, h):
+    """The persistent tracer draws pixels from a per-launch counter that its last wave
+    clears: frames traced back to back (different sizes and frames in between) are
+    byte-identical to the megakernel's, so every pixel is traced exactly once per
+    launch and the counter starts from zero each time."""
+    gen, scene, _ = cfg1
+    for frame in (0, 3, 3):
+        pc, _, _ = helpers.frame_push_constants(gen, w, h, frame)
+        ref = helpers.gbuffer(cuda, w, h)
+        helpers.trace(cuda, scene, pc, ref, w, h, max_depth=3, tracer=capi.TRACER_MEGAKERNEL)
+        img = {k: cuda.full_like(v, 7) for k, v in helpers.gbuffer(cuda, w, h).items()}
+        helpers.trace(cuda, scene, pc, img, w, h, max_depth=3, tracer=capi.TRACER_PERSISTENT)
+        other = helpers.gbuffer(cuda, 13, 11)  # another size between the launches
+        helpers.trace(cuda, scene, pc, other, 13, 11, tracer=capi.TRACER_PERSISTENT)
+        cuda.cuda.synchronize()
+        for k in ref:
+            a, b = ref[k].cpu().numpy(), img[k].cpu().numpy()
+            assert a.tobytes() == b.tobytes(), f"{k} differs at {w}x{h} frame {frame}"
 
 
 def test_trace_timing_records_every_stage(cuda, cfg1):
@@ -349,13 +373,13 @@ def test_autotune_picks_a_measured_candidate(cuda, cfg1):
     ms = [tun.ms[i] for i in range(tun.candidates)]
     print("autotune candidates (tracer, block, queue, ms):",
           [(tun.tracer[i], tun.block_threads[i], tun.queue_paths[i], round(tun.ms[i], 3)) for i in range(tun.candidates)])
-    assert tun.candidates == 10 and all(m > 0 for m in ms)
+    assert tun.candidates == 11 and all(m > 0 for m in ms)
     best = int(np.argmin(ms))
     assert (tun.best.tracer, tun.best.block_threads, tun.best.queue_paths) == \
         (tun.tracer[best], tun.block_threads[best], tun.queue_paths[best])
 
 
-@pytest.mark.parametrize("tracer", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("tracer", helpers.TRACERS)
 @pytest.mark.parametrize("max_depth", [0, 1])
 def test_tracer_on_the_pinned_floor_scene(cuda, pbrt_lib, oracle_lib, tracer, max_depth):
     """The scene whose NEE and first bounce the oracle pins in closed form
@@ -365,7 +389,7 @@ def test_tracer_on_the_pinned_floor_scene(cuda, pbrt_lib, oracle_lib, tracer, ma
     fl = FloorUnderSun()
     scene = capi.Scene(fl.d.desc)
     n = 32
-    kind = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+    kind = helpers.tracer_kind(tracer)
     for frame in (1, 2):
         pc = capi.PushConstants()
         pc.view_inverse[:] = fl.inv_view.tolist()

@@ -33,6 +33,7 @@ OUTPUT_DEMOD_ILLUMINATION_FLOAT = 0
 OUTPUT_FINAL_IMAGE = 1
 TRACER_MEGAKERNEL = 0
 TRACER_WAVEFRONT = 1
+TRACER_PERSISTENT = 2
 
 
 class PushConstants(C.Structure):

@@ -17,6 +17,7 @@
 // buffers and object matrix exactly as ptClosesthit.rchit does.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -277,12 +278,22 @@ struct pbrt_scene {
     DeviceScene dev{};
     pbrt_scene_info info{};
     mutable TraceTiming timing;
+    // Pixel counters of the persistent tracer: a ring of kWorkSlots slots of
+    // (next pixel, finished waves), 64 B apart; each launch takes the next slot
+    // and its last wave clears it.
+    DeviceBuffer work;
+    mutable std::atomic<uint32_t> work_next{0};
 };
 
 namespace pbrt {
 const DeviceScene& scene_device_view(const pbrt_scene* s) { return s->dev; }
 
 TraceTiming* scene_timing(const pbrt_scene* s) { return s->timing.active() ? &s->timing : nullptr; }
+
+uint32_t* scene_work_counter(const pbrt_scene* s) {
+    const uint32_t slot = s->work_next.fetch_add(1, std::memory_order_relaxed) % kWorkSlots;
+    return s->work.as<uint32_t>() + (size_t)slot * kWorkSlotWords;
+}
 
 TraceTiming::~TraceTiming() {
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
@@ -437,6 +448,10 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
         s->textures.upload(ps.textures.data(), ps.textures.size() * sizeof(DeviceTexture));
         s->texels.upload(ps.texels.data(), ps.texels.size() * sizeof(uint32_t));
         s->lights.upload(ps.lights.data(), ps.lights.size() * sizeof(DeviceLight));
+        {
+            const std::vector<uint32_t> zeros((size_t)kWorkSlots * kWorkSlotWords, 0u);
+            s->work.upload(zeros.data(), zeros.size() * sizeof(uint32_t));
+        }
 
         DeviceScene& dv = s->dev;
         dv.tris = s->tris.as<float4>();

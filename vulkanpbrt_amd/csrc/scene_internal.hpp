@@ -65,4 +65,13 @@ private:
 // The scene's timing log when recording is active, else nullptr.
 TraceTiming* scene_timing(const pbrt_scene* s);
 
+// Work counters of the persistent tracer: each call returns the next slot of a
+// ring of kWorkSlots zeroed pairs (next pixel, finished waves); the launch's last
+// wave sets its slot back to zero, so traces of one scene that run at the same
+// time (on several streams) use different slots while fewer than kWorkSlots are
+// in flight.
+constexpr uint32_t kWorkSlots = 64;
+constexpr uint32_t kWorkSlotWords = 16;
+uint32_t* scene_work_counter(const pbrt_scene* s);
+
 }  // namespace pbrt

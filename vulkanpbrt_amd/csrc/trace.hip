@@ -12,6 +12,9 @@
 //   * the motion/alpha/format rules of the other passes live in their files.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "capi_common.hpp"
@@ -149,6 +152,181 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
     }
 }
 
+// Pixel of the id-th path of a frame: the frame is cut into bands of 8 rows and
+// each band into tiles of 8 columns (narrower at the right edge, shorter in the
+// last band); ids run tile by tile, row-major inside a tile, so consecutive ids
+// are neighbouring pixels and ids 0 .. width * height - 1 cover the frame once.
+__device__ __forceinline__ void persistent_pixel(uint32_t id, uint32_t w, uint32_t h, uint32_t& px, uint32_t& py) {
+    const uint32_t band = id / (8u * w);
+    const uint32_t rows = min(8u, h - 8u * band);
+    const uint32_t l = id - band * 8u * w;
+    const uint32_t tc = l / (8u * rows);
+    const uint32_t tw = min(8u, w - 8u * tc);
+    const uint32_t k = l - tc * 8u * rows;
+    px = 8u * tc + k % tw;
+    py = 8u * band + k / tw;
+}
+
+// The megakernel with path regeneration.  Each 64-lane workgroup stays resident
+// and runs the path of ptRaygen.rgen:24-104 one ray segment per iteration: trace
+// (primary or bounce ray), shading, next-event estimation, then the next
+// bounce's BRDF sample and Russian roulette, exactly as pt_megakernel orders
+// them (the random draws of a pixel come in the same sequence).  A lane whose
+// path has ended writes its pixel and takes the next pixel id from work[0]
+// (one atomic per wave for all lanes that need one), so lanes stay busy until
+// the frame runs out instead of idling while the wave's longest path finishes.
+// The wave that finishes last sets work[0..1] back to zero for the next launch.
+template <bool kCount>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(4))) pt_persistent(DeviceScene sc, TraceConsts c, float* depth_img,
+                                                              float* normal_img, uint8_t* material_img,
+                                                              uint8_t* albedo_img, float4* illum_img,
+                                                              unsigned long long* counters, uint32_t* work) {
+    __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
+    const uint32_t lane = threadIdx.x;
+    int* stack = stack_mem + lane;
+    const uint32_t total = c.width * c.height;
+    const bool anti_alias = c.output == PBRT_OUTPUT_FINAL_IMAGE && c.sample_number > 0;
+    TraversalCounters cnt{0, 0}, cnt_sh{0, 0};
+    uint32_t rays = 0, shadow_rays = 0;
+
+    bool alive = false, exhausted = false, primary = false;
+    uint32_t pix = 0;
+    Rng re{0, 0, 0, 0};
+    vec3 ro = make_vec3(0.0f), rd = make_vec3(0.0f), throughput = make_vec3(1.0f), color = make_vec3(0.0f),
+         albedo = make_vec3(0.0f);
+    int depth_i = 0, trans_depth = 0;
+    while (true) {
+        const unsigned long long need = __ballot(!alive && !exhausted);
+        if (need != 0ull) {
+            const int leader = __ffsll(need) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(work, (uint32_t)__popcll(need));
+            base = __shfl(base, leader);
+            if (!alive && !exhausted) {
+                const uint32_t id = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                if (id < total) {
+                    uint32_t px, py;
+                    persistent_pixel(id, c.width, c.height, px, py);
+                    pix = py * c.width + px;
+                    re = rng_init(px, py, c.frame_number);
+                    throughput = make_vec3(1.0f);
+                    create_ray(c, px, py, anti_alias, re, ro, rd);
+                    alive = true;
+                    primary = true;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+
+        SurfaceInfo s;
+        vec3 hit_pos;
+        uint32_t category;
+        ++rays;
+        trace_closest<kCount>(sc, ro, rd, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+        // throughput is (1, 1, 1) on the primary segment
+        const vec3 ne = nee_inline<kCount>(sc, c.light_sampling, hit_pos, primary ? -normalize(rd) : -rd, s,
+                                           throughput, re, stack, cnt_sh, shadow_rays);
+        bool finish;
+        if (primary) {
+            color = make_vec3(0.0f);
+            color += ne;
+            color += s.emissive_color;
+            albedo = s.diffuse_color + s.specular_color;
+            if (c.write_gbuffer) {
+                depth_img[pix] = distance(hit_pos, ro);
+                normal_img[2 * pix + 0] = acosf(s.normal.z);
+                normal_img[2 * pix + 1] = atan2f(s.normal.y, s.normal.x);
+                uint8_t* mat = material_img + 4 * (size_t)pix;
+                mat[0] = unorm8((float)category / 255.0f);
+                mat[1] = 0;
+                mat[2] = 0;
+                mat[3] = 0;
+                uint8_t* alb = albedo_img + 4 * (size_t)pix;
+                alb[0] = unorm8(albedo.x);
+                alb[1] = unorm8(albedo.y);
+                alb[2] = unorm8(albedo.z);
+                alb[3] = 255;
+            }
+            primary = false;
+            depth_i = 0;
+            trans_depth = 0;
+            finish = is_miss(s);
+        } else {
+            color += ne + s.emissive_color * throughput;
+            finish = is_miss(s);
+            ++depth_i;
+        }
+        // head of the bounce loop (lighting.glsl:263-287 as in pt_megakernel)
+        if (!finish) finish = !(depth_i < (int)c.max_depth && trans_depth < 10);
+        if (!finish) {
+            if (s.illumination_type == 7) {
+                --depth_i;
+                ++trans_depth;
+            }
+            const vec3 v = normalize(ro - hit_pos);
+            vec3 l;
+            float pdf;
+            const vec3 brdf = sample_brdf(s, re, v, l, pdf);
+            if (brdf == make_vec3(0.0f) || pdf < kEpsilon) {
+                finish = true;
+            } else {
+                float tc = dot(l, s.normal);
+                if (s.illumination_type == 7) tc = 1.0f;
+                vec3 path_tp = throughput * (brdf * tc) / pdf;
+                if ((uint32_t)depth_i > c.min_depth) {
+                    if (rng_float(re) < kMinTermination)
+                        finish = true;
+                    else
+                        path_tp /= 1.0f - kMinTermination;
+                }
+                if (!finish) {
+                    throughput = path_tp;
+                    ro = hit_pos;
+                    rd = l;
+                }
+            }
+        }
+        if (finish) {
+            color = vclamp(color, 0.0f, kMaxRadiance);
+            if (c.output == PBRT_OUTPUT_DEMOD_ILLUMINATION_FLOAT) {
+                vec3 demod = color;
+                if (!isinf(hit_pos.x)) demod = vmin(color / (albedo + make_vec3(kEpsilon)), 1e3f);
+                illum_img[pix] = make_float4(demod.x, demod.y, demod.z, 1.0f);
+            } else {
+                vec3 fc = color;
+                if (c.sample_number > 0) {
+                    float4 prev = illum_img[pix];
+                    float alpha = 1.0f / (float)(c.sample_number + 1);
+                    vec3 pl = srgb_to_linear(make_vec3(prev.x, prev.y, prev.z));
+                    fc = vmix(pl, fc, alpha);
+                }
+                fc = linear_to_srgb(fc);
+                fc = vclamp(fc, 0.0f, 1.0f);
+                illum_img[pix] = make_float4(fc.x, fc.y, fc.z, 1.0f);
+            }
+            alive = false;
+        }
+    }
+    if (kCount) {
+        atomicAdd(&counters[0], (unsigned long long)rays);
+        atomicAdd(&counters[1], (unsigned long long)shadow_rays);
+        atomicAdd(&counters[2], (unsigned long long)cnt.nodes);
+        atomicAdd(&counters[3], (unsigned long long)cnt.tris);
+        atomicAdd(&counters[4], (unsigned long long)cnt_sh.nodes);
+        atomicAdd(&counters[5], (unsigned long long)cnt_sh.tris);
+    }
+    if (lane == 0) {
+        // every other wave has drawn its last id before it counts itself finished
+        if (atomicAdd(work + 1, 1u) == gridDim.x - 1) {
+            atomicExch(work, 0u);
+            atomicExch(work + 1, 0u);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kTileThreads) k_intersect(DeviceScene sc, const float* rays, uint32_t n,
                                                             uint32_t* hits, float* bary) {
     __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
@@ -205,6 +383,31 @@ __global__ void k_rng_sequence(uint32_t width, uint32_t height, uint32_t frame, 
 
 using namespace pbrt;
 
+namespace {
+
+// Workgroups of the persistent tracer: as many as stay resident on the current
+// device at once (occupancy of the kernel variant times the compute units).
+uint32_t persistent_waves(bool counting) {
+    int dev = 0;
+    PBRT_HIP(hipGetDevice(&dev));
+    static std::mutex m;
+    static std::map<std::pair<int, bool>, uint32_t> cache;
+    std::lock_guard<std::mutex> lock(m);
+    auto it = cache.find({dev, counting});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (counting)
+        PBRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pt_persistent<true>, kTileThreads, 0));
+    else
+        PBRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pt_persistent<false>, kTileThreads, 0));
+    PBRT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t n = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    cache[{dev, counting}] = n;
+    return n;
+}
+
+}  // namespace
+
 extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* p, const pbrt_push_constants* pc,
                               const pbrt_trace_images* img, unsigned long long* counters, void* stream) {
     return guard([&] {
@@ -215,8 +418,10 @@ extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* 
         PBRT_REQUIRE(img->illumination, "pbrt_trace: illumination image is NULL");
         PBRT_REQUIRE(!p->write_gbuffer || (img->depth && img->normal && img->material && img->albedo),
                      "pbrt_trace: G-buffer requested but an image is NULL");
-        PBRT_REQUIRE(p->tracer == PBRT_TRACER_MEGAKERNEL || p->tracer == PBRT_TRACER_WAVEFRONT,
+        PBRT_REQUIRE(p->tracer == PBRT_TRACER_MEGAKERNEL || p->tracer == PBRT_TRACER_WAVEFRONT ||
+                         p->tracer == PBRT_TRACER_PERSISTENT,
                      "pbrt_trace: unknown tracer");
+        PBRT_REQUIRE((uint64_t)p->width * p->height < (1ull << 31), "pbrt_trace: more than 2^31 pixels");
         const DeviceScene& sc = scene_device_view(scene);
         TraceConsts c{};
         std::memcpy(c.inv_view, pc->view_inverse, sizeof(c.inv_view));
@@ -240,6 +445,19 @@ extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* 
         TraceTiming* tm = scene_timing(scene);
         if (p->tracer == PBRT_TRACER_WAVEFRONT) {
             wavefront_trace(sc, c, *img, counters, tm, p->block_threads, p->queue_paths, st);
+        } else if (p->tracer == PBRT_TRACER_PERSISTENT) {
+            const uint32_t pixels = p->width * p->height;
+            const uint32_t waves =
+                std::min(persistent_waves(counters != nullptr), (pixels + kTileThreads - 1) / kTileThreads);
+            uint32_t* work = scene_work_counter(scene);
+            if (tm) tm->frame_begin(st);
+            if (counters)
+                hipLaunchKernelGGL(pt_persistent<true>, dim3(waves), dim3(kTileThreads), 0, st, sc, c, img->depth,
+                                   img->normal, img->material, img->albedo, illum, counters, work);
+            else
+                hipLaunchKernelGGL(pt_persistent<false>, dim3(waves), dim3(kTileThreads), 0, st, sc, c, img->depth,
+                                   img->normal, img->material, img->albedo, illum, counters, work);
+            if (tm) tm->mark(kTimeMegakernel, st);
         } else {
             if (tm) tm->frame_begin(st);
             if (counters)
@@ -301,7 +519,7 @@ extern "C" int32_t pbrt_autotune_trace(const pbrt_scene* scene, const pbrt_trace
         struct Cand {
             uint32_t tracer, block, queue;
         };
-        std::vector<Cand> cands{{PBRT_TRACER_MEGAKERNEL, 0, 0}};
+        std::vector<Cand> cands{{PBRT_TRACER_MEGAKERNEL, 0, 0}, {PBRT_TRACER_PERSISTENT, 0, 0}};
         const uint32_t pixels = p->width * p->height;
         if (img->workspace) {
             for (uint32_t block : {64u, 128u, 256u})

@@ -17,7 +17,8 @@ class Frame:
         self.torch, self.capi, self.camera = torch, capi, camera
         self.denoiser = denoiser
         self.gen, self.scene, self.w, self.h, self.depth = gen, scene, w, h, depth
-        self.tracer = capi.TRACER_WAVEFRONT if tracer == "wavefront" else capi.TRACER_MEGAKERNEL
+        self.tracer = {"wavefront": capi.TRACER_WAVEFRONT, "persistent": capi.TRACER_PERSISTENT}.get(
+            tracer, capi.TRACER_MEGAKERNEL)
         self.block_threads, self.queue_paths = 0, 0
         self.bmfr_threads = 0  # Gram-fit workgroup size (0: the library default)
         self.fit = capi.BMFR_FIT_GRAM_F64 if bmfr_fit == "gram" else capi.BMFR_FIT_HOUSEHOLDER
@@ -98,7 +99,8 @@ class Frame:
                                                   C.byref(self.trace_images()), repeats, C.byref(tun), self.stream()))
         self.tracer, self.block_threads, self.queue_paths = tun.best.tracer, tun.best.block_threads, \
             tun.best.queue_paths
-        names = {capi.TRACER_MEGAKERNEL: "megakernel", capi.TRACER_WAVEFRONT: "wavefront"}
+        names = {capi.TRACER_MEGAKERNEL: "megakernel", capi.TRACER_WAVEFRONT: "wavefront",
+                 capi.TRACER_PERSISTENT: "persistent"}
         out = {"best": {"tracer": names[tun.best.tracer], "block_threads": tun.best.block_threads,
                         "queue_paths": tun.best.queue_paths},
                "candidates": [{"tracer": names[tun.tracer[i]], "block_threads": tun.block_threads[i],
