@@ -42,7 +42,7 @@ def parse():
                          "tile shape (pbrt_autotune_bmfr) on the first frame before timing")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time --tracer with the library's default launch shapes")
-    ap.add_argument("--bvh", default=None, choices=["sah", "sbvh", "lbvh"],
+    ap.add_argument("--bvh", default=None, choices=["sah", "sbvh", "lbvh", "ploc"],
                     help="BVH builder: binned SAH on the host, SAH with spatial splits on the host, or the "
                          "Morton-code LBVH on the GPU (default: spatial splits; binned SAH for the 5M-triangle "
                          "city, whose small triangles gain nothing from spatial splits)")
@@ -216,7 +216,8 @@ def main():
     t_setup = time.perf_counter()
     gen = capi.GeneratedScene(args.config, cfg["seed"])
     bvh = args.bvh or cfg.get("bvh", "sbvh")
-    builder = {"lbvh": capi.PBRT_BVH_LBVH_GPU, "sbvh": capi.PBRT_BVH_SBVH_HOST}.get(bvh, capi.PBRT_BVH_SAH_HOST)
+    builder = {"lbvh": capi.PBRT_BVH_LBVH_GPU, "sbvh": capi.PBRT_BVH_SBVH_HOST,
+               "ploc": capi.PBRT_BVH_PLOC_GPU}.get(bvh, capi.PBRT_BVH_SAH_HOST)
     gen.desc.bvh_builder = builder
     desc, loaded, load_info = gen.desc, None, None
     if args.scene:

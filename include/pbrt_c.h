@@ -112,8 +112,11 @@ typedef struct pbrt_instance_desc {
 enum pbrt_bvh_builder {
     PBRT_BVH_SAH_HOST = 0,  /* binned SAH on the host */
     PBRT_BVH_LBVH_GPU = 1,  /* Morton-code LBVH built by gfx950 kernels */
-    PBRT_BVH_SBVH_HOST = 2  /* binned SAH with spatial splits on the host: fewer node visits per
+    PBRT_BVH_SBVH_HOST = 2, /* binned SAH with spatial splits on the host: fewer node visits per
                                ray, a few percent more triangle references, slower build */
+    PBRT_BVH_PLOC_GPU = 3   /* parallel locally-ordered clustering (nearest-neighbour merges
+                               along the Morton order) with a SAH leaf collapse, built by gfx950
+                               kernels: SAH-quality trees in milliseconds */
 };
 
 typedef struct pbrt_scene_desc {
@@ -139,8 +142,9 @@ typedef struct pbrt_scene_info {
     uint32_t bvh_depth;
     uint32_t light_sampling;   /* enum pbrt_light_sampling chosen at packing */
     uint64_t device_bytes;
-    uint32_t bvh_builder;      /* enum pbrt_bvh_builder that built the tree (LBVH needs >= 2 triangles) */
-    float bvh_build_ms;        /* host wall time (SAH) or device time (LBVH) of the build */
+    uint32_t bvh_builder;      /* enum pbrt_bvh_builder that built the tree (LBVH and PLOC need >= 2
+                                  triangles; a single triangle gets the host SAH tree) */
+    float bvh_build_ms;        /* host wall time (SAH, SBVH) or device time (LBVH, PLOC) of the build */
 } pbrt_scene_info;
 
 typedef struct pbrt_scene pbrt_scene;
@@ -167,6 +171,15 @@ int32_t pbrt_scene_destroy(pbrt_scene* scene);
 int32_t pbrt_scene_get_info(const pbrt_scene* scene, pbrt_scene_info* out_info);
 /* Copies the packed light list (PBRTPipeline.cpp:154-162 data) to host memory. */
 int32_t pbrt_scene_get_lights(const pbrt_scene* scene, pbrt_light* out_lights, uint32_t capacity);
+
+/* Stable LSD radix sort of n (key, value) pairs of device uint32 by key bits [0, bits), the
+ * device sort of the GPU BVH builders (Morton codes), on `stream`.  keys_tmp / vals_tmp are
+ * device arrays of n entries and `scratch` holds pbrt_radix_sort_scratch_size(n) bytes; the
+ * sorted pairs end in keys / vals.  No counterpart in the reference (the Vulkan driver builds
+ * its acceleration structures, AccelerationStructure.cpp:44). */
+int32_t pbrt_radix_sort_scratch_size(uint32_t n, size_t* out_bytes);
+int32_t pbrt_radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
+                              uint32_t bits, void* scratch, size_t scratch_bytes, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* Path tracing: replaces vkCmdTraceRaysKHR over ptRaygen.rgen /              */

@@ -25,6 +25,7 @@ PBRT_ERR_INTERNAL = -5
 PBRT_BVH_SAH_HOST = 0
 PBRT_BVH_LBVH_GPU = 1
 PBRT_BVH_SBVH_HOST = 2
+PBRT_BVH_PLOC_GPU = 3
 LIGHT_SAMPLE_SURFACE_STRENGTH = 0
 LIGHT_SAMPLE_LIGHT_STRENGTH = 1
 LIGHT_SAMPLE_UNIFORM = 2
@@ -270,6 +271,9 @@ SIGNATURES = {
     "pbrt_scene_destroy": (C.c_int32, [C.c_void_p]),
     "pbrt_scene_get_info": (C.c_int32, [C.c_void_p, C.POINTER(SceneInfo)]),
     "pbrt_scene_get_lights": (C.c_int32, [C.c_void_p, C.POINTER(Light), C.c_uint32]),
+    "pbrt_radix_sort_scratch_size": (C.c_int32, [C.c_uint32, C.POINTER(C.c_size_t)]),
+    "pbrt_radix_sort_pairs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbrt_trace": (C.c_int32, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(PushConstants),
                                C.POINTER(TraceImages), C.c_void_p, C.c_void_p]),
     "pbrt_trace_workspace_size": (C.c_int32, [C.c_uint32, C.c_uint32, C.POINTER(C.c_size_t)]),

@@ -4,7 +4,7 @@
 //
 //   1. centroid bounds          per-block min/max, one folding block
 //   2. Morton codes             30 bits (10 per axis) of the centroid in its bounds
-//   3. sort                     rocPRIM radix sort of (code, triangle) pairs
+//   3. sort                     stable radix sort of (code, triangle) pairs (gpu_sort.hpp)
 //   4. hierarchy                Karras' binary radix tree: internal node i finds its
 //                               key range and split from longest common prefixes
 //                               (equal codes are told apart by their sorted index)
@@ -17,11 +17,10 @@
 // (AccelerationStructure.cpp:44); this builder is the project's own.
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include <vector>
 
 #include "capi_common.hpp"
+#include "gpu_sort.hpp"
 #include "lbvh.hpp"
 #include "scene_types.hpp"
 
@@ -251,10 +250,7 @@ LbvhResult build_lbvh(const std::vector<float>& world_tris, const std::vector<ui
     arrivals.allocate((size_t)(n - 1) * sizeof(int));
     boxes.allocate((size_t)(2 * n - 1) * sizeof(Aabb));
     depth.allocate(sizeof(int));
-    size_t temp_bytes = 0;
-    PBRT_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
-                                       vals.as<uint32_t>(), vals2.as<uint32_t>(), n, 0, 30, st));
-    temp.allocate(temp_bytes);
+    temp.allocate(gpusort::sort_scratch_bytes(n));
     PBRT_HIP(hipMemset(parent.ptr, 0xFF, parent.bytes));  // root's parent: -1
     PBRT_HIP(hipDeviceSynchronize());
 
@@ -263,12 +259,14 @@ LbvhResult build_lbvh(const std::vector<float>& world_tris, const std::vector<ui
     hipLaunchKernelGGL(k_fold_bounds, dim3(1), dim3(64), 0, st, partial.as<float>(), rb);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(256), 0, st, tris.as<float>(), n, partial.as<float>(),
                        keys.as<uint32_t>(), vals.as<uint32_t>());
-    PBRT_HIP(rocprim::radix_sort_pairs(temp.ptr, temp_bytes, keys.as<uint32_t>(), keys2.as<uint32_t>(),
-                                       vals.as<uint32_t>(), vals2.as<uint32_t>(), n, 0, 30, st));
-    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(256), 0, st, keys2.as<uint32_t>(), (int)n,
+    const bool in2 = gpusort::sort_pairs(keys.as<uint32_t>(), vals.as<uint32_t>(), keys2.as<uint32_t>(),
+                                         vals2.as<uint32_t>(), n, 30, temp.ptr, st);
+    const uint32_t* skeys = in2 ? keys2.as<uint32_t>() : keys.as<uint32_t>();
+    const uint32_t* svals = in2 ? vals2.as<uint32_t>() : vals.as<uint32_t>();
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(256), 0, st, skeys, (int)n,
                        child.as<int2>(), range.as<int2>(), parent.as<int>());
     PBRT_HIP(hipMemsetAsync(arrivals.ptr, 0, arrivals.bytes, st));
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n)), dim3(256), 0, st, tris.as<float>(), vals2.as<uint32_t>(), (int)n,
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n)), dim3(256), 0, st, tris.as<float>(), svals, (int)n,
                        child.as<int2>(), parent.as<int>(), arrivals.as<int>(), boxes.as<Aabb>());
     nodes_out.allocate((size_t)(n - 1) * 4 * sizeof(float4));
     tris_out.allocate((size_t)n * 3 * sizeof(float4));
@@ -290,7 +288,7 @@ LbvhResult build_lbvh(const std::vector<float>& world_tris, const std::vector<ui
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(n - 1)), dim3(256), 0, st, (int)n, child.as<int2>(), range.as<int2>(),
                        boxes.as<Aabb>(), max_leaf, nodes_out.as<float4>());
     hipLaunchKernelGGL(k_gather_tris, dim3(blocks_for(n)), dim3(256), 0, st, tris.as<float>(), meta.as<uint4>(),
-                       vals2.as<uint32_t>(), n, tris_out.as<float4>());
+                       svals, n, tris_out.as<float4>());
     PBRT_HIP(hipEventRecord(e1, st));
     PBRT_HIP(hipGetLastError());
     PBRT_HIP(hipEventSynchronize(e1));

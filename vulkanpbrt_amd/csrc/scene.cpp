@@ -27,6 +27,7 @@
 #include "bvh_host.hpp"
 #include "bvh_quant.hpp"
 #include "lbvh.hpp"
+#include "ploc.hpp"
 #include "capi_common.hpp"
 #include "scene_internal.hpp"
 
@@ -378,7 +379,7 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
         PBRT_REQUIRE(desc != nullptr && out_scene != nullptr, "pbrt_scene_create: NULL argument");
         *out_scene = nullptr;
         PBRT_REQUIRE(desc->bvh_builder == PBRT_BVH_SAH_HOST || desc->bvh_builder == PBRT_BVH_LBVH_GPU ||
-                         desc->bvh_builder == PBRT_BVH_SBVH_HOST,
+                         desc->bvh_builder == PBRT_BVH_SBVH_HOST || desc->bvh_builder == PBRT_BVH_PLOC_GPU,
                      "pbrt_scene_create: unknown bvh_builder");
         PBRT_REQUIRE(desc->instance_count > 0, "pbrt_scene_create: scene has no instances");
         auto s = std::make_unique<pbrt_scene>();
@@ -409,6 +410,15 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
             node_count = r.node_count;
             s->info.bvh_build_ms = r.build_ms;
             s->info.bvh_builder = PBRT_BVH_LBVH_GPU;
+        } else if (desc->bvh_builder == PBRT_BVH_PLOC_GPU && ntri >= 2) {
+            std::vector<uint32_t> flags(ntri);
+            for (uint32_t t = 0; t < ntri; ++t) flags[t] = ps.instances[ps.tri_inst[t]].flags;
+            PlocResult r = build_ploc(ps.world_tris, ps.tri_inst, ps.tri_prim, flags, kPlocMaxLeaf,
+                                      kTraversalStackSize, s->tris, s->nodes);
+            s->bvh.depth = r.depth;
+            node_count = r.node_count;
+            s->info.bvh_build_ms = r.build_ms;
+            s->info.bvh_builder = PBRT_BVH_PLOC_GPU;
         } else {
             const bool spatial = desc->bvh_builder == PBRT_BVH_SBVH_HOST;
             s->bvh = spatial ? build_bvh_sbvh(ps.world_tris.data(), ntri, 4, kTraversalStackSize)

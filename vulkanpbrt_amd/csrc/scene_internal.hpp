@@ -11,11 +11,15 @@
 
 namespace pbrt {
 
-// Per-thread traversal stack entries (LDS, 4 B each).  The BVH builder bounds
+// Per-thread traversal stack entries (LDS, 4 B each).  The BVH builders bound
 // the tree depth by this value (median splits once SAH splits could overrun
-// it), and a traversal holds at most depth - 1 deferred children.  32 entries
-// keep the stack at 8 KB per 64-lane wave.
-constexpr int kTraversalStackSize = 32;
+// it), and a traversal holds at most depth - 1 deferred children.  40 entries
+// keep the stack at 10 KB per 64-lane wave: the megakernel's 16 resident waves
+// per CU (four per SIMD, set by its registers) fill the 160 KB of LDS exactly.
+constexpr int kTraversalStackSize = 40;
+
+// Largest leaf the GPU PLOC builder's SAH collapse may form (the node format holds 16).
+constexpr int kPlocMaxLeaf = 8;
 
 struct PackedScene {
     std::vector<DeviceTexture> textures;

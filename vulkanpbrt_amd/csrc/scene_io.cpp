@@ -908,7 +908,7 @@ extern "C" int32_t pbrt_load_obj(const char* path, const pbrt_load_options* opti
     return guard([&] {
         PBRT_REQUIRE(path && out, "pbrt_load_obj: NULL argument");
         *out = nullptr;
-        PBRT_REQUIRE(!options || options->bvh_builder <= PBRT_BVH_LBVH_GPU, "pbrt_load_obj: unknown bvh_builder");
+        PBRT_REQUIRE(!options || options->bvh_builder <= PBRT_BVH_PLOC_GPU, "pbrt_load_obj: unknown bvh_builder");
         auto sc = std::make_unique<pbrt_loaded_scene>();
         load_obj(path, options, *sc);
         *out = sc.release();
