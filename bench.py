@@ -46,7 +46,7 @@ def parse():
                     help="BVH builder: binned SAH on the host, SAH with spatial splits on the host, or the "
                          "Morton-code LBVH on the GPU (default: spatial splits; binned SAH for the 5M-triangle "
                          "city, whose small triangles gain nothing from spatial splits)")
-    ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn", "cnn2"],
+    ap.add_argument("--denoiser", default="bmfr", choices=["bmfr", "cnn", "cnn2", "cnnc"],
                     help="denoiser of the timed frame: BMFR 32x32, the 6-layer learned kernel-predicting network "
                          "or the fused 2-layer one")
     ap.add_argument("--no-denoise-eval", action="store_true",
@@ -132,10 +132,12 @@ def denoise_comparison(torch, fr, frame, runs, evaluate):
     accumulate -> denoise, frames 0 and 7 scored against 1024-spp references (BMFR
     with its temporal history, the networks on the same accumulated input; the full
     4096-spp evaluation is `--denoise-quality`)."""
-    fr.ensure_cnn(6)
-    fr.ensure_cnn(2)
+    from vulkanpbrt_amd import denoiser_train as dt
+    kinds = [6, 2] + (["c"] if dt.COMPACT_WEIGHTS.exists() else [])
+    for k in kinds:
+        fr.ensure_cnn(k)
     out = {}
-    for name, fn in (("bmfr", fr.bmfr), ("cnn6", lambda f: fr.cnn(f, 6)), ("cnn2", lambda f: fr.cnn(f, 2))):
+    for name, fn in [("bmfr", fr.bmfr)] + [(f"cnn{k}", lambda f, k=k: fr.cnn(f, k)) for k in kinds]:
         fn(frame)  # warm-up
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -145,15 +147,16 @@ def denoise_comparison(torch, fr, frame, runs, evaluate):
         b.record()
         torch.cuda.synchronize()
         out[f"{name}_ms"] = round(a.elapsed_time(b) / runs, 4)
-    out["cnn2_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn2_ms"], 3)
-    out["cnn6_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out["cnn6_ms"], 3)
+    for k in kinds:
+        out[f"cnn{k}_speedup_vs_bmfr"] = round(out["bmfr_ms"] / out[f"cnn{k}_ms"], 3)
     if evaluate:
-        from vulkanpbrt_amd import denoiser_train as dt
-        q = dt.sequence_quality(torch, fr, 8, (0, 7), 1024)
-        out["relmse"] = {k: round(v, 5) for k, v in q["mean"].items() if k in (
-            "accumulated", "bmfr", "cnn6", "cnn2", "bmfr_demodulated", "cnn6_demodulated", "cnn2_demodulated")}
-        out["cnn2_relmse_vs_bmfr"] = round(q["mean"]["cnn2"] / q["mean"]["bmfr"], 3)
-        out["cnn2_at_or_below_bmfr_relmse"] = bool(q["mean"]["cnn2"] <= q["mean"]["bmfr"])
+        q = dt.sequence_quality(torch, fr, 8, (0, 7), 1024, layers=tuple(kinds))
+        names = ["accumulated", "bmfr"] + [f"cnn{k}" for k in kinds]
+        out["relmse"] = {k: round(v, 5) for k, v in q["mean"].items()
+                         if k in names or k in [n + "_demodulated" for n in names[1:]]}
+        for k in kinds:
+            out[f"cnn{k}_relmse_vs_bmfr"] = round(q["mean"][f"cnn{k}"] / q["mean"]["bmfr"], 3)
+            out[f"cnn{k}_at_or_below_bmfr_relmse"] = bool(q["mean"][f"cnn{k}"] <= q["mean"]["bmfr"])
         out["eval_set"] = ("the timed workload's first 8 camera-path frames, frames 0 and 7 scored against 1024-spp "
                            "references; relMSE of the remodulated radiance (and of the demodulated radiance), "
                            "mean of the two frames")
@@ -242,14 +245,17 @@ def main():
         from vulkanpbrt_amd import denoiser_train as dt
         score = sorted({int(s) for s in args.quality_score.split(",")})
         t0 = time.perf_counter()
-        q = dt.sequence_quality(torch, fr, args.quality_frames, score, args.quality_spp, progress=True)
+        kinds = (6, 2) + (("c",) if dt.COMPACT_WEIGHTS.exists() else ())
+        q = dt.sequence_quality(torch, fr, args.quality_frames, score, args.quality_spp, layers=kinds,
+                                progress=True)
         print(json.dumps({"denoise_quality": {
             "workload": cfg["name"], "frames": args.quality_frames, "scored_frames": score,
             "reference_spp": args.quality_spp, "metric": "relMSE of the remodulated radiance "
             "(albedo x demodulated), mean over pixels and rgb of (x - ref)^2 / (ref^2 + 0.01); "
             "*_demodulated: of the demodulated radiance",
-            "networks": "6-layer and fused 2-layer kernel-predicting networks, committed parameters "
-                        "trained on Cornell-box scenes", "seconds": round(time.perf_counter() - t0, 1),
+            "networks": "6-layer, fused 2-layer and compact (3x3 then pointwise) kernel-predicting networks "
+                        "(cnn6, cnn2, cnnc), committed parameters trained on Cornell-box and atrium scenes of "
+                        "other seeds", "seconds": round(time.perf_counter() - t0, 1),
             "mean": {n: round(v, 5) for n, v in q["mean"].items()},
             "per_frame": {str(f): {n: round(v, 5) for n, v in row.items()} for f, row in q["frames"].items()}}}))
         return

@@ -550,6 +550,14 @@ typedef struct pbrt_cnn pbrt_cnn;
  * reference; BMFR::add_dispatch_to_command_graph, BMFR.cpp:206-229, is what it
  * stands in for).  Parameter count and order follow the layer count. */
 int32_t pbrt_cnn_create_layers(uint32_t layers, uint64_t seed, pbrt_cnn** out_net);
+/* The compact network: two layers whose second convolution is pointwise (1x1; stored as
+ * a 3x3 one whose eight outer taps stay zero: set_parameters zeroes them, training masks
+ * their gradient), so inference needs no hidden halo and runs as one fused kernel that
+ * keeps both layers in registers per 16-pixel row: the fastest learned stand-in for
+ * BMFR (BMFR::add_dispatch_to_command_graph, BMFR.cpp:206-229; no counterpart in the
+ * reference).  Parameters as for two layers; pbrt_cnn_layers reports 2. */
+int32_t pbrt_cnn_create_compact(uint64_t seed, pbrt_cnn** out_net);
+int32_t pbrt_cnn_is_compact(const pbrt_cnn* net, uint32_t* out_compact);
 int32_t pbrt_cnn_layers(const pbrt_cnn* net, uint32_t* out_layers);
 int32_t pbrt_cnn_layers_parameter_count(uint32_t layers, uint32_t* out_count);
 

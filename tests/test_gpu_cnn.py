@@ -45,9 +45,14 @@ def synthetic_batch(seed):
 
 class Net:
     def __init__(self, seed=1, layers=6):
+        """layers: 2..6, or "c" for the compact network (two layers, the second pointwise)."""
         self.h = C.c_void_p()
-        self.layers = layers
-        capi.check(capi.lib().pbrt_cnn_create_layers(layers, seed, C.byref(self.h)))
+        if layers == "c":
+            self.layers = 2
+            capi.check(capi.lib().pbrt_cnn_create_compact(seed, C.byref(self.h)))
+        else:
+            self.layers = layers
+            capi.check(capi.lib().pbrt_cnn_create_layers(layers, seed, C.byref(self.h)))
 
     def set(self, params):
         p = np.ascontiguousarray(params, np.float32)
@@ -84,15 +89,29 @@ def oracle_forward(params, inp, layers=6):
     return out, cache
 
 
-@pytest.mark.parametrize("layers", [6, 2])
+def outer_taps(layers):
+    """Canonical indices of the last layer's eight outer taps (W[cout][cin][ky][kx])."""
+    n = cnn.parameter_count(cnn.shape_of(layers))
+    o = n - 25 - 9 * 32 * 25
+    k = np.arange(9 * 32 * 25)
+    return o + k[k % 9 != 4]
+
+
+@pytest.mark.parametrize("layers", [6, 2, "c"])
 def test_forward_matches_oracle(cuda, layers):
     """Six layers: the layer-by-layer path with kernel prediction fused into the last
-    layer; two layers: the single fused kernel (features to filtered radiance in LDS)."""
+    layer; two layers: the single fused kernel (features to filtered radiance in LDS);
+    the compact network ("c": the second layer pointwise, its outer taps zero): its own
+    fused kernel (both layers in registers per 16-pixel row).  The oracle runs the
+    two-layer shape with the same parameters."""
     inp = synthetic_batch(3)
     net = Net(7, layers)
+    nl = net.layers
     params = net.get()
+    if layers == "c":
+        assert not params[outer_taps(2)].any()
     # a non-trivial kernel: larger last-layer weights than the init
-    o = cnn.parameter_count(cnn.shape_of(layers)) - 25 - 9 * 32 * 25
+    o = cnn.parameter_count(cnn.shape_of(nl)) - 25 - 9 * 32 * 25
     params[o:o + 9 * 32 * 25] *= 10.0
     net.set(params)
     np.testing.assert_array_equal(net.get(), params)
@@ -100,7 +119,7 @@ def test_forward_matches_oracle(cuda, layers):
     capi.check(capi.lib().pbrt_cnn_denoise(net.h, C.byref(im), helpers.stream(cuda)))
     cuda.cuda.synchronize()
     g = t["denoised"].cpu().numpy()[..., :3]
-    ref, _ = oracle_forward(params.astype(np.float64), inp, layers)
+    ref, _ = oracle_forward(params.astype(np.float64), inp, nl)
     rel = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3)
     print(f"cnn forward ({layers} layers): rel err median {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}")
     assert np.quantile(rel, 0.99) < 2e-2 and np.median(rel) < 2e-3
@@ -113,10 +132,15 @@ def test_forward_matches_oracle(cuda, layers):
     net.close()
 
 
-@pytest.mark.parametrize("layers", [6, 2])
+@pytest.mark.parametrize("layers", [6, 2, "c"])
 def test_gradients_match_oracle(cuda, layers):
+    """Every layer's gradient against the oracle's; the compact network reports zero
+    gradient for its pointwise layer's outer taps and the oracle's for every other
+    parameter."""
     inp = synthetic_batch(4)
     net = Net(9, layers)
+    compact = layers == "c"
+    layers = net.layers
     params = net.get().astype(np.float64)
     im, t = device_images(cuda, inp, training=True)
     n = cnn.parameter_count(cnn.shape_of(layers))
@@ -130,6 +154,10 @@ def test_gradients_match_oracle(cuda, layers):
     tgt = inp["target"][..., :3].astype(np.float64)
     ref_loss = cnn.loss(out, tgt, inp["albedo"])
     ref = cnn.backward(cache, out, tgt, inp["albedo"])
+    if compact:
+        outer = outer_taps(2)
+        assert not g[outer].any()
+        ref[outer] = 0.0
     print(f"cnn loss gpu {loss.item():.6f} oracle {ref_loss:.6f}")
     assert abs(loss.item() - ref_loss) <= 2e-3 * ref_loss
     o = 0
@@ -189,3 +217,31 @@ def test_invalid_arguments(cuda):
     n = C.c_uint32()
     assert capi.lib().pbrt_cnn_layers(two.h, C.byref(n)) == 0 and n.value == 2
     two.close()
+
+
+def test_compact_training_keeps_pointwise_layer(cuda):
+    """Adam steps on the compact network: the loss falls and the second layer stays
+    pointwise (its outer taps exactly zero, also after set_parameters is handed
+    non-zero ones)."""
+    inp = synthetic_batch(6)
+    net = Net(5, "c")
+    params = net.get()
+    params[outer_taps(2)] = 0.25
+    net.set(params)
+    assert not net.get()[outer_taps(2)].any()
+    im, t = device_images(cuda, inp, training=True)
+    loss = cuda.zeros(1, dtype=cuda.float32, device="cuda")
+    losses = []
+    for _ in range(60):
+        capi.check(capi.lib().pbrt_cnn_train_step(net.h, C.byref(im), C.c_float(2e-3), capi.ptr(loss),
+                                                  helpers.stream(cuda)))
+        cuda.cuda.synchronize()
+        losses.append(loss.item())
+    print(f"compact training: loss {losses[0]:.5f} -> {losses[-1]:.5f}")
+    assert losses[-1] < 0.8 * losses[0]
+    p = net.get()
+    assert not p[outer_taps(2)].any() and p[outer_taps(2)[0] + 4::9][:10].any()
+    flag = C.c_uint32()
+    capi.check(capi.lib().pbrt_cnn_is_compact(net.h, C.byref(flag)))
+    assert flag.value == 1
+    net.close()

@@ -298,6 +298,8 @@ SIGNATURES = {
     "pbrt_running_mean": (C.c_int32, [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "pbrt_cnn_create_layers": (C.c_int32, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pbrt_cnn_layers": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pbrt_cnn_create_compact": (C.c_int32, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pbrt_cnn_is_compact": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pbrt_cnn_layers_parameter_count": (C.c_int32, [C.c_uint32, C.POINTER(C.c_uint32)]),
     "pbrt_frame_graph_create": (C.c_int32, [C.POINTER(C.c_void_p)]),
     "pbrt_frame_graph_destroy": (C.c_int32, [C.c_void_p]),

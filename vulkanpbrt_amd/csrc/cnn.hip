@@ -628,6 +628,76 @@ LayerTable layer_table(int layers) {
     return lt;
 }
 
+// Softmax over one output pixel's 25 bf16 logits (LDS, `logits`: 32 channels, 25
+// used) and the 5x5 weighted sum of its noisy taps; thread `tid` of the 16x16 tile at
+// (x0, y0).  `taps` is the tile's noisy halo (raw FP16 RGBA, clamped to the image)
+// with row length `halo`, offset so that tap (ky, kx) of tile pixel (py, px) sits at
+// (py + ky, px + kx).  Writes the denoised radiance and / or the final image.
+__device__ __forceinline__ void kpn_apply_pixel(const CnnGeom& g, const pbrt_cnn_images& im, const uint16_t* logits,
+                                                const uint2* ntile, int halo, size_t image_base, int x0, int y0,
+                                                int tid) {
+    const int py = tid / kTile, px = tid % kTile;
+    const int gy = y0 + py, gx = x0 + px;
+    if (gx >= g.width || gy >= g.height) return;
+    // the 25 bf16 logits (and 7 padding channels): four 16-byte reads, twice
+    // (maximum, then the weighted sum) to keep few registers live
+    const uint4* lz = reinterpret_cast<const uint4*>(logits);
+    float zmax = -3.4e38f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = lz[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = 8 * q + k;
+            const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
+            if (c < kKernelTaps) zmax = fmaxf(zmax, zc);
+        }
+    }
+    // exp(z - max) = exp2(z log2(e) - max log2(e)); the normalisation is applied once
+    const float l2e = 1.4426950408889634f, zm = zmax * l2e;
+    // the halo holds the clamped noisy pixel at every position, so tap (ky, kx) of
+    // this pixel sits at (py + ky, px + kx) also on border tiles: the 25 reads are
+    // constant offsets from one address
+    const uint2* taps = &ntile[py * halo + px];
+    float sum = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = lz[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int c = 8 * q + k;
+            if (c < kKernelTaps) {
+                const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
+                const float e = __builtin_amdgcn_exp2f(fmaf(zc, l2e, -zm));
+                // FP16 taps widen exactly inside the multiply-add (v_fma_mix_f32)
+                const uint2 nv = taps[(c / 5) * halo + c % 5];
+                const half2v rg = __builtin_bit_cast(half2v, nv.x), ba = __builtin_bit_cast(half2v, nv.y);
+                sum += e;
+                ox = fmaf((float)rg.x, e, ox);
+                oy = fmaf((float)rg.y, e, oy);
+                oz = fmaf((float)ba.x, e, oz);
+            }
+        }
+    }
+    const float inv = __builtin_amdgcn_rcpf(sum);  // 1 ulp
+    ox *= inv;
+    oy *= inv;
+    oz *= inv;
+    const size_t i = image_base + (size_t)gy * g.width + gx;
+    if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(ox, oy, oz, 1.0f);
+    if (im.final_image) {
+        // remodulated, gamma 1/2.2
+        const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
+        const float k = 1.0f / 255.0f;
+        const float tx = fmaxf(((float)ac.x * k + 1e-6f) * ox, 0.0f);
+        const float ty = fmaxf(((float)ac.y * k + 1e-6f) * oy, 0.0f);
+        const float tz = fmaxf(((float)ac.z * k + 1e-6f) * oz, 0.0f);
+        store_bgra8(im.final_image, i, make_vec4(fast_gamma(tx), fast_gamma(ty), fast_gamma(tz), 1.0f));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two-layer network, fused: features -> 3x3 conv 16->32 (ReLU) -> 3x3 conv
 // 32->25 logits -> softmax 5x5 kernel -> filtered radiance, one workgroup per
@@ -767,67 +837,130 @@ __global__ void __launch_bounds__(256, 3) k_two_layer(CnnGeom g, pbrt_cnn_images
         __syncthreads();
 
         // softmax over this thread's pixel's 25 logits and the 5x5 weighted sum
-        const int py = tid / kTile, px = tid % kTile;
-        const int gy = y0 + py, gx = x0 + px;
-        if (gx < g.width && gy < g.height) {
-            // the 25 bf16 logits (and 7 padding channels): four 16-byte reads, twice
-            // (maximum, then the weighted sum) to keep few registers live
-            const uint4* lz = reinterpret_cast<const uint4*>(&buf_f[tid * kHidStride]);
-            float zmax = -3.4e38f;
+        kpn_apply_pixel(g, im, buf_f + tid * kHidStride, ntile, kTwoHalo, plane * img, x0, y0, tid);
+        __syncthreads();  // the next tile's features overwrite the logits and taps
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compact network, fused: features -> 3x3 conv 16->32 (ReLU) -> 1x1 conv 32->25
+// logits -> softmax 5x5 kernel -> filtered radiance (inference only).
+// ---------------------------------------------------------------------------
+// The two-layer network with its second convolution pointwise (pbrt_cnn_create_compact:
+// the eight outer taps of that layer are held at zero in training), which needs no
+// hidden halo: per 16x16 tile the 18x18 feature halo goes to LDS and each 16-pixel
+// row of the tile runs both layers in registers.  The first convolution pairs two
+// taps per v_mfma_f32_16x16x32_bf16 (K = 16 channels of tap 2p, then of tap 2p + 1;
+// 5 instructions per 16 output channels); its ReLU'd, bf16-rounded output is already
+// the B operand of the pointwise layer, whose K order (each lane's four channels of
+// the first 16, then its four of the second 16) the second layer's A operand follows.
+// The logits go to LDS for the per-pixel softmax (kpn_apply_pixel).  Rounding points
+// and biases are those of the two-layer network.
+constexpr int kCompactHalo = kTile + 2;  // features of the 3x3 convolution
+constexpr int kCompactTaps = kTile + 4;  // noisy taps of the 5x5 filter
+
+__global__ void __launch_bounds__(256, 4) k_compact(CnnGeom g, pbrt_cnn_images im, const uint16_t* __restrict__ wf,
+                                                    const float* __restrict__ master) {
+    __shared__ __attribute__((aligned(16))) uint16_t feat[kCompactHalo * kCompactHalo * kFeatStride];
+    __shared__ __attribute__((aligned(16))) uint16_t logits[kTile * kTile * kHidStride];
+    __shared__ __attribute__((aligned(16))) uint2 ntile[kCompactTaps * kCompactTaps];  // noisy taps, raw FP16
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, grp = lane >> 4;
+    const int tiles_x = (g.width + kTile - 1) / kTile, tiles_y = (g.height + kTile - 1) / kTile;
+    const int per_image = tiles_x * tiles_y, ntiles = per_image * g.count;
+    // persistent, XCD-aware tile ranges as in k_two_layer
+    const int slots = gridDim.x / 8, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int chunk = (ntiles + 7) / 8, begin = xcd * chunk, end = min(ntiles, begin + chunk);
+    if (begin + slot >= end) return;
+    const size_t plane = (size_t)g.width * g.height;
+
+    // first layer, A operand of tap pair p (row = output channel ct * 16 + j): lanes of
+    // groups 0 / 1 hold tap 2p's input channels 0-7 / 8-15, groups 2 / 3 tap 2p + 1's
+    // (zero past tap 8)
+    const int pair_tap = grp >> 1, chan = 8 * (grp & 1);
+    uint4 w0[5][2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 v = lz[q];
-                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int pr = 0; pr < 5; ++pr)
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int c = 8 * q + k;
-                    const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
-                    if (c < kKernelTaps) zmax = fmaxf(zmax, zc);
-                }
-            }
-            // exp(z - max) = exp2(z log2(e) - max log2(e)); the normalisation is applied once
-            const float l2e = 1.4426950408889634f, zm = zmax * l2e;
-            // ntile holds the clamped noisy pixel at every halo position, so tap (ky, kx)
-            // of this pixel sits at halo (py + ky, px + kx) also on border tiles: the 25
-            // reads are constant offsets from one address
-            const uint2* taps = &ntile[py * kTwoHalo + px];
-            float sum = 0.0f, ox = 0.0f, oy = 0.0f, oz = 0.0f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 v = lz[q];
-                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int c = 8 * q + k;
-                    if (c < kKernelTaps) {
-                        const float zc = __uint_as_float((k & 1) ? (w[k / 2] & 0xFFFF0000u) : (w[k / 2] << 16));
-                        const float e = __builtin_amdgcn_exp2f(fmaf(zc, l2e, -zm));
-                        // FP16 taps widen exactly inside the multiply-add (v_fma_mix_f32)
-                        const uint2 nv = taps[(c / 5) * kTwoHalo + c % 5];
-                        const half2v rg = __builtin_bit_cast(half2v, nv.x), ba = __builtin_bit_cast(half2v, nv.y);
-                        sum += e;
-                        ox = fmaf((float)rg.x, e, ox);
-                        oy = fmaf((float)rg.y, e, oy);
-                        oz = fmaf((float)ba.x, e, oz);
-                    }
-                }
-            }
-            const float inv = __builtin_amdgcn_rcpf(sum);  // 1 ulp
-            ox *= inv;
-            oy *= inv;
-            oz *= inv;
-            const size_t i = plane * img + (size_t)gy * g.width + gx;
-            if (im.denoised) reinterpret_cast<float4*>(im.denoised)[i] = make_float4(ox, oy, oz, 1.0f);
-            if (im.final_image) {
-                // remodulated, gamma 1/2.2
-                const uchar4 ac = reinterpret_cast<const uchar4*>(im.albedo)[i];
-                const float k = 1.0f / 255.0f;
-                const float tx = fmaxf(((float)ac.x * k + 1e-6f) * ox, 0.0f);
-                const float ty = fmaxf(((float)ac.y * k + 1e-6f) * oy, 0.0f);
-                const float tz = fmaxf(((float)ac.z * k + 1e-6f) * oz, 0.0f);
-                store_bgra8(im.final_image, i, make_vec4(fast_gamma(tx), fast_gamma(ty), fast_gamma(tz), 1.0f));
-            }
+        for (int ct = 0; ct < 2; ++ct) {
+            const int tap = 2 * pr + pair_tap;
+            w0[pr][ct] = tap < kTaps ? *reinterpret_cast<const uint4*>(wf + weight_offset(0) +
+                                                                       (tap * kCh + ct * 16 + j) * kFeatCh + chan)
+                                     : make_uint4(0, 0, 0, 0);
         }
+    // pointwise layer (centre tap), A operand (row = logit ct * 16 + j): K slot
+    // 8 * grp + k is hidden channel 4 * grp + k (k < 4) or 16 + 4 * grp + k - 4
+    uint4 w1[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const uint16_t* row = wf + weight_offset(1) + (4 * kCh + ct * 16 + j) * kCh;
+        const uint2 lo = *reinterpret_cast<const uint2*>(row + 4 * grp);
+        const uint2 hi = *reinterpret_cast<const uint2*>(row + 16 + 4 * grp);
+        w1[ct] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+    const float* b0 = master + layer_offset(0) + layer_weights(0);
+    const float* b1 = master + layer_offset(1) + layer_weights(1);
+    float4v bv0[2], bv1[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            bv0[ct][i] = b0[ct * 16 + 4 * grp + i];
+            bv1[ct][i] = b1[ct * 16 + 4 * grp + i];
+        }
+
+    for (int t = begin + slot; t < end; t += slots) {
+        const int img = t / per_image, rem = t % per_image;
+        const int x0 = (rem % tiles_x) * kTile, y0 = (rem / tiles_x) * kTile;
+        for (int p = tid; p < kCompactHalo * kCompactHalo; p += 256) {
+            const int gx = x0 - 1 + p % kCompactHalo, gy = y0 - 1 + p / kCompactHalo;
+            uint4 lo = make_uint4(0, 0, 0, 0), hi = lo;
+            if (gx >= 0 && gy >= 0 && gx < g.width && gy < g.height)
+                pixel_features(im, plane * img + (size_t)gy * g.width + gx, lo, hi);
+            *reinterpret_cast<uint4*>(&feat[p * kFeatStride]) = lo;
+            *reinterpret_cast<uint4*>(&feat[p * kFeatStride + 8]) = hi;
+        }
+        for (int p = tid; p < kCompactTaps * kCompactTaps; p += 256) {
+            const int gx = x0 - 2 + p % kCompactTaps, gy = y0 - 2 + p / kCompactTaps;
+            const int yy = min(max(gy, 0), g.height - 1), xx = min(max(gx, 0), g.width - 1);
+            ntile[p] = reinterpret_cast<const uint2*>(im.noisy)[plane * img + (size_t)yy * g.width + xx];
+        }
+        __syncthreads();
+        for (int oy = wave; oy < kTile; oy += 4) {
+            // tile row oy, pixel j: the first layer over tap pairs
+            float4v acc[2] = {bv0[0], bv0[1]};
+#pragma unroll
+            for (int pr = 0; pr < 5; ++pr) {
+                const int tap = 2 * pr + pair_tap;
+                bf16x8 bx = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+                if (tap < kTaps)
+                    bx = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                        &feat[((oy + tap / 3) * kCompactHalo + j + tap % 3) *
+                                                                  kFeatStride + chan]));
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w0[pr][ct]), bx,
+                                                                      acc[ct], 0, 0, 0);
+            }
+            // ReLU, bf16: this lane's hidden channels 4 grp .. + 3 and 16 + 4 grp .. + 3 of pixel j
+            const uint4 hid = make_uint4(pack_bf16(fmaxf(acc[0][0], 0.0f), fmaxf(acc[0][1], 0.0f)),
+                                         pack_bf16(fmaxf(acc[0][2], 0.0f), fmaxf(acc[0][3], 0.0f)),
+                                         pack_bf16(fmaxf(acc[1][0], 0.0f), fmaxf(acc[1][1], 0.0f)),
+                                         pack_bf16(fmaxf(acc[1][2], 0.0f), fmaxf(acc[1][3], 0.0f)));
+            const bf16x8 hx = __builtin_bit_cast(bf16x8, hid);
+            float4v z[2] = {bv1[0], bv1[1]};
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                z[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w1[ct]), hx, z[ct], 0, 0,
+                                                                0);
+            const int idx = oy * kTile + j;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+                *reinterpret_cast<uint2*>(&logits[idx * kHidStride + ct * 16 + 4 * grp]) =
+                    make_uint2(pack_bf16(z[ct][0], z[ct][1]), pack_bf16(z[ct][2], z[ct][3]));
+        }
+        __syncthreads();
+        kpn_apply_pixel(g, im, logits + tid * kHidStride, ntile, kCompactTaps, plane * img, x0, y0, tid);
         __syncthreads();  // the next tile's features overwrite the logits and taps
     }
 }
@@ -885,6 +1018,7 @@ struct pbrt_cnn {
     DeviceBuffer master, m, v, wf, wb, grad_map;
     uint64_t step = 0;
     int layers = kMaxLayers;
+    bool compact = false;  // two layers, the second pointwise (its outer taps held at zero)
 };
 
 namespace pbrt {
@@ -936,6 +1070,30 @@ void forward_two_layer(const pbrt_cnn* net, const pbrt_cnn_images& im, hipStream
     PBRT_REQUIRE(tiles < (1u << 30), "pbrt_cnn_denoise: batch too large");
     const dim3 grid = persistent_grid(k_two_layer, (int)tiles);
     hipLaunchKernelGGL(k_two_layer, grid, dim3(256), 0, st, g, im, net->wf.as<uint16_t>(), net->master.as<float>());
+}
+
+// inference of the compact network: one fused kernel, no workspace
+void forward_compact(const pbrt_cnn* net, const pbrt_cnn_images& im, hipStream_t st) {
+    const CnnGeom g{(int)im.width, (int)im.height, (int)im.count};
+    const size_t tiles = (size_t)((im.width + kTile - 1) / kTile) * ((im.height + kTile - 1) / kTile) * im.count;
+    const dim3 grid = persistent_grid(k_compact, (int)tiles);
+    hipLaunchKernelGGL(k_compact, grid, dim3(256), 0, st, g, im, net->wf.as<uint16_t>(), net->master.as<float>());
+}
+
+// zero the padded gradient of every tap but the centre one of layer l (the compact
+// network's pointwise layer): with zero gradient and moments, Adam leaves those
+// weights at zero
+__global__ void __launch_bounds__(256) k_mask_outer_taps(float* grad, int cin_pad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kTaps * kCh * cin_pad) return;
+    if (i / (kCh * cin_pad) != kTaps / 2) grad[i] = 0.0f;
+}
+
+void mask_pointwise_gradient(const pbrt_cnn* net, float* padded_grad, hipStream_t st) {
+    if (!net->compact) return;
+    const int l = net->layers - 1;
+    hipLaunchKernelGGL(k_mask_outer_taps, dim3((layer_weights(l) + 255) / 256), dim3(256), 0, st,
+                       padded_grad + layer_offset(l), cin_pad(l));
 }
 
 // inference: kernel prediction fused into the last layer
@@ -1085,6 +1243,36 @@ extern "C" int32_t pbrt_cnn_create_layers(uint32_t layers, uint64_t seed, pbrt_c
     });
 }
 
+extern "C" int32_t pbrt_cnn_create_compact(uint64_t seed, pbrt_cnn** out) {
+    return guard([&] {
+        PBRT_REQUIRE(out, "pbrt_cnn_create_compact: NULL output");
+        *out = nullptr;
+        pbrt_cnn* net = nullptr;
+        int32_t rc = pbrt_cnn_create_layers(2, seed, &net);
+        if (rc != PBRT_OK) fail(rc, std::string("pbrt_cnn_create_compact: ") + pbrt_last_error());
+        std::unique_ptr<pbrt_cnn> hold(net);
+        net->compact = true;
+        // the two-layer initialisation with the second layer's fan-in of one tap: its
+        // centre weights scaled by 3 (sqrt of the 9 taps), the outer ones zeroed
+        std::vector<float> canon(canonical_count(2));
+        rc = pbrt_cnn_get_parameters(net, canon.data());
+        if (rc != PBRT_OK) fail(rc, std::string("pbrt_cnn_create_compact: ") + pbrt_last_error());
+        const size_t o = (size_t)cout_real(0, 2) * cin_real(0) * kTaps + cout_real(0, 2);
+        for (size_t i = 0; i < (size_t)cout_real(1, 2) * cin_real(1) * kTaps; ++i)
+            canon[o + i] = i % kTaps == kTaps / 2 ? 3.0f * canon[o + i] : 0.0f;
+        rc = pbrt_cnn_set_parameters(net, canon.data());
+        if (rc != PBRT_OK) fail(rc, std::string("pbrt_cnn_create_compact: ") + pbrt_last_error());
+        *out = hold.release();
+    });
+}
+
+extern "C" int32_t pbrt_cnn_is_compact(const pbrt_cnn* net, uint32_t* out) {
+    return guard([&] {
+        PBRT_REQUIRE(net && out, "pbrt_cnn_is_compact: NULL argument");
+        *out = net->compact ? 1u : 0u;
+    });
+}
+
 extern "C" int32_t pbrt_cnn_destroy(pbrt_cnn* net) {
     return guard([&] { delete net; });
 }
@@ -1107,6 +1295,12 @@ extern "C" int32_t pbrt_cnn_set_parameters(pbrt_cnn* net, const float* host) {
         std::vector<float> padded(n, 0.0f);
         const std::vector<int> map = canonical_map(net->layers);
         for (size_t i = 0; i < map.size(); ++i) padded[map[i]] = host[i];
+        if (net->compact) {
+            // the pointwise layer keeps only its centre tap
+            const int l = net->layers - 1;
+            for (int i = 0; i < layer_weights(l); ++i)
+                if (i / (kCh * cin_pad(l)) != kTaps / 2) padded[layer_offset(l) + i] = 0.0f;
+        }
         PBRT_HIP(hipMemcpy(net->master.ptr, padded.data(), n * 4, hipMemcpyHostToDevice));
         PBRT_HIP(hipMemset(net->m.ptr, 0, n * 4));
         PBRT_HIP(hipMemset(net->v.ptr, 0, n * 4));
@@ -1130,7 +1324,10 @@ extern "C" int32_t pbrt_cnn_denoise(const pbrt_cnn* net, const pbrt_cnn_images* 
     return guard([&] {
         PBRT_REQUIRE(net, "pbrt_cnn_denoise: NULL network");
         hipStream_t st = (hipStream_t)stream;
-        if (net->layers == 2) {
+        if (net->compact) {
+            check_images(im, false, 0, "pbrt_cnn_denoise");
+            forward_compact(net, *im, st);
+        } else if (net->layers == 2) {
             check_images(im, false, 0, "pbrt_cnn_denoise");
             forward_two_layer(net, *im, st);
         } else {
@@ -1153,6 +1350,7 @@ extern "C" int32_t pbrt_cnn_gradients(const pbrt_cnn* net, const pbrt_cnn_images
             layout(im->width, im->height, im->count, true, net->layers, static_cast<char*>(im->workspace));
         forward(net, *im, k, st);
         backward(net, *im, k, loss, st);
+        mask_pointwise_gradient(net, k.grad, st);
         const int n = canonical_count(net->layers);
         hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, k.grad, net->grad_map.as<int>(), n,
                            grad);
@@ -1171,6 +1369,7 @@ extern "C" int32_t pbrt_cnn_train_step(pbrt_cnn* net, const pbrt_cnn_images* im,
             layout(im->width, im->height, im->count, true, net->layers, static_cast<char*>(im->workspace));
         forward(net, *im, k, st);
         backward(net, *im, k, loss, st);
+        mask_pointwise_gradient(net, k.grad, st);
         net->step += 1;
         const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)net->step));
         const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)net->step));

@@ -45,15 +45,27 @@ def relmse(x, ref, albedo=None):
 WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
 DEFAULT_WEIGHTS = WEIGHTS_DIR / "kpn6.bin"  # 6 layers (the learned-denoiser configuration)
 TWO_LAYER_WEIGHTS = WEIGHTS_DIR / "kpn2.bin"  # 2 layers, fused
+COMPACT_WEIGHTS = WEIGHTS_DIR / "kpnc.bin"  # 2 layers, the second pointwise, fused
+
+
+def weights_for(kind):
+    """Committed parameters of a network kind: 6, 2 or "c" (compact)."""
+    return {6: DEFAULT_WEIGHTS, 2: TWO_LAYER_WEIGHTS, "c": COMPACT_WEIGHTS}[kind]
 
 
 class Network:
-    """A pbrt_cnn handle (2..6 layers) with host parameter I/O (float32, canonical order)."""
+    """A pbrt_cnn handle (2..6 layers, or "c": the compact two-layer network whose second
+    convolution is pointwise) with host parameter I/O (float32, canonical order)."""
 
-    def __init__(self, seed: int = 1, layers: int = 6):
+    def __init__(self, seed: int = 1, layers=6):
         self.handle = C.c_void_p()
-        self.layers = layers
-        capi.check(capi.lib().pbrt_cnn_create_layers(layers, seed, C.byref(self.handle)))
+        self.kind = layers
+        if layers == "c":
+            self.layers = 2
+            capi.check(capi.lib().pbrt_cnn_create_compact(seed, C.byref(self.handle)))
+        else:
+            self.layers = int(layers)
+            capi.check(capi.lib().pbrt_cnn_create_layers(self.layers, seed, C.byref(self.handle)))
 
     def parameter_count(self) -> int:
         n = C.c_uint32()
@@ -484,7 +496,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--layers", type=int, nargs="+", default=[6], help="convolutions (2..6), one network each")
+    ap.add_argument("--layers", nargs="+", default=["6"],
+                    help="convolutions (2..6) or c (the compact network: 3x3 then pointwise), one network each")
     ap.add_argument("--out", type=Path, nargs="+", required=True,
                     help="trained parameters (.bin raw float32 or .npy), one per --layers entry")
     args = ap.parse_args()
@@ -507,6 +520,7 @@ def main():
     tr = concat_pairs(torch, sets) if len(sets) > 1 else sets[0]
     t_render = time.perf_counter() - t0
     for layers, out in zip(args.layers, args.out):
+        layers = layers if layers == "c" else int(layers)
         net = Network(args.seed, layers)
         before = evaluate(torch, net, ev)
         print(f"{layers} layers, before training", before, flush=True)

@@ -10,6 +10,10 @@ import ctypes as C
 import numpy as np
 
 
+# learned denoisers by frame option: 6 layers, the fused 2-layer and the compact network
+CNN_KINDS = {"cnn": 6, "cnn2": 2, "cnnc": "c"}
+
+
 class Frame:
     """Device images and the per-frame launch sequence (all on torch's current stream)."""
 
@@ -52,17 +56,17 @@ class Frame:
         self.img_sets = None  # trace outputs, double-buffered by run_pipelined
         self.graph_nodes, self.graph_instantiations = 0, 0
         self.nets = {}
-        if denoiser in ("cnn", "cnn2"):
-            self.ensure_cnn(2 if denoiser == "cnn2" else 6)
+        if denoiser in CNN_KINDS:
+            self.ensure_cnn(CNN_KINDS[denoiser])
 
     def ensure_cnn(self, layers=6):
-        """A learned denoiser with its committed trained parameters: the 6-layer network
-        or the fused 2-layer one."""
+        """A learned denoiser with its committed trained parameters: the 6-layer network,
+        the fused 2-layer one or the compact one ("c")."""
         if layers in self.nets:
             return self.nets[layers]
-        from vulkanpbrt_amd.denoiser_train import DEFAULT_WEIGHTS, TWO_LAYER_WEIGHTS, Network
+        from vulkanpbrt_amd.denoiser_train import Network, weights_for
         net = Network(1, layers)
-        net.load(TWO_LAYER_WEIGHTS if layers == 2 else DEFAULT_WEIGHTS)
+        net.load(weights_for(layers))
         self.nets[layers] = net
         if not hasattr(self, "cnn_ws"):
             cn = C.c_size_t()
@@ -179,8 +183,8 @@ class Frame:
         capi.check(capi.lib().pbrt_cnn_denoise(self.nets[layers].handle, C.byref(im), self.stream()))
 
     def denoise(self, frame):
-        if self.denoiser in ("cnn", "cnn2"):
-            self.cnn(frame, 2 if self.denoiser == "cnn2" else 6)
+        if self.denoiser in CNN_KINDS:
+            self.cnn(frame, CNN_KINDS[self.denoiser])
         else:
             self.bmfr(frame)
 
