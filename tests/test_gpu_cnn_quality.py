@@ -1,7 +1,7 @@
 """What the trained networks gain.
 
 On held-out Cornell-box scenes (seeds never used in training), the committed parameters
-(vulkanpbrt_amd/weights/kpn6.bin, kpn2.bin) against the method without
+(vulkanpbrt_amd/weights/kpn6.bin, kpn2.bin, kpnc.bin) against the method without
 learned kernels - a hand-designed 5x5 kernel guided by the same G-buffer features,
 its bandwidths tuned on the evaluation scenes - and against the 1-spp input and BMFR,
 by relMSE of the remodulated radiance against 256-spp references rendered by the
@@ -11,7 +11,7 @@ neighbours.)
 
 On the atrium's camera sequence (the timed workload's scene and camera path, at
 320x180), the frame pipeline's denoisers on the same accumulated input: BMFR with its
-temporal history against both networks, and each of them on the frame's own 1-spp
+temporal history against the networks, and each of them on the frame's own 1-spp
 trace alone, scored against 1024-spp references.  (The networks were trained on
 Cornell-box scenes and on atrium scenes of other seeds.)"""
 import numpy as np
@@ -62,19 +62,21 @@ def guided(cornell_pairs):
     return best_guided(cornell_pairs)
 
 
-@pytest.mark.parametrize("layers", [6, 2])
+@pytest.mark.parametrize("layers", [6, 2, "c"])
 def test_trained_network_beats_guided_filter_noisy_and_bmfr(cuda, cornell_pairs, guided, layers):
     """The learned per-pixel kernels against the best hand-tuned feature-guided kernel of
     the same 5x5 footprint, the 1-spp input and BMFR (6 layers: kpn6.bin; the
-    fused 2 layers: kpn2.bin)."""
+    fused 2 layers: kpn2.bin; the compact network, 3x3 then pointwise: kpnc.bin)."""
     net = dt.Network(1, layers=layers)
-    net.load(dt.DEFAULT_WEIGHTS if layers == 6 else dt.TWO_LAYER_WEIGHTS)
+    net.load(dt.weights_for(layers))
     t = dt.evaluate(cuda, net, cornell_pairs)
     noisy = cornell_pairs.noisy.cpu().numpy().view(np.float16).astype(np.float32)
     box = dt.relmse(dt.box5(noisy), cornell_pairs.target.cpu().numpy(), cornell_pairs.albedo.cpu().numpy())
     print(f"{layers} layers: relMSE noisy {t['relmse_noisy']:.4f} bmfr {t['relmse_bmfr']:.4f} box {box:.1f} "
           f"guided {guided[0]:.4f} (bandwidths {guided[1]}) trained {t['relmse_cnn']:.4f}")
-    assert t["relmse_cnn"] < 0.8 * guided[0]
+    # the compact network (3x3 receptive field for its kernels, against 5x5 for two
+    # 3x3 layers) gains less over the guided filter: measured 0.85 of its relMSE
+    assert t["relmse_cnn"] < (0.9 if layers == "c" else 0.8) * guided[0]
     assert t["relmse_cnn"] < 0.25 * t["relmse_noisy"]
     assert t["relmse_cnn"] < 0.5 * t["relmse_bmfr"]
     net.close()
@@ -84,26 +86,28 @@ def test_atrium_sequence_denoisers(cuda, pbrt_lib):
     """Eight frames of the timed atrium's camera path (seed 2, never used in training)
     through accumulate -> BMFR / networks: the accumulator's history lowers the input's
     error by frame 7, every denoiser removes most of the remaining error, and both
-    networks halve BMFR's, with its temporal history and without."""
+    6- and 2-layer networks halve BMFR's, with its temporal history and without; the
+    compact network stays below BMFR's."""
     from vulkanpbrt_amd import camera, capi
     from vulkanpbrt_amd.frame import Frame
     gen = capi.GeneratedScene(2, 2)
     gen.desc.bvh_builder = capi.PBRT_BVH_SAH_HOST
     scene = capi.Scene(gen.desc, keepalive=(gen,))
     fr = Frame(cuda, capi, camera, gen, scene, 320, 180, 4, "megakernel", "gram")
-    q = dt.sequence_quality(cuda, fr, 8, (0, 7), 1024)
+    q = dt.sequence_quality(cuda, fr, 8, (0, 7), 1024, layers=(6, 2, "c"))
     for f, row in q["frames"].items():
         print(f"frame {f}: " + " ".join(f"{k} {v:.4f}" for k, v in row.items()))
     first, last = q["frames"][0], q["frames"][7]
     assert last["mean_accumulated_spp"] > 2.0
     assert last["accumulated"] < 0.75 * last["traced"]
     for row in (first, last):
-        for name in ("bmfr", "cnn6", "cnn2"):
+        for name in ("bmfr", "cnn6", "cnn2", "cnnc"):
             assert row[name] < 0.3 * row["accumulated"], name
             assert row[f"{name}_single"] < 0.3 * row["traced"], name
         for name in ("cnn6", "cnn2"):
             assert row[name] < 0.5 * row["bmfr"], name
             assert row[f"{name}_single"] < 0.5 * row["bmfr_single"], name
+        assert row["cnnc"] < row["bmfr"] and row["cnnc_single"] < row["bmfr_single"]
     for n in fr.nets.values():
         n.close()
     scene.close()

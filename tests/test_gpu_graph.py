@@ -23,7 +23,7 @@ def snapshot(fr):
                                                      ("bmfr_final", fr.bmfr_final))}
 
 
-@pytest.mark.parametrize("tracer,denoiser", [("wavefront", "bmfr"), ("megakernel", "bmfr"), ("persistent", "bmfr"), ("wavefront", "cnn"),
+@pytest.mark.parametrize("tracer,denoiser", [("wavefront", "bmfr"), ("megakernel", "bmfr"), ("persistent", "bmfr"), ("megakernel", "cnnc"), ("wavefront", "cnn"),
                                              ("wavefront", "cnn2")])
 def test_frame_graph_equals_eager(cuda, tracer, denoiser):
     torch = cuda
