@@ -983,7 +983,7 @@ __global__ void __launch_bounds__(256, 3) k_compact(CnnGeom g, pbrt_cnn_images i
         }
         if (t + slots < end) fetch(t + slots);
         __syncthreads();
-        for (int oy = wave; oy < kTile; oy += 4) {
+        for (int oy = 4 * wave; oy < 4 * wave + 4; ++oy) {
             // tile row oy, pixel j: the first layer over tap pairs
             float4v acc[2] = {bv0[0], bv0[1]};
 #pragma unroll
@@ -1016,9 +1016,10 @@ __global__ void __launch_bounds__(256, 3) k_compact(CnnGeom g, pbrt_cnn_images i
                 *reinterpret_cast<uint2*>(&logits[idx * kHidStride + ct * 16 + 4 * grp]) =
                     make_uint2(pack_bf16(z[ct][0], z[ct][1]), pack_bf16(z[ct][2], z[ct][3]));
         }
-        // each wave filters the 64 pixels whose logits it wrote (rows wave, wave + 4, ...):
+        // each wave filters the 64 pixels whose logits it wrote (rows 4 wave .. 4 wave + 3,
+        // lane = pixel: the noisy-tap reads of its four rows fall in distinct LDS banks):
         // LDS operations of one wave complete in order, so no workgroup barrier here
-        const int own = (wave + 4 * (lane >> 4)) * kTile + (lane & 15);
+        const int own = 64 * wave + lane;
         kpn_apply_pixel(g, im, logits + own * kHidStride, ntile, kCompactTaps, plane * img, x0, y0, own,
                         &albedo[(own / kTile + 1) * kCompactHalo + own % kTile + 1]);
         __syncthreads();  // the next tile's features overwrite the logits and taps
