@@ -376,7 +376,21 @@ void LearnedDenoiser::compile(Context&) {
         throw Exception(PBRT_ERR_INVALID_ARG, "LearnedDenoiser: cannot read " + _weights_path);
     if (_net) (void)pbrt_cnn_destroy(_net);
     _net = nullptr;
-    check(pbrt_cnn_create_layers(layers, 1, &_net));
+    // two layers whose second convolution has only its centre tap (weights/kpnc.bin) run
+    // as the compact network: the same function on its faster fused kernel
+    bool pointwise = layers == 2;
+    if (pointwise) {
+        uint32_t first = 0;
+        check(pbrt_cnn_layers_parameter_count(2, &first));
+        const size_t second = 25 * 32 * 9 + 25;  // W[25][32][3][3], b[25]
+        first -= (uint32_t)second;
+        for (size_t i = 0; i < 25 * 32 * 9 && pointwise; ++i)
+            if (i % 9 != 4 && params[first + i] != 0.0f) pointwise = false;
+    }
+    if (pointwise)
+        check(pbrt_cnn_create_compact(1, &_net));
+    else
+        check(pbrt_cnn_create_layers(layers, 1, &_net));
     check(pbrt_cnn_set_parameters(_net, params.data()));
     size_t bytes = 0;
     check(pbrt_cnn_workspace_size(_width, _height, 1, 0, &bytes));

@@ -181,7 +181,8 @@ public:
         return std::make_shared<LearnedDenoiser>(w, h, g, i, a);
     }
     // weights/kpn6.bin beside libpbrt.so (6 layers); a file of 2-layer
-    // parameters (weights/kpn2.bin) selects the fused two-layer network
+    // parameters (weights/kpn2.bin) selects the fused two-layer network, and one
+    // whose second layer holds only centre taps (weights/kpnc.bin) the compact one
     static std::string default_weights_path();
     void compile(Context& context);
     void update_image_layouts(Context&) {}
