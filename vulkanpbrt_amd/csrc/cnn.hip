@@ -964,6 +964,10 @@ __global__ void __launch_bounds__(256, 3) k_compact(CnnGeom g, pbrt_cnn_images i
             }
         }
     };
+    // the feature tile's padding (channels 16..23 of every pixel) stays zero: the B
+    // operand past the last tap reads it
+    for (int p = tid; p < kCompactHalo * kCompactHalo; p += 256)
+        *reinterpret_cast<uint4*>(&feat[p * kFeatStride + kFeatCh]) = make_uint4(0, 0, 0, 0);
     fetch(begin + slot);
 
     for (int t = begin + slot; t < end; t += slots) {
@@ -988,22 +992,23 @@ __global__ void __launch_bounds__(256, 3) k_compact(CnnGeom g, pbrt_cnn_images i
             float4v acc[2] = {bv0[0], bv0[1]};
 #pragma unroll
             for (int pr = 0; pr < 5; ++pr) {
+                // past tap 8 the lane reads its pixel's zero padding (LDS bytes 32..47)
                 const int tap = 2 * pr + pair_tap;
-                bf16x8 bx = __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
-                if (tap < kTaps)
-                    bx = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
-                                                        &feat[((oy + tap / 3) * kCompactHalo + j + tap % 3) *
-                                                                  kFeatStride + chan]));
+                const int off = tap < kTaps ? ((oy + tap / 3) * kCompactHalo + j + tap % 3) * kFeatStride + chan
+                                            : ((oy + 1) * kCompactHalo + j + 1) * kFeatStride + kFeatCh;
+                const bf16x8 bx = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&feat[off]));
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct)
                     acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w0[pr][ct]), bx,
                                                                       acc[ct], 0, 0, 0);
             }
-            // ReLU, bf16: this lane's hidden channels 4 grp .. + 3 and 16 + 4 grp .. + 3 of pixel j
-            const uint4 hid = make_uint4(pack_bf16(fmaxf(acc[0][0], 0.0f), fmaxf(acc[0][1], 0.0f)),
-                                         pack_bf16(fmaxf(acc[0][2], 0.0f), fmaxf(acc[0][3], 0.0f)),
-                                         pack_bf16(fmaxf(acc[1][0], 0.0f), fmaxf(acc[1][1], 0.0f)),
-                                         pack_bf16(fmaxf(acc[1][2], 0.0f), fmaxf(acc[1][3], 0.0f)));
+            // ReLU (one v_med3_f32 each), bf16: this lane's hidden channels 4 grp .. + 3 and
+            // 16 + 4 grp .. + 3 of pixel j
+            auto relu = [](float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, INFINITY); };
+            const uint4 hid = make_uint4(pack_bf16(relu(acc[0][0]), relu(acc[0][1])),
+                                         pack_bf16(relu(acc[0][2]), relu(acc[0][3])),
+                                         pack_bf16(relu(acc[1][0]), relu(acc[1][1])),
+                                         pack_bf16(relu(acc[1][2]), relu(acc[1][3])));
             const bf16x8 hx = __builtin_bit_cast(bf16x8, hid);
             float4v z[2] = {bv1[0], bv1[1]};
 #pragma unroll
