@@ -360,7 +360,9 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr float kSpatialAlpha = 1e-3f;
+// alpha 1e-4 (1e-3 before): on the atrium spatial splits then reach 12 % more nodes and
+// the frame's trace 1 % less time; 1e-5 with a budget of 2 gains nothing more
+constexpr float kSpatialAlpha = 1e-4f;
 constexpr float kSpatialBudget = 1.5f;
 
 struct Ref {

@@ -421,7 +421,9 @@ extern "C" int32_t pbrt_scene_create(const pbrt_scene_desc* desc, pbrt_scene** o
             s->info.bvh_builder = PBRT_BVH_PLOC_GPU;
         } else {
             const bool spatial = desc->bvh_builder == PBRT_BVH_SBVH_HOST;
-            s->bvh = spatial ? build_bvh_sbvh(ps.world_tris.data(), ntri, 4, kTraversalStackSize)
+            // spatial-split leaves of at most 2 triangles (atrium trace: 4.23 ms against 4.29 with
+            // 4, 4.25 with 3, 4.93 with 1)
+            s->bvh = spatial ? build_bvh_sbvh(ps.world_tris.data(), ntri, 2, kTraversalStackSize)
                              : build_bvh_sah(lo.data(), hi.data(), ntri, 4, kTraversalStackSize);
             PBRT_REQUIRE(s->bvh.depth <= (uint32_t)kTraversalStackSize,
                          "pbrt_scene_create: BVH deeper than the traversal stack");
