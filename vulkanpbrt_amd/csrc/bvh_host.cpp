@@ -126,7 +126,8 @@ void emit_layout(const std::vector<BuildNode>& nodes, bool nonempty, BvhHost& ou
 
 }  // namespace
 
-BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int max_leaf, int max_depth) {
+BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int max_leaf, int max_depth,
+                      const float* weight) {
     BvhHost out;
     out.tri_order.resize(n);
     for (uint32_t i = 0; i < n; ++i) out.tri_order[i] = i;
@@ -193,12 +194,15 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
             int best_split = -1, best_axis = axis;
             Box bins[3][kBins];
             uint32_t counts[3][kBins];
+            float weights[3][kBins];
             auto bin_axis = [&](int ax) {
                 Box* bin_box = bins[ax];
                 uint32_t* bin_cnt = counts[ax];
+                float* bin_w = weights[ax];
                 for (int b = 0; b < kBins; ++b) {
                     bin_box[b].reset();
                     bin_cnt[b] = 0;
+                    bin_w[b] = 0.0f;
                 }
                 const float scale = (float)kBins / ext[ax], base = cbox.lo[ax];
                 for (uint32_t i = t.begin; i < t.end; ++i) {
@@ -209,6 +213,7 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
                     std::memcpy(tb.hi, tri_hi + 3 * (size_t)tri, sizeof(tb.hi));
                     bin_box[b].grow(tb);
                     bin_cnt[b]++;
+                    bin_w[b] += weight ? weight[tri] : 1.0f;
                 }
             };
             for (int ax = 0; ax < 3; ++ax)
@@ -217,24 +222,30 @@ BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int 
                 if (!(ext[ax] > 0.0f)) continue;
                 const Box* bin_box = bins[ax];
                 const uint32_t* bin_cnt = counts[ax];
-                float right_area[kBins];
+                const float* bin_w = weights[ax];
+                float right_area[kBins], right_w[kBins];
                 uint32_t right_cnt[kBins];
                 Box acc;
                 acc.reset();
                 uint32_t c = 0;
+                float wsum = 0.0f;
                 for (int b = kBins - 1; b > 0; --b) {
                     acc.grow(bin_box[b]);
                     c += bin_cnt[b];
+                    wsum += bin_w[b];
                     right_area[b] = acc.area();
                     right_cnt[b] = c;
+                    right_w[b] = wsum;
                 }
                 acc.reset();
                 c = 0;
+                wsum = 0.0f;
                 for (int b = 0; b < kBins - 1; ++b) {
                     acc.grow(bin_box[b]);
                     c += bin_cnt[b];
+                    wsum += bin_w[b];
                     if (c == 0 || right_cnt[b + 1] == 0) continue;
-                    float cost = acc.area() * (float)c + right_area[b + 1] * (float)right_cnt[b + 1];
+                    float cost = acc.area() * wsum + right_area[b + 1] * right_w[b + 1];
                     if (cost < best) {
                         best = cost;
                         best_split = b;

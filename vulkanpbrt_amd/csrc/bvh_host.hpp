@@ -15,8 +15,11 @@ struct BvhHost {
     uint32_t depth = 0;               // levels of inner nodes + leaf level
 };
 
-// tri_lo / tri_hi: 3 floats per triangle (axis-aligned bounds).
-BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int max_leaf, int max_depth);
+// tri_lo / tri_hi: 3 floats per triangle (axis-aligned bounds).  `weight` (optional):
+// each primitive's intersection cost in triangle tests, for primitives that are
+// whole subtrees (the GPU PLOC builder's clusters); unit cost when NULL.
+BvhHost build_bvh_sah(const float* tri_lo, const float* tri_hi, uint32_t n, int max_leaf, int max_depth,
+                      const float* weight = nullptr);
 // tris: 9 floats per triangle (three vertices); spatial splits may reference a
 // triangle from several leaves.
 BvhHost build_bvh_sbvh(const float* tris, uint32_t n, int max_leaf, int max_depth);

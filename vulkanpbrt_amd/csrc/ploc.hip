@@ -28,11 +28,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
 #include "capi_common.hpp"
 #include "gpu_sort.hpp"
+#include "bvh_host.hpp"
 #include "ploc.hpp"
 #include "scene_types.hpp"
 
@@ -42,6 +46,9 @@ namespace {
 // Neighbours searched on either side (on the 4.9M-triangle city, 8 bounces at 2160p:
 // 11.6 ms per trace with 32, 11.8 with 16, 12.6 with 128; the LDS window holds kMaxRadius).
 constexpr int kRadius = 32;
+// clusters a host SAH top tree starts from on large scenes (city, 8 bounces at 2160p:
+// 9.6 ms per trace with 16384, 9.9 with 4096, 10.4 with 1024; clustering to the root: 11.6)
+constexpr uint32_t kTopClusters = 16384;
 constexpr int kMaxRadius = 128;
 constexpr float kNodeCost = 1.5f;  // an inner-node visit priced in triangle tests (as the host SAH)
 
@@ -199,6 +206,7 @@ struct NodeArrays {
     Box* box;        // the node's box
     uint32_t* info;  // triangle count << 1 | collapse
     float* cost;     // SAH cost times the node's area, as a subtree of leaves <= max_leaf
+    uint8_t* levels; // levels of the emitted subtree, its leaf level included
 };
 
 __device__ __forceinline__ void child_stats(const NodeArrays& nd, int c, const Box& b, uint32_t& count, float& cost) {
@@ -235,6 +243,8 @@ __global__ void __launch_bounds__(256) k_merge(const int* __restrict__ nn, const
         const float inner = kNodeCost * area + sa + sb;
         const float leaf = (float)count * area;
         const bool collapse = (int)count <= max_leaf && leaf <= inner;
+        const int la = cnode[i] < 0 ? 1 : nd.levels[cnode[i]], lb = cnode[j] < 0 ? 1 : nd.levels[cnode[j]];
+        nd.levels[id] = (uint8_t)min(255, collapse ? 1 : 1 + max(la, lb));
         nd.child[id] = make_int2(cnode[i], cnode[j]);
         nd.box[id] = u;
         nd.info[id] = count << 1 | (collapse ? 1u : 0u);
@@ -271,6 +281,48 @@ __device__ __forceinline__ uint32_t leaf_code(uint32_t first, uint32_t count) {
 __device__ __forceinline__ Box ch_box(const NodeArrays& nd, int c, const float* __restrict__ tris,
                                      const uint32_t* __restrict__ order) {
     return c < 0 ? triangle_box(tris + 9 * (size_t)order[~c]) : nd.box[c];
+}
+
+// The clusters left under a tree built on the host: inner clusters get their place
+// (triangle offset, depth, leaf or inner node) like children of an emitted node;
+// single-triangle clusters are written as records.
+struct ClusterSeed {
+    int node;         // creation number, or ~sorted triangle
+    uint32_t offset;  // first triangle record
+    uint32_t depth;   // level of the cluster's node (root 1)
+    uint32_t pad;
+};
+
+__global__ void __launch_bounds__(256) k_seed_clusters(const ClusterSeed* __restrict__ seeds, int count,
+                                                      NodeArrays nd, EmitArrays em, const float* __restrict__ tris,
+                                                      const uint4* __restrict__ meta,
+                                                      const uint32_t* __restrict__ order, float4* __restrict__ records) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const ClusterSeed c = seeds[k];
+    if (c.node < 0) {
+        write_record(tris, meta, order, (uint32_t)~c.node, c.offset, records);
+        return;
+    }
+    em.offset[c.node] = c.offset;
+    em.depth[c.node] = c.depth;
+    em.state[c.node] = (nd.info[c.node] & 1u) ? 2u : 1u;
+}
+
+// (triangle count << 1 | collapse, emitted levels, SAH cost of the subtree per unit of its
+// box's area) of each remaining cluster, for the host
+__global__ void __launch_bounds__(256) k_cluster_stats(const int* __restrict__ cnode, const Box* __restrict__ cbox,
+                                                      int live, NodeArrays nd, uint4* out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= live) return;
+    const int c = cnode[k];
+    if (c < 0) {
+        out[k] = make_uint4(2u, 1u, __float_as_uint(1.0f), 0u);
+        return;
+    }
+    const float area = half_area(cbox[k]);
+    const float per_area = area > 0.0f ? nd.cost[c] / area : 1.0f;
+    out[k] = make_uint4(nd.info[c], nd.levels[c], __float_as_uint(per_area), 0u);
 }
 
 __global__ void k_emit_root(EmitArrays em, int root) {
@@ -347,6 +399,130 @@ unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+namespace {
+
+// The top of the tree over `live` clusters as the host's binned SAH would build it,
+// with single-cluster leaves, each cluster priced at its subtree's SAH cost per unit of
+// its box's area (depth budget: the stack less the deepest cluster subtree), and the
+// SAH cost of the whole tree that results.
+struct TopPlan {
+    uint32_t live = 0;
+    std::vector<int> node;
+    std::vector<uint4> stats;  // count << 1 | collapse, levels, cost per area (bits)
+    BvhHost top;
+    double cost = 0.0;  // SAH cost (triangle tests x half area) of the tree with this top
+};
+
+TopPlan plan_top(uint32_t live, const DeviceBuffer& cnode, const DeviceBuffer& cbox, const NodeArrays& nd,
+                 int max_depth, hipStream_t st) {
+    TopPlan p;
+    p.live = live;
+    DeviceBuffer stats_d;
+    stats_d.allocate(live * sizeof(uint4));
+    hipLaunchKernelGGL(k_cluster_stats, dim3(blocks_for(live)), dim3(256), 0, st, cnode.as<int>(), cbox.as<Box>(),
+                       (int)live, nd, stats_d.as<uint4>());
+    p.node.resize(live);
+    std::vector<Box> box(live);
+    p.stats.resize(live);
+    PBRT_HIP(hipMemcpyAsync(p.node.data(), cnode.ptr, live * sizeof(int), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipMemcpyAsync(box.data(), cbox.ptr, live * sizeof(Box), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipMemcpyAsync(p.stats.data(), stats_d.ptr, live * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    PBRT_HIP(hipStreamSynchronize(st));
+    auto area = [](const float* lo, const float* hi) {
+        const double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    std::vector<float> lo(3 * (size_t)live), hi(3 * (size_t)live), weight(live);
+    uint32_t deepest = 1;
+    for (uint32_t k = 0; k < live; ++k) {
+        std::memcpy(&weight[k], &p.stats[k].z, 4);
+        for (int a = 0; a < 3; ++a) {
+            lo[3 * (size_t)k + a] = box[k].lo[a];
+            hi[3 * (size_t)k + a] = box[k].hi[a];
+        }
+        deepest = std::max(deepest, p.stats[k].y);
+        p.cost += (double)weight[k] * area(box[k].lo, box[k].hi);
+    }
+    // a top leaf at level L puts its cluster's leaves at level L - 1 + levels
+    const int budget = max_depth + 1 - (int)deepest;
+    if (budget < 2) {
+        p.cost = INFINITY;
+        return p;
+    }
+    p.top = build_bvh_sah(lo.data(), hi.data(), live, 1, budget, weight.data());
+    PBRT_REQUIRE(p.top.nodes.size() / 16 == live - 1, "PLOC: top tree node count mismatch");
+    for (size_t h = 0; h + 1 < live; ++h) {
+        const float* r = &p.top.nodes[16 * h];  // {c0 x, c1 x, (c0 z, c1 z)}: the union of both child boxes
+        const float ulo[3] = {std::min(r[0], r[4]), std::min(r[2], r[6]), std::min(r[8], r[10])};
+        const float uhi[3] = {std::max(r[1], r[5]), std::max(r[3], r[7]), std::max(r[9], r[11])};
+        p.cost += kNodeCost * area(ulo, uhi);
+    }
+    return p;
+}
+
+// Writes a planned top as the first live - 1 output nodes (root = node 0; the clustered
+// nodes take the indices after them), with each cluster's triangle range in
+// depth-first order.  Returns the deepest level among the single-triangle clusters
+// placed here.
+uint32_t apply_top(TopPlan& p, const NodeArrays& nd, const EmitArrays& em, DeviceBuffer& tris, DeviceBuffer& meta,
+                   const uint32_t* order, DeviceBuffer& nodes_out, DeviceBuffer& tris_out, hipStream_t st) {
+    const uint32_t live = p.live;
+    BvhHost& top = p.top;
+    // depth-first triangle offsets of the clusters (leaf order = order of top.tri_order)
+    std::vector<uint32_t> offset(live);
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < live; ++q) {
+        const uint32_t c = top.tri_order[q];
+        offset[c] = run;
+        run += p.stats[c].x >> 1;
+    }
+    const uint32_t inner_total = (uint32_t)(nodes_out.bytes / (4 * sizeof(float4)));
+    std::vector<ClusterSeed> seeds;
+    seeds.reserve(live);
+    uint32_t tri_depth = 0;
+    // walk the top tree from the root: translate leaf children to the clusters' references
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 1u}};  // (node, level)
+    while (!stack.empty()) {
+        const auto [h, level] = stack.back();
+        stack.pop_back();
+        float* rec = &top.nodes[16 * (size_t)h];
+        for (int s = 0; s < 2; ++s) {
+            int32_t code;
+            std::memcpy(&code, &rec[12 + s], 4);
+            if (code >= 0) {
+                stack.push_back({(uint32_t)code, level + 1});
+                continue;
+            }
+            const uint32_t leaf = ~(uint32_t)code;
+            const uint32_t c = top.tri_order[leaf >> kLeafCountBits];
+            const int cn = p.node[c];
+            const uint32_t count = p.stats[c].x >> 1;
+            uint32_t ref;
+            if (cn < 0) {
+                ref = ~(offset[c] << kLeafCountBits);
+                tri_depth = std::max(tri_depth, level + 1);
+            } else if (p.stats[c].x & 1u) {
+                ref = ~((offset[c] << kLeafCountBits) | (count - 1u));
+            } else {
+                ref = inner_total - 1 - (uint32_t)cn;
+            }
+            std::memcpy(&rec[12 + s], &ref, 4);
+            seeds.push_back(ClusterSeed{cn, offset[c], level + 1, 0});
+        }
+    }
+    PBRT_REQUIRE(seeds.size() == live, "PLOC: top tree leaves do not cover the clusters");
+    DeviceBuffer seeds_d;
+    PBRT_HIP(hipMemcpyAsync(nodes_out.ptr, top.nodes.data(), top.nodes.size() * sizeof(float), hipMemcpyHostToDevice,
+                            st));
+    seeds_d.upload(seeds.data(), seeds.size() * sizeof(ClusterSeed));
+    hipLaunchKernelGGL(k_seed_clusters, dim3(blocks_for(live)), dim3(256), 0, st, seeds_d.as<ClusterSeed>(),
+                       (int)live, nd, em, tris.as<float>(), meta.as<uint4>(), order, tris_out.as<float4>());
+    PBRT_HIP(hipStreamSynchronize(st));  // the host vectors and seed buffer go out of scope
+    return std::max(tri_depth, top.depth);
+}
+
+}  // namespace
+
 PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<uint32_t>& inst,
                       const std::vector<uint32_t>& prim, const std::vector<uint32_t>& flags, int max_leaf,
                       int max_depth, DeviceBuffer& tris_out, DeviceBuffer& nodes_out) {
@@ -365,7 +541,7 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
     PBRT_HIP(hipEventCreate(&ev.e[1]));
     const uint32_t inner = n - 1;
     DeviceBuffer tris, meta, keys, vals, keys2, vals2, partial, sort_tmp, cn[2], cb[2], nnb, fl, pre, scan_tmp;
-    DeviceBuffer child, box, info, cost, offset, depth, state, maxd, total;
+    DeviceBuffer child, box, info, cost, levels, offset, depth, state, maxd, total;
     tris.upload(world_tris.data(), world_tris.size() * sizeof(float));
     std::vector<uint4> m(n);
     for (uint32_t i = 0; i < n; ++i) m[i] = make_uint4(inst[i], prim[i], flags[i], 0);
@@ -389,6 +565,7 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
     box.allocate(inner * sizeof(Box));
     info.allocate(inner * sizeof(uint32_t));
     cost.allocate(inner * sizeof(float));
+    levels.allocate(inner);
     offset.allocate(inner * sizeof(uint32_t));
     depth.allocate(inner * sizeof(uint32_t));
     state.allocate(inner * sizeof(uint32_t));
@@ -412,7 +589,7 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
     hipLaunchKernelGGL(k_leaf_clusters, dim3(blocks_for(n)), dim3(256), 0, st, tris.as<float>(), order, n,
                        cn[0].as<int>(), cb[0].as<Box>());
 
-    NodeArrays nd{child.as<int2>(), box.as<Box>(), info.as<uint32_t>(), cost.as<float>()};
+    NodeArrays nd{child.as<int2>(), box.as<Box>(), info.as<uint32_t>(), cost.as<float>(), levels.as<uint8_t>()};
     struct Range {
         int begin, count;
     };
@@ -420,7 +597,22 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
     uint32_t live = n;
     int node_base = 0, cur = 0;
     uint64_t host_total[2] = {0, 0};
+    // large scenes: clustering stops at kTopClusters clusters, whose tree is built by the
+    // host's binned SAH (the top levels, which every ray visits, are where a 1-D window
+    // over the Morton order chooses worst)
+    DeviceBuffer snap_node, snap_box;
+    uint32_t snap_live = 0;
+    size_t snap_ranges = 0;
     while (live > 1) {
+        if (!snap_live && live <= kTopClusters && n > 2 * kTopClusters) {
+            // the clusters a host SAH top would start from
+            snap_live = live;
+            snap_ranges = ranges.size();
+            snap_node.allocate(live * sizeof(int));
+            snap_box.allocate(live * sizeof(Box));
+            PBRT_HIP(hipMemcpyAsync(snap_node.ptr, cn[cur].ptr, live * sizeof(int), hipMemcpyDeviceToDevice, st));
+            PBRT_HIP(hipMemcpyAsync(snap_box.ptr, cb[cur].ptr, live * sizeof(Box), hipMemcpyDeviceToDevice, st));
+        }
         PBRT_REQUIRE(ranges.size() < 4096, "PLOC: clustering did not converge");
         const int nl = (int)live;
         hipLaunchKernelGGL(k_nearest, dim3(blocks_for(live)), dim3(256), 0, st, cb[cur].as<Box>(), nl, kRadius,
@@ -442,10 +634,27 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
         cur = 1 - cur;
     }
     PBRT_REQUIRE(node_base == (int)inner, "PLOC: inner node count mismatch");
-
     EmitArrays em{offset.as<uint32_t>(), depth.as<uint32_t>(), state.as<uint32_t>()};
-    hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, st, em, (int)inner - 1);
-    for (size_t r = ranges.size(); r-- > 0;) {
+    // The tree's top: the clustering's own or, on large scenes, the host SAH's over the
+    // clusters of the snapshot, whichever has the lower SAH cost (the host's wins on the
+    // city, where the last clustering iterations pair big clusters through a narrow
+    // window of the Morton order; the clustering's on the atrium, whose instances of
+    // very different sizes a top-down SAH over cluster boxes splits badly).
+    uint32_t host_depth = 0;
+    size_t emit_ranges = ranges.size();
+    bool host_top = false;
+    if (snap_live > 1) {
+        TopPlan plan = plan_top(snap_live, snap_node, snap_box, nd, max_depth, st);
+        float root_cost = 0.0f;
+        PBRT_HIP(hipMemcpy(&root_cost, nd.cost + (inner - 1), sizeof(float), hipMemcpyDeviceToHost));
+        if (plan.cost < (double)root_cost) {
+            host_depth = apply_top(plan, nd, em, tris, meta, order, nodes_out, tris_out, st);
+            emit_ranges = snap_ranges;  // the nodes clustered after the snapshot are not used
+            host_top = true;
+        }
+    }
+    if (!host_top) hipLaunchKernelGGL(k_emit_root, dim3(1), dim3(1), 0, st, em, (int)inner - 1);
+    for (size_t r = emit_ranges; r-- > 0;) {
         hipLaunchKernelGGL(k_emit_range, dim3(blocks_for(ranges[r].count)), dim3(256), 0, st, ranges[r].begin,
                            ranges[r].count, (int)inner, nd, em, tris.as<float>(), meta.as<uint4>(), order,
                            nodes_out.as<float4>(), tris_out.as<float4>(), maxd.as<uint32_t>());
@@ -457,6 +666,7 @@ PlocResult build_ploc(const std::vector<float>& world_tris, const std::vector<ui
     PBRT_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     uint32_t d = 0;
     PBRT_HIP(hipMemcpy(&d, maxd.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    d = std::max(d, host_depth);
     if ((int)d > max_depth)
         fail(PBRT_ERR_UNSUPPORTED, "pbrt_scene_create: PLOC tree deeper than the traversal stack (" +
                                        std::to_string(d) + " levels); use PBRT_BVH_SAH_HOST for this scene");
