@@ -6,18 +6,21 @@
 // LBVH) emit full-precision nodes; this pass rewrites each node in place of
 // its index as
 //
-//   word 0..2  node origin x, y, z: fp32 rounded towards -inf to 16 significant
-//              bits, whose low 8 bits carry the axis' biased scale exponent
-//              e + 127 (scale s = 2^e)
+//   word 0..2  node origin x, y, z: an fp32 value at or below the node's lower
+//              bound whose low 8 significand bits hold the axis' scale exponent e
+//              as a signed byte (scale s = 2^e); the word is the origin as it
+//              stands, exponent byte included
 //   word 3     child 0 reference (inner node index, or ~leaf code)
 //   word 4..6  12 bytes: child 0 lo.xyz, hi.xyz, child 1 lo.xyz, hi.xyz as
-//              integers q in [0, 255]; the bound is fma(q, s, origin)
+//              integers q in [0, 255]; the bound is origin + q s
 //   word 7     child 1 reference
 //
-// Each child's bounds are rounded outwards and checked with the same fmaf the
-// traversal evaluates, so every dequantized box contains its child's exact box.
-// A node is a function of its own two child boxes only, so the pass is one
-// thread per node.
+// Each child's bounds are rounded outwards and checked in FP64 (where origin + q s
+// is exact unless the scale is below 2^-29 of the origin's own last bit, i.e. for
+// boxes flat to within a fraction of an fp32 ulp), so every box origin + [qlo, qhi] s
+// contains its child's box.  The traversal never forms the bound itself: it meets
+// the plane at fma(q, 2^e / d, fma(origin, 1/d, -o/d)).  A node is a function of
+// its own two child boxes only, so the pass is one thread per node.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -28,25 +31,33 @@ namespace pbrt {
 
 namespace {
 
-// fp32 bits rounded towards -inf with the low 8 significand bits cleared
-__device__ uint32_t origin_bits_down(float f) {
-    uint32_t b = __float_as_uint(f);
+// The origin word of an axis whose lower bound is f and whose scale exponent is e:
+// the fp32 value with e's byte as its low 8 significand bits that is closest to f
+// from below (the byte adds to a negative value's magnitude, to a positive one's
+// value); zero, denormals and values next to them take -2^-126 - byte ulps
+// (normal, so no denormal origin reaches the traversal).
+__device__ uint32_t origin_word(float f, int e) {
+    const uint32_t byte = (uint32_t)e & 0xFFu, b = __float_as_uint(f), low = b & 0xFFu;
     uint32_t t = b & ~0xFFu;
-    if ((b & 0x80000000u) && (b & 0xFFu)) t += 0x100u;  // negative: round the magnitude up
-    return t;
+    if (b & 0x80000000u) {
+        if (byte < low) t += 0x100u;
+    } else if (byte > low) {
+        t -= 0x100u;  // 0 - 0x100 wraps to a NaN pattern, caught below
+    }
+    const bool normal = (t & 0x7F800000u) != 0u && (t & 0x7F800000u) != 0x7F800000u;
+    return (normal ? t : 0x80800000u) | byte;
 }
 
-__device__ __forceinline__ float pow2(int e) { return __uint_as_float((uint32_t)(e + 127) << 23); }
-
-// Quantizes [lo, hi] against origin o and scale 2^e; false if 255 steps do not reach hi.
+// Quantizes [lo, hi] against origin o and scale 2^e: o + qlo 2^e <= lo and
+// o + qhi 2^e >= hi in exact arithmetic; false if 255 steps do not reach hi.
 __device__ bool quantize_range(float lo, float hi, float o, int e, uint32_t& qlo, uint32_t& qhi) {
-    const float s = pow2(e);
-    double a = floor(((double)lo - (double)o) / (double)s);
-    double b = ceil(((double)hi - (double)o) / (double)s);
+    const double s = ldexp(1.0, e), od = (double)o;
+    const double a = floor(((double)lo - od) / s);
+    const double b = ceil(((double)hi - od) / s);
     int ql = (int)fmin(fmax(a, 0.0), 255.0);
     int qh = (int)fmin(fmax(b, 0.0), 256.0);
-    while (ql > 0 && fmaf((float)ql, s, o) > lo) --ql;
-    while (qh <= 255 && fmaf((float)qh, s, o) < hi) ++qh;
+    while (ql > 0 && od + ql * s > (double)lo) --ql;
+    while (qh <= 255 && od + qh * s < (double)hi) ++qh;
     if (qh > 255) return false;
     qlo = (uint32_t)ql;
     qhi = (uint32_t)qh;
@@ -71,12 +82,11 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
                 nhi = fmaxf(nhi, hi[c][a]);
             }
         if (!(nlo <= nhi)) nlo = nhi = 0.0f;  // both children empty: a point box at 0
-        const uint32_t ob = origin_bits_down(nlo);
-        const float o = __uint_as_float(ob);
-        const double ext = (double)nhi - (double)o;
+        const double ext = (double)nhi - (double)nlo;
         int e = ext > 0.0 ? (int)ceil(log2(ext / 254.0)) : -126;
         e = e < -126 ? -126 : (e > 127 ? 127 : e);
         for (;; ++e) {
+            const float o = __uint_as_float(origin_word(nlo, e));
             bool ok = true;
             for (int c = 0; c < 2 && ok; ++c) {
                 if (empty[c]) continue;  // stays the point box (0, 0)
@@ -84,7 +94,7 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
             }
             if (ok || e >= 127) break;
         }
-        w[a] = ob | (uint32_t)(e + 127);
+        w[a] = origin_word(nlo, e);
     }
     auto pack = [](uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) { return b0 | (b1 << 8) | (b2 << 16) | (b3 << 24); };
     out[2 * (size_t)i + 0] = make_uint4(w[0], w[1], w[2], __float_as_uint(n3.x));

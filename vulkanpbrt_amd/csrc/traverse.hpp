@@ -173,10 +173,14 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
     return stack[sp * stride];
 }
 
-// One inner node (quantized layout, 2 x uint4): dequantize both child boxes
-// (bound = fma(q, 2^e, origin), exactly as the quantization pass checked them)
-// and slab-test them against [tmin, tcur]; returns the nearer hit child (the
-// farther one pushed) or the next node off the stack.
+// One inner node (quantized layout, 2 x uint4): slab-test both child boxes
+// against [tmin, tcur]; returns the nearer hit child (the farther one pushed) or
+// the next node off the stack.  A plane at origin + q 2^e along an axis is met at
+// t = (origin + q 2^e - o) / d, evaluated as fma(q, S, C) with S = 2^e / d (exact:
+// a power-of-two scaling) and C = fma(origin, 1/d, -o/d) once per axis and node,
+// so each of the twelve planes costs one multiply-add.  The origin word is used
+// whole (its low byte, the exponent, is part of the origin the quantization pass
+// checked the boxes against).
 //   kPlanes 1 and 2: only the near and far plane of each axis is dequantized,
 //   picked by the ray's octant (with a positive reciprocal the lower bound gives
 //   the smaller t, fma rounding being monotonic; with a negative one the upper),
@@ -191,10 +195,12 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
                                            float tcur, int* stack, int stride, int& sp) {
     const uint4 na = nodes[2 * node + 0];
     const uint4 nb = nodes[2 * node + 1];
-    const float gx = __uint_as_float(na.x & ~0xFFu), gy = __uint_as_float(na.y & ~0xFFu),
-                gz = __uint_as_float(na.z & ~0xFFu);
-    const float sx = __uint_as_float((na.x & 0xFFu) << 23), sy = __uint_as_float((na.y & 0xFFu) << 23),
-                sz = __uint_as_float((na.z & 0xFFu) << 23);
+    // the exponent e is the origin word's low byte, signed
+    const float sx = __builtin_amdgcn_ldexpf(r.ix, (int)(int8_t)na.x),
+                sy = __builtin_amdgcn_ldexpf(r.iy, (int)(int8_t)na.y),
+                sz = __builtin_amdgcn_ldexpf(r.iz, (int)(int8_t)na.z);
+    const float cx = fmaf(__uint_as_float(na.x), r.ix, -r.ox), cy = fmaf(__uint_as_float(na.y), r.iy, -r.oy),
+                cz = fmaf(__uint_as_float(na.z), r.iz, -r.oz);
     auto q = [](uint32_t w, int byte) { return (float)((w >> (8 * byte)) & 0xFFu); };
     float near0, far0, near1, far1;
     if constexpr (kPlanes != 0) {
@@ -210,26 +216,22 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
         const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, sf0);
         const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, sn1);
         const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, sf1);
-        auto tx = [&](uint32_t w) { return fmaf(fmaf(q(w, 0), sx, gx), r.ix, -r.ox); };
-        auto ty = [&](uint32_t w) { return fmaf(fmaf(q(w, 1), sy, gy), r.iy, -r.oy); };
-        auto tz = [&](uint32_t w) { return fmaf(fmaf(q(w, 2), sz, gz), r.iz, -r.oz); };
+        auto tx = [&](uint32_t w) { return fmaf(q(w, 0), sx, cx); };
+        auto ty = [&](uint32_t w) { return fmaf(q(w, 1), sy, cy); };
+        auto tz = [&](uint32_t w) { return fmaf(q(w, 2), sz, cz); };
         near0 = fmaxf(fmaxf(tx(n0), ty(n0)), fmaxf(tz(n0), r.tmin));
         far0 = fminf(fminf(tx(f0), ty(f0)), fminf(tz(f0), tcur)) * 1.00000024f;
         near1 = fmaxf(fmaxf(tx(n1), ty(n1)), fmaxf(tz(n1), r.tmin));
         far1 = fminf(fminf(tx(f1), ty(f1)), fminf(tz(f1), tcur)) * 1.00000024f;
     } else {
-        const float l0x = fmaf(q(nb.x, 0), sx, gx), l0y = fmaf(q(nb.x, 1), sy, gy), l0z = fmaf(q(nb.x, 2), sz, gz);
-        const float h0x = fmaf(q(nb.x, 3), sx, gx), h0y = fmaf(q(nb.y, 0), sy, gy), h0z = fmaf(q(nb.y, 1), sz, gz);
-        const float l1x = fmaf(q(nb.y, 2), sx, gx), l1y = fmaf(q(nb.y, 3), sy, gy), l1z = fmaf(q(nb.z, 0), sz, gz);
-        const float h1x = fmaf(q(nb.z, 1), sx, gx), h1y = fmaf(q(nb.z, 2), sy, gy), h1z = fmaf(q(nb.z, 3), sz, gz);
-        const float a0 = fmaf(l0x, r.ix, -r.ox), a1 = fmaf(h0x, r.ix, -r.ox);
-        const float b0 = fmaf(l0y, r.iy, -r.oy), b1 = fmaf(h0y, r.iy, -r.oy);
-        const float c0 = fmaf(l0z, r.iz, -r.oz), c1 = fmaf(h0z, r.iz, -r.oz);
+        const float a0 = fmaf(q(nb.x, 0), sx, cx), a1 = fmaf(q(nb.x, 3), sx, cx);
+        const float b0 = fmaf(q(nb.x, 1), sy, cy), b1 = fmaf(q(nb.y, 0), sy, cy);
+        const float c0 = fmaf(q(nb.x, 2), sz, cz), c1 = fmaf(q(nb.y, 1), sz, cz);
         near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
         far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
-        const float d0 = fmaf(l1x, r.ix, -r.ox), d1 = fmaf(h1x, r.ix, -r.ox);
-        const float e0 = fmaf(l1y, r.iy, -r.oy), e1 = fmaf(h1y, r.iy, -r.oy);
-        const float f0 = fmaf(l1z, r.iz, -r.oz), f1 = fmaf(h1z, r.iz, -r.oz);
+        const float d0 = fmaf(q(nb.y, 2), sx, cx), d1 = fmaf(q(nb.z, 1), sx, cx);
+        const float e0 = fmaf(q(nb.y, 3), sy, cy), e1 = fmaf(q(nb.z, 2), sy, cy);
+        const float f0 = fmaf(q(nb.z, 0), sz, cz), f1 = fmaf(q(nb.z, 3), sz, cz);
         near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
         far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
     }
