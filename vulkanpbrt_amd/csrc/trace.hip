@@ -23,19 +23,21 @@
 
 namespace pbrt {
 
-template <bool kCount>
+// kPlanes: the node-visit variant (visit_inner); the megakernel fits the four
+// per-ray plane selectors in its 128 registers, the persistent kernel derives three
+template <bool kCount, int kPlanes>
 __device__ __forceinline__ void trace_closest(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax,
                                               int* stack, SurfaceInfo& s, vec3& position, uint32_t& category,
                                               TraversalCounters& cnt) {
     Hit h;
-    if (traverse<false, kCount, 2>(sc, o, d, tmin, tmax, h, stack, kTileThreads, cnt))
+    if (traverse<false, kCount, kPlanes>(sc, o, d, tmin, tmax, h, stack, kTileThreads, cnt))
         shade_hit(sc, h, d, s, position, category);
     else
         miss_surface(s, position, category);
 }
 
 // nextEventEsitmation with the shadow ray traced inline.
-template <bool kCount>
+template <bool kCount, int kPlanes>
 __device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode, vec3 pos, vec3 o,
                                            const SurfaceInfo& s, vec3 throughput, Rng& re, int* stack,
                                            TraversalCounters& cnt, uint32_t& shadow_rays) {
@@ -44,7 +46,7 @@ __device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode,
     if (!ls.valid) return make_vec3(0.0f);
     ++shadow_rays;
     Hit h;
-    bool occluded = traverse<true, kCount, 2>(sc, pos, ls.l, 0.001f, ls.tmax, h, stack, kTileThreads, cnt);
+    bool occluded = traverse<true, kCount, kPlanes>(sc, pos, ls.l, 0.001f, ls.tmax, h, stack, kTileThreads, cnt);
     if (occluded) return make_vec3(0.0f);
     return nee_unshadowed(ls, o, s, throughput);
 }
@@ -73,9 +75,9 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
     vec3 hit_pos;
     uint32_t category;
     ++rays;
-    trace_closest<kCount>(sc, cam_pos, dir, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+    trace_closest<kCount, 1>(sc, cam_pos, dir, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
     vec3 color = make_vec3(0.0f);
-    color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -normalize(dir), s, throughput, re, stack, cnt_sh,
+    color += nee_inline<kCount, 1>(sc, c.light_sampling, hit_pos, -normalize(dir), s, throughput, re, stack, cnt_sh,
                                 shadow_rays);
     color += s.emissive_color;
     vec3 albedo = s.diffuse_color + s.specular_color;
@@ -118,8 +120,8 @@ __global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, Tr
             }
             throughput = path_tp;
             ++rays;
-            trace_closest<kCount>(sc, wpos, l, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
-            color += nee_inline<kCount>(sc, c.light_sampling, hit_pos, -l, s, throughput, re, stack, cnt_sh,
+            trace_closest<kCount, 1>(sc, wpos, l, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+            color += nee_inline<kCount, 1>(sc, c.light_sampling, hit_pos, -l, s, throughput, re, stack, cnt_sh,
                                         shadow_rays) +
                      s.emissive_color * throughput;
             if (is_miss(s)) break;
@@ -225,9 +227,9 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
         vec3 hit_pos;
         uint32_t category;
         ++rays;
-        trace_closest<kCount>(sc, ro, rd, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+        trace_closest<kCount, 2>(sc, ro, rd, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
         // throughput is (1, 1, 1) on the primary segment
-        const vec3 ne = nee_inline<kCount>(sc, c.light_sampling, hit_pos, primary ? -normalize(rd) : -rd, s,
+        const vec3 ne = nee_inline<kCount, 2>(sc, c.light_sampling, hit_pos, primary ? -normalize(rd) : -rd, s,
                                            throughput, re, stack, cnt_sh, shadow_rays);
         bool finish;
         if (primary) {

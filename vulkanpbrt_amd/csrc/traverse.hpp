@@ -188,13 +188,17 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
 //   per node.  1 keeps the four selectors in registers; 2 keeps child 0's near
 //   selector only and derives the other three per node (3 operations: the far
 //   byte of an axis is lo + hi - near, child 1's bytes sit two further on), for
-//   the megakernel, whose occupancy three more registers would lower.
+//   the persistent kernel, whose occupancy three more registers would lower.
 //   kPlanes 0: all twelve planes dequantized and ordered by min / max.
 template <int kPlanes>
 __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
                                            float tcur, int* stack, int stride, int& sp) {
-    const uint4 na = nodes[2 * node + 0];
-    const uint4 nb = nodes[2 * node + 1];
+    // a 32-bit byte offset from the array's base (node indices stay below 2^27), so
+    // the loads take the scalar-base addressing form
+    const char* base = reinterpret_cast<const char*>(nodes);
+    const uint32_t off = (uint32_t)node << 5;
+    const uint4 na = *reinterpret_cast<const uint4*>(base + off);
+    const uint4 nb = *reinterpret_cast<const uint4*>(base + off + 16u);
     // the exponent e is the origin word's low byte, signed
     const float sx = __builtin_amdgcn_ldexpf(r.ix, (int)(int8_t)na.x),
                 sy = __builtin_amdgcn_ldexpf(r.iy, (int)(int8_t)na.y),
