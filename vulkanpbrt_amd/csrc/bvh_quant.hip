@@ -7,19 +7,22 @@
 // its index as
 //
 //   word 0..2  node origin x, y, z: an fp32 value at or below the node's lower
-//              bound whose low 8 significand bits hold the axis' scale exponent e
-//              as a signed byte (scale s = 2^e); the word is the origin as it
-//              stands, exponent byte included
+//              bound whose low 8 significand bits hold e + 24, signed, for the
+//              axis' scale 2^e (e in [-126, 103]); the word is the origin as it
+//              stands, byte included
 //   word 3     child 0 reference (inner node index, or ~leaf code)
-//   word 4..6  12 bytes: child 0 lo.xyz, hi.xyz, child 1 lo.xyz, hi.xyz as
-//              integers q in [0, 255]; the bound is origin + q s
+//   word 4..6  12 bytes of integers q in [0, 255], grouped by axis: word 4 =
+//              child 0 lo x, hi x, lo y, hi y; word 5 = child 0 lo z, hi z,
+//              child 1 lo z, hi z; word 6 = child 1 lo x, hi x, lo y, hi y; the
+//              bound is origin + q 2^e
 //   word 7     child 1 reference
 //
 // Each child's bounds are rounded outwards and checked in FP64 (where origin + q s
 // is exact unless the scale is below 2^-29 of the origin's own last bit, i.e. for
 // boxes flat to within a fraction of an fp32 ulp), so every box origin + [qlo, qhi] s
 // contains its child's box.  The traversal never forms the bound itself: it meets
-// the plane at fma(q, 2^e / d, fma(origin, 1/d, -o/d)).  A node is a function of
+// the plane at fma(q 2^-24, 2^(e + 24) / d, fma(origin, 1/d, -o/d)), q 2^-24 an
+// fp16 subnormal.  A node is a function of
 // its own two child boxes only, so the pass is one thread per node.
 #include <hip/hip_runtime.h>
 
@@ -31,13 +34,16 @@ namespace pbrt {
 
 namespace {
 
+// Largest scale exponent: e + 24 fits the origin word's signed byte.
+constexpr int kMaxScaleExp = 103;
+
 // The origin word of an axis whose lower bound is f and whose scale exponent is e:
-// the fp32 value with e's byte as its low 8 significand bits that is closest to f
-// from below (the byte adds to a negative value's magnitude, to a positive one's
-// value); zero, denormals and values next to them take -2^-126 - byte ulps
-// (normal, so no denormal origin reaches the traversal).
+// the fp32 value with the byte e + 24 as its low 8 significand bits that is
+// closest to f from below (the byte adds to a negative value's magnitude, to a
+// positive one's value); zero, denormals and values next to them take -2^-126 -
+// byte ulps (normal, so no denormal origin reaches the traversal).
 __device__ uint32_t origin_word(float f, int e) {
-    const uint32_t byte = (uint32_t)e & 0xFFu, b = __float_as_uint(f), low = b & 0xFFu;
+    const uint32_t byte = (uint32_t)(e + 24) & 0xFFu, b = __float_as_uint(f), low = b & 0xFFu;
     uint32_t t = b & ~0xFFu;
     if (b & 0x80000000u) {
         if (byte < low) t += 0x100u;
@@ -84,7 +90,7 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
         if (!(nlo <= nhi)) nlo = nhi = 0.0f;  // both children empty: a point box at 0
         const double ext = (double)nhi - (double)nlo;
         int e = ext > 0.0 ? (int)ceil(log2(ext / 254.0)) : -126;
-        e = e < -126 ? -126 : (e > 127 ? 127 : e);
+        e = e < -126 ? -126 : (e > kMaxScaleExp ? kMaxScaleExp : e);
         for (;; ++e) {
             const float o = __uint_as_float(origin_word(nlo, e));
             bool ok = true;
@@ -92,15 +98,15 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
                 if (empty[c]) continue;  // stays the point box (0, 0)
                 ok = quantize_range(lo[c][a], hi[c][a], o, e, q[c][0][a], q[c][1][a]);
             }
-            if (ok || e >= 127) break;
+            if (ok || e >= kMaxScaleExp) break;
         }
         w[a] = origin_word(nlo, e);
     }
     auto pack = [](uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) { return b0 | (b1 << 8) | (b2 << 16) | (b3 << 24); };
     out[2 * (size_t)i + 0] = make_uint4(w[0], w[1], w[2], __float_as_uint(n3.x));
     out[2 * (size_t)i + 1] =
-        make_uint4(pack(q[0][0][0], q[0][0][1], q[0][0][2], q[0][1][0]), pack(q[0][1][1], q[0][1][2], q[1][0][0], q[1][0][1]),
-                   pack(q[1][0][2], q[1][1][0], q[1][1][1], q[1][1][2]), __float_as_uint(n3.y));
+        make_uint4(pack(q[0][0][0], q[0][1][0], q[0][0][1], q[0][1][1]), pack(q[0][0][2], q[0][1][2], q[1][0][2], q[1][1][2]),
+                   pack(q[1][0][0], q[1][1][0], q[1][0][1], q[1][1][1]), __float_as_uint(n3.y));
 }
 
 }  // namespace

@@ -119,6 +119,9 @@ __device__ __forceinline__ bool alpha_rejects(const DeviceScene& sc, uint32_t in
 // ---------------------------------------------------------------------------
 // Traversal
 // ---------------------------------------------------------------------------
+// two fp16 values in one register (a quantized node's planes, see make_traversal_ray)
+typedef _Float16 PlanePair __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float safe_rcp(float d) {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
 }
@@ -129,19 +132,16 @@ __device__ __forceinline__ float safe_rcp(float d) {
 struct TraversalRay {
     vec3 o, d;
     float ix, iy, iz, ox, oy, oz, tmin;
-    uint32_t sel_near0, sel_far0, sel_near1, sel_far1;
+    uint32_t sel_near_xy, sel_far_xy, sel_z0, sel_z1;
 };
 
-// Byte positions of the child boxes in a quantized node's second half (x, y, z
-// words): child 0 as bytes 0..5 of (y:x) = lo x y z, hi x y z; child 1 as bytes
-// 2..7 of (z:y).  Selector byte 0x0c yields zero.
-__device__ __forceinline__ uint32_t plane_selector(bool neg_x, bool neg_y, bool neg_z, uint32_t lo, bool far) {
-    const uint32_t hi = lo + 3;
-    const uint32_t sx = (neg_x != far) ? hi : lo, sy = (neg_y != far) ? hi + 1 : lo + 1,
-                   sz = (neg_z != far) ? hi + 2 : lo + 2;
-    return sx | sy << 8 | sz << 16 | 0x0c000000u;
-}
-
+// A quantized node's plane bytes (second half, words x, y, z) are grouped by axis:
+// x word = child 0 (lo x, hi x, lo y, hi y), z word = the same for child 1, y word
+// = child 0 (lo z, hi z), child 1 (lo z, hi z).  A selector builds two fp16 values
+// from two of those bytes, each byte as the significand of an fp16 subnormal with
+// selector byte 0x0c (zero) above it, i.e. q 2^-24 exactly.  sel_near_xy picks the
+// near x and y planes (x into the low half), sel_far_xy the far ones; sel_z0 child
+// 0's near and far z planes, sel_z1 child 1's.
 __device__ __forceinline__ TraversalRay make_traversal_ray(vec3 o, vec3 d, float tmin) {
     TraversalRay r;
     r.o = o;
@@ -153,12 +153,13 @@ __device__ __forceinline__ TraversalRay make_traversal_ray(vec3 o, vec3 d, float
     r.oy = o.y * r.iy;
     r.oz = o.z * r.iz;
     r.tmin = tmin;
-    // safe_rcp never returns 0 or NaN, so the sign decides which plane is near
-    const bool nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
-    r.sel_near0 = plane_selector(nx, ny, nz, 0, false);
-    r.sel_far0 = plane_selector(nx, ny, nz, 0, true);
-    r.sel_near1 = plane_selector(nx, ny, nz, 2, false);
-    r.sel_far1 = plane_selector(nx, ny, nz, 2, true);
+    // safe_rcp never returns 0 or NaN, so the sign decides which plane is near:
+    // the lower bound with a positive reciprocal, the upper with a negative one
+    const uint32_t nx = r.ix < 0.0f, ny = r.iy < 0.0f, nz = r.iz < 0.0f;
+    r.sel_near_xy = nx | (2u + ny) << 16 | 0x0c000c00u;
+    r.sel_far_xy = r.sel_near_xy ^ 0x00010001u;
+    r.sel_z0 = nz | (1u - nz) << 16 | 0x0c000c00u;
+    r.sel_z1 = r.sel_z0 + 0x00020002u;
     return r;
 }
 
@@ -176,20 +177,18 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
 // One inner node (quantized layout, 2 x uint4): slab-test both child boxes
 // against [tmin, tcur]; returns the nearer hit child (the farther one pushed) or
 // the next node off the stack.  A plane at origin + q 2^e along an axis is met at
-// t = (origin + q 2^e - o) / d, evaluated as fma(q, S, C) with S = 2^e / d (exact:
-// a power-of-two scaling) and C = fma(origin, 1/d, -o/d) once per axis and node,
-// so each of the twelve planes costs one multiply-add.  The origin word is used
-// whole (its low byte, the exponent, is part of the origin the quantization pass
-// checked the boxes against).
-//   kPlanes 1 and 2: only the near and far plane of each axis is dequantized,
-//   picked by the ray's octant (with a positive reciprocal the lower bound gives
-//   the smaller t, fma rounding being monotonic; with a negative one the upper),
-//   so the result equals min / max over both planes with 8 fewer VALU operations
-//   per node.  1 keeps the four selectors in registers; 2 keeps child 0's near
-//   selector only and derives the other three per node (3 operations: the far
-//   byte of an axis is lo + hi - near, child 1's bytes sit two further on), for
-//   the persistent kernel, whose occupancy three more registers would lower.
-//   kPlanes 0: all twelve planes dequantized and ordered by min / max.
+// t = (origin + q 2^e - o) / d, evaluated as one v_fma_mix_f32 of the fp16
+// subnormal q 2^-24, S = 2^(e + 24) / d (exact: a power-of-two scaling) and
+// C = fma(origin, 1/d, -o/d) (once per axis and node); the product is exact, so t
+// is (origin - o) / d + q 2^e / d rounded once.  The origin word is used whole (its
+// low byte, e + 24, is part of the origin the quantization pass checked the boxes
+// against).
+//   kPlanes 1 and 2: only the near and far plane of each axis is formed, picked by
+//   the ray's octant (fma rounding being monotonic, this equals min / max over both
+//   planes).  1 keeps the four selectors in registers; 2 keeps sel_near_xy and
+//   sel_z0 and derives the other two per node (2 operations), for the persistent
+//   kernel, whose occupancy two more registers would lower.
+//   kPlanes 0: all twelve planes formed and ordered by min / max.
 template <int kPlanes>
 __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
                                            float tcur, int* stack, int stride, int& sp) {
@@ -199,43 +198,41 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
     const uint32_t off = (uint32_t)node << 5;
     const uint4 na = *reinterpret_cast<const uint4*>(base + off);
     const uint4 nb = *reinterpret_cast<const uint4*>(base + off + 16u);
-    // the exponent e is the origin word's low byte, signed
+    // the origin word's low byte, signed, is e + 24
     const float sx = __builtin_amdgcn_ldexpf(r.ix, (int)(int8_t)na.x),
                 sy = __builtin_amdgcn_ldexpf(r.iy, (int)(int8_t)na.y),
                 sz = __builtin_amdgcn_ldexpf(r.iz, (int)(int8_t)na.z);
     const float cx = fmaf(__uint_as_float(na.x), r.ix, -r.ox), cy = fmaf(__uint_as_float(na.y), r.iy, -r.oy),
                 cz = fmaf(__uint_as_float(na.z), r.iz, -r.oz);
-    auto q = [](uint32_t w, int byte) { return (float)((w >> (8 * byte)) & 0xFFu); };
+    // two planes as fp16 subnormals; the widening folds into v_fma_mix_f32
+    auto pair = [](uint32_t w, uint32_t sel) { return __builtin_bit_cast(PlanePair, __builtin_amdgcn_perm(w, w, sel)); };
+    auto tx = [&](_Float16 h) { return fmaf((float)h, sx, cx); };
+    auto ty = [&](_Float16 h) { return fmaf((float)h, sy, cy); };
+    auto tz = [&](_Float16 h) { return fmaf((float)h, sz, cz); };
     float near0, far0, near1, far1;
     if constexpr (kPlanes != 0) {
-        uint32_t sn0 = r.sel_near0, sf0 = r.sel_far0, sn1 = r.sel_near1, sf1 = r.sel_far1;
+        uint32_t sn = r.sel_near_xy, sf = r.sel_far_xy, sz0 = r.sel_z0, sz1 = r.sel_z1;
         if constexpr (kPlanes == 2) {
             // an opaque copy, so the derived selectors are not hoisted out of the walk
-            asm volatile("" : "+v"(sn0));
-            sf0 = 0x18070503u - sn0;  // per byte: 3 - b, 5 - b, 7 - b; 0x0c stays 0x0c
-            sn1 = sn0 + 0x00020202u;
-            sf1 = sf0 + 0x00020202u;
+            asm volatile("" : "+v"(sn), "+v"(sz0));
+            sf = sn ^ 0x00010001u;
+            sz1 = sz0 + 0x00020002u;
         }
-        const uint32_t n0 = __builtin_amdgcn_perm(nb.y, nb.x, sn0);
-        const uint32_t f0 = __builtin_amdgcn_perm(nb.y, nb.x, sf0);
-        const uint32_t n1 = __builtin_amdgcn_perm(nb.z, nb.y, sn1);
-        const uint32_t f1 = __builtin_amdgcn_perm(nb.z, nb.y, sf1);
-        auto tx = [&](uint32_t w) { return fmaf(q(w, 0), sx, cx); };
-        auto ty = [&](uint32_t w) { return fmaf(q(w, 1), sy, cy); };
-        auto tz = [&](uint32_t w) { return fmaf(q(w, 2), sz, cz); };
-        near0 = fmaxf(fmaxf(tx(n0), ty(n0)), fmaxf(tz(n0), r.tmin));
-        far0 = fminf(fminf(tx(f0), ty(f0)), fminf(tz(f0), tcur)) * 1.00000024f;
-        near1 = fmaxf(fmaxf(tx(n1), ty(n1)), fmaxf(tz(n1), r.tmin));
-        far1 = fminf(fminf(tx(f1), ty(f1)), fminf(tz(f1), tcur)) * 1.00000024f;
+        const PlanePair n0 = pair(nb.x, sn), f0 = pair(nb.x, sf), z0 = pair(nb.y, sz0);
+        const PlanePair n1 = pair(nb.z, sn), f1 = pair(nb.z, sf), z1 = pair(nb.y, sz1);
+        near0 = fmaxf(fmaxf(tx(n0.x), ty(n0.y)), fmaxf(tz(z0.x), r.tmin));
+        far0 = fminf(fminf(tx(f0.x), ty(f0.y)), fminf(tz(z0.y), tcur)) * 1.00000024f;
+        near1 = fmaxf(fmaxf(tx(n1.x), ty(n1.y)), fmaxf(tz(z1.x), r.tmin));
+        far1 = fminf(fminf(tx(f1.x), ty(f1.y)), fminf(tz(z1.y), tcur)) * 1.00000024f;
     } else {
-        const float a0 = fmaf(q(nb.x, 0), sx, cx), a1 = fmaf(q(nb.x, 3), sx, cx);
-        const float b0 = fmaf(q(nb.x, 1), sy, cy), b1 = fmaf(q(nb.y, 0), sy, cy);
-        const float c0 = fmaf(q(nb.x, 2), sz, cz), c1 = fmaf(q(nb.y, 1), sz, cz);
+        // (lo, hi) pairs: x and y of each child, z of both
+        const PlanePair x0 = pair(nb.x, 0x0c010c00u), y0 = pair(nb.x, 0x0c030c02u);
+        const PlanePair x1 = pair(nb.z, 0x0c010c00u), y1 = pair(nb.z, 0x0c030c02u);
+        const PlanePair z0 = pair(nb.y, 0x0c010c00u), z1 = pair(nb.y, 0x0c030c02u);
+        const float a0 = tx(x0.x), a1 = tx(x0.y), b0 = ty(y0.x), b1 = ty(y0.y), c0 = tz(z0.x), c1 = tz(z0.y);
         near0 = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), r.tmin));
         far0 = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fminf(fmaxf(c0, c1), tcur)) * 1.00000024f;
-        const float d0 = fmaf(q(nb.y, 2), sx, cx), d1 = fmaf(q(nb.z, 1), sx, cx);
-        const float e0 = fmaf(q(nb.y, 3), sy, cy), e1 = fmaf(q(nb.z, 2), sy, cy);
-        const float f0 = fmaf(q(nb.z, 0), sz, cz), f1 = fmaf(q(nb.z, 3), sz, cz);
+        const float d0 = tx(x1.x), d1 = tx(x1.y), e0 = ty(y1.x), e1 = ty(y1.y), f0 = tz(z1.x), f1 = tz(z1.y);
         near1 = fmaxf(fmaxf(fminf(d0, d1), fminf(e0, e1)), fmaxf(fminf(f0, f1), r.tmin));
         far1 = fminf(fminf(fmaxf(d0, d1), fmaxf(e0, e1)), fminf(fmaxf(f0, f1), tcur)) * 1.00000024f;
     }
