@@ -289,6 +289,8 @@ struct pbrt_scene {
 namespace pbrt {
 const DeviceScene& scene_device_view(const pbrt_scene* s) { return s->dev; }
 
+uint32_t scene_bvh_depth(const pbrt_scene* s) { return s->bvh.depth; }
+
 TraceTiming* scene_timing(const pbrt_scene* s) { return s->timing.active() ? &s->timing : nullptr; }
 
 uint32_t* scene_work_counter(const pbrt_scene* s) {

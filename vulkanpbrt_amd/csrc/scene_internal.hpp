@@ -17,6 +17,11 @@ namespace pbrt {
 // keep the stack at 10 KB per 64-lane wave: the megakernel's 16 resident waves
 // per CU (four per SIMD, set by its registers) fill the 160 KB of LDS exactly.
 constexpr int kTraversalStackSize = 40;
+// Trees of depth <= 32 (the host builders' on the atrium and the city) run the
+// megakernel with a 32-entry stack: 8 KB per wave, so five waves per SIMD fit the
+// LDS, and the kernel is built for five (at most 96 registers, spilling a few
+// path-state values around the bounce loop, none inside the walk).
+constexpr int kShortStackSize = 32;
 
 // Largest leaf the GPU PLOC builder's SAH collapse may form (the node format holds 16).
 constexpr int kPlocMaxLeaf = 8;
@@ -38,6 +43,7 @@ struct PackedScene {
 
 void pack_scene(const pbrt_scene_desc& d, PackedScene& ps);
 const DeviceScene& scene_device_view(const pbrt_scene* s);
+uint32_t scene_bvh_depth(const pbrt_scene* s);
 
 // Opt-in per-stage timing of pbrt_trace (pbrt_trace_timing_begin/_end): HIP
 // events recorded on the launch stream at every stage boundary; each pair of

@@ -23,8 +23,8 @@
 
 namespace pbrt {
 
-// kPlanes: the node-visit variant (visit_inner); the megakernel fits the four
-// per-ray plane selectors in its 128 registers, the persistent kernel derives three
+// kPlanes: the node-visit variant (visit_inner); both kernels keep the four per-ray
+// plane selectors in registers
 template <bool kCount, int kPlanes>
 __device__ __forceinline__ void trace_closest(const DeviceScene& sc, vec3 o, vec3 d, float tmin, float tmax,
                                               int* stack, SurfaceInfo& s, vec3& position, uint32_t& category,
@@ -51,12 +51,14 @@ __device__ __forceinline__ vec3 nee_inline(const DeviceScene& sc, uint32_t mode,
     return nee_unshadowed(ls, o, s, throughput);
 }
 
-template <bool kCount>
-__global__ void __launch_bounds__(kTileThreads) pt_megakernel(DeviceScene sc, TraceConsts c, float* depth_img,
+// kStack: traversal stack entries per lane (kShortStackSize: five waves per SIMD)
+template <bool kCount, int kStack>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(kStack <= kShortStackSize ? 5 : 4)))
+pt_megakernel(DeviceScene sc, TraceConsts c, float* depth_img,
                                                               float* normal_img, uint8_t* material_img,
                                                               uint8_t* albedo_img, float4* illum_img,
                                                               unsigned long long* counters) {
-    __shared__ int stack_mem[kTraversalStackSize * kTileThreads];
+    __shared__ int stack_mem[kStack * kTileThreads];
     const uint32_t lid = threadIdx.x;
     int* stack = stack_mem + lid;
     const uint32_t px = blockIdx.x * kTileX + (lid % kTileX);
@@ -227,9 +229,9 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
         vec3 hit_pos;
         uint32_t category;
         ++rays;
-        trace_closest<kCount, 2>(sc, ro, rd, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
+        trace_closest<kCount, 1>(sc, ro, rd, 0.001f, 10000.0f, stack, s, hit_pos, category, cnt);
         // throughput is (1, 1, 1) on the primary segment
-        const vec3 ne = nee_inline<kCount, 2>(sc, c.light_sampling, hit_pos, primary ? -normalize(rd) : -rd, s,
+        const vec3 ne = nee_inline<kCount, 1>(sc, c.light_sampling, hit_pos, primary ? -normalize(rd) : -rd, s,
                                            throughput, re, stack, cnt_sh, shadow_rays);
         bool finish;
         if (primary) {
@@ -462,12 +464,14 @@ extern "C" int32_t pbrt_trace(const pbrt_scene* scene, const pbrt_trace_params* 
             if (tm) tm->mark(kTimeMegakernel, st);
         } else {
             if (tm) tm->frame_begin(st);
-            if (counters)
-                hipLaunchKernelGGL(pt_megakernel<true>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
-                                   img->normal, img->material, img->albedo, illum, counters);
-            else
-                hipLaunchKernelGGL(pt_megakernel<false>, grid, dim3(kTileThreads), 0, st, sc, c, img->depth,
-                                   img->normal, img->material, img->albedo, illum, counters);
+            // a tree of depth d needs d - 1 stack entries at most
+            const bool short_stack = scene_bvh_depth(scene) <= (uint32_t)kShortStackSize;
+            auto kernel = counters ? (short_stack ? pt_megakernel<true, kShortStackSize>
+                                                  : pt_megakernel<true, kTraversalStackSize>)
+                                   : (short_stack ? pt_megakernel<false, kShortStackSize>
+                                                  : pt_megakernel<false, kTraversalStackSize>);
+            hipLaunchKernelGGL(kernel, grid, dim3(kTileThreads), 0, st, sc, c, img->depth, img->normal,
+                               img->material, img->albedo, illum, counters);
             if (tm) tm->mark(kTimeMegakernel, st);
         }
         PBRT_HIP(hipGetLastError());
