@@ -64,8 +64,9 @@ __device__ __forceinline__ vec4 texture_sample(const DeviceScene& sc, int tex, v
     float y = uv.y * (float)t.height - 0.5f;
     float fx0 = floorf(x), fy0 = floorf(y);
     float fx = x - fx0, fy = y - fy0;
-    int x0 = wrap_repeat((int)fx0, (int)t.width), x1 = wrap_repeat((int)fx0 + 1, (int)t.width);
-    int y0 = wrap_repeat((int)fy0, (int)t.height), y1 = wrap_repeat((int)fy0 + 1, (int)t.height);
+    // the second tap wraps from the first: (i + 1) mod n = i mod n + 1, or 0 at n
+    const int x0 = wrap_repeat((int)fx0, (int)t.width), y0 = wrap_repeat((int)fy0, (int)t.height);
+    const int x1 = x0 + 1 == (int)t.width ? 0 : x0 + 1, y1 = y0 + 1 == (int)t.height ? 0 : y0 + 1;
     vec4 c00 = unpack_rgba8(texels[(size_t)y0 * t.width + x0]);
     vec4 c10 = unpack_rgba8(texels[(size_t)y0 * t.width + x1]);
     vec4 c01 = unpack_rgba8(texels[(size_t)y1 * t.width + x0]);
