@@ -190,7 +190,7 @@ __device__ __forceinline__ int pop_node(const int* stack, int stride, int& sp) {
 //   sel_z0 and derives the other two per node (2 operations), for the persistent
 //   kernel, whose occupancy two more registers would lower.
 //   kPlanes 0: all twelve planes formed and ordered by min / max.
-template <int kPlanes>
+template <int kPlanes, bool kOrdered = true>
 __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int node, const TraversalRay& r,
                                            float tcur, int* stack, int stride, int& sp) {
     // a 32-bit byte offset from the array's base (node indices stay below 2^27), so
@@ -242,7 +242,7 @@ __device__ __forceinline__ int visit_inner(const uint4* __restrict__ nodes, int 
     int ch0 = (int)na.w;
     int ch1 = (int)nb.w;
     if (h0 && h1) {
-        bool swap = near1 < near0;
+        bool swap = kOrdered && near1 < near0;
         stack[sp * stride] = swap ? ch0 : ch1;
         ++sp;
         return swap ? ch1 : ch0;
@@ -332,7 +332,7 @@ __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, floa
     while (node != kNodeDone || leaf != 0) {
         while (node >= 0) {
             if (kCount) cnt.nodes++;
-            node = visit_inner<kPlanes>(sc.nodes, node, r, hit.t, stack, stride, sp);
+            node = visit_inner<kPlanes, !kAnyHit>(sc.nodes, node, r, hit.t, stack, stride, sp);
             if (node < 0 && node != kNodeDone && leaf == 0) {
                 leaf = node;
                 node = pop_node(stack, stride, sp);
