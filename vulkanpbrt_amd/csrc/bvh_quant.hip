@@ -57,9 +57,10 @@ __device__ uint32_t origin_word(float f, int e) {
 // Quantizes [lo, hi] against origin o and scale 2^e: o + qlo 2^e <= lo and
 // o + qhi 2^e >= hi in exact arithmetic; false if 255 steps do not reach hi.
 __device__ bool quantize_range(float lo, float hi, float o, int e, uint32_t& qlo, uint32_t& qhi) {
-    const double s = ldexp(1.0, e), od = (double)o;
-    const double a = floor(((double)lo - od) / s);
-    const double b = ceil(((double)hi - od) / s);
+    // scaling by powers of two is exact: no FP64 division
+    const double s = ldexp(1.0, e), inv_s = ldexp(1.0, -e), od = (double)o;
+    const double a = floor(((double)lo - od) * inv_s);
+    const double b = ceil(((double)hi - od) * inv_s);
     int ql = (int)fmin(fmax(a, 0.0), 255.0);
     int qh = (int)fmin(fmax(b, 0.0), 256.0);
     while (ql > 0 && od + ql * s > (double)lo) --ql;
@@ -68,6 +69,12 @@ __device__ bool quantize_range(float lo, float hi, float o, int e, uint32_t& qlo
     qlo = (uint32_t)ql;
     qhi = (uint32_t)qh;
     return true;
+}
+
+// ceil(log2(x)) for x > 0, from the binary exponent (no transcendental)
+__device__ int ceil_log2(double x) {
+    const int e = ilogb(x);
+    return x > ldexp(1.0, e) ? e + 1 : e;
 }
 
 __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restrict__ out, uint32_t n) {
@@ -89,7 +96,8 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
             }
         if (!(nlo <= nhi)) nlo = nhi = 0.0f;  // both children empty: a point box at 0
         const double ext = (double)nhi - (double)nlo;
-        int e = ext > 0.0 ? (int)ceil(log2(ext / 254.0)) : -126;
+        // the first exponent tried (the loop below moves up until both children fit)
+        int e = ext > 0.0 ? ceil_log2(ext * (1.0 / 254.0)) : -126;
         e = e < -126 ? -126 : (e > kMaxScaleExp ? kMaxScaleExp : e);
         for (;; ++e) {
             const float o = __uint_as_float(origin_word(nlo, e));
