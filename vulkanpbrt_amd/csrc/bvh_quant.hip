@@ -95,9 +95,12 @@ __global__ void k_quantize_nodes(const float4* __restrict__ in, uint4* __restric
                 nhi = fmaxf(nhi, hi[c][a]);
             }
         if (!(nlo <= nhi)) nlo = nhi = 0.0f;  // both children empty: a point box at 0
-        const double ext = (double)nhi - (double)nlo;
-        // the first exponent tried (the loop below moves up until both children fit)
-        int e = ext > 0.0 ? ceil_log2(ext * (1.0 / 254.0)) : -126;
+        // The first exponent tried (the loop below moves up until both children
+        // fit).  The origin lies up to ~512 ulps below nlo, so the span to cover is
+        // at least that much even on a flat axis (whose own extent, 0, would start
+        // the search a hundred exponents too low).
+        const double ext = fmax((double)nhi - (double)nlo, fabs((double)nlo) * 0x1p-14 + 0x1p-120);
+        int e = ceil_log2(ext * (1.0 / 254.0));
         e = e < -126 ? -126 : (e > kMaxScaleExp ? kMaxScaleExp : e);
         for (;; ++e) {
             const float o = __uint_as_float(origin_word(nlo, e));
