@@ -99,7 +99,11 @@ def test_lbvh_queries_match_sah(cuda, pbrt_lib, config, builder):
     assert np.all(same_tri | tie)
     assert np.array_equal(ts[same_tri], tl[same_tri])
     assert np.array_equal(os_, ol)
-    # a traced frame (megakernel) is the same image up to tie pixels
+    # a traced frame (megakernel) is the same image up to tie pixels; trees of depth <= 32
+    # run the megakernel built with a 32-entry stack for five waves per SIMD, deeper ones
+    # (the atrium's PLOC and LBVH trees) the 40-entry build, so config 2 compares the two
+    print(f"megakernel stacks: SAH tree {32 if s.bvh_depth <= 32 else 40} entries, "
+          f"{builder} tree {32 if l.bvh_depth <= 32 else 40} entries")
     imgs = []
     for sc in (sah, lb):
         img = helpers.gbuffer(cuda, 320, 180)
