@@ -206,15 +206,18 @@ def test_bmfr_gram_mfma_vs_fp64_lsq(cuda, oracle_lib, frames, fit, threads):
             errs.append(np.abs(g - f64).max() / max(1e-2, np.abs(f64).max()))
             werrs.append(np.abs(weights[by, bx] - W64[by, bx]).max() / max(1e-6, np.abs(W64[by, bx]).max()))
     errs, werrs = np.array(errs), np.array(werrs)
-    print(f"gram-{fit} fitted-value rel error vs FP64 LSQ: max {errs.max():.3e} median {np.median(errs):.3e}; "
+    print(f"gram-{fit} fitted-value rel error vs FP64 LSQ: max {errs.max():.3e} median {np.median(errs):.3e} "
+          f"p90 {np.quantile(errs, 0.9):.3e} p99 {np.quantile(errs, 0.99):.3e} blocks {len(errs)} "
+          f"under 1e-3 {np.mean(errs < 1e-3):.4f}; "
           f"weights rel error max {werrs.max():.3e} median {np.median(werrs):.3e}")
     if fit == "f64":
         # tighter than the reference's own FP32 Householder on these blocks (~5e-5)
         assert errs.max() < 5e-5
     else:
-        # FP32-accumulated Gram: accurate on well-conditioned blocks, stated error on
-        # the planar / sky blocks where cond(A^T A) ~ 1e10
-        assert np.median(errs) < 1e-4 and errs.max() < 0.5
+        # FP32-accumulated Gram: accurate on well-conditioned blocks (75 of these 80 within
+        # 1e-3, median 2.2e-6), a stated error on the planar / sky blocks where
+        # cond(A^T A) ~ 1e10 (the worst measured 0.138)
+        assert np.median(errs) < 1e-5 and np.mean(errs < 1e-3) >= 0.9 and errs.max() < 0.25
     assert np.isfinite(final).all()
 
 
