@@ -14,8 +14,9 @@ namespace pbrt {
 // Per-thread traversal stack entries (LDS, 4 B each).  The BVH builders bound
 // the tree depth by this value (median splits once SAH splits could overrun
 // it), and a traversal holds at most depth - 1 deferred children.  40 entries
-// keep the stack at 10 KB per 64-lane wave: the megakernel's 16 resident waves
-// per CU (four per SIMD, set by its registers) fill the 160 KB of LDS exactly.
+// keep the stack at 10 KB per 64-lane wave: 16 resident waves per CU (four per
+// SIMD, the persistent and wavefront kernels' and the deep-tree megakernel's
+// register budget) fill the 160 KB of LDS exactly.
 constexpr int kTraversalStackSize = 40;
 // Trees of depth <= 32 (the host builders' on the atrium and the city) run the
 // megakernel with a 32-entry stack: 8 KB per wave, so five waves per SIMD fit the
