@@ -341,6 +341,9 @@ __device__ bool traverse(const DeviceScene& sc, vec3 o, vec3 d, float tmin, floa
                 leaf = node;
                 node = pop_node(stack, stride, sp);
             }
+            // leave only when no lane still walking lacks a set-aside leaf: every lane
+            // then holds a leaf, a leaf code in `node` or kNodeDone, never an inner
+            // node, which the leaf test below would read as a triangle range
             if (!__any(leaf == 0)) break;
         }
         const int lf = leaf != 0 ? leaf : (node != kNodeDone ? node : 0);
