@@ -831,13 +831,33 @@ __device__ __forceinline__ void bmfr_fit_raw_gram_block(const BmfrGeom& g, const
     }
 }
 
+// Workgroup b of a launch of n runs on XCD b % 8 (the dispatcher deals workgroups
+// out round-robin).  The workgroups of XCD x take the x-th contiguous share of the
+// block order instead, so horizontally neighbouring blocks, whose grid is shifted by
+// the frame's pixel offset and whose rows therefore share cache lines at their
+// ends, are read through one XCD's L2.  A permutation of [0, n).
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+    const uint32_t x = b % 8u, s = b / 8u, q = n / 8u, r = n % 8u;
+    return x * q + min(x, r) + s;
+}
+
+// block (bx, by) of the 2D launch grid over the blocks_x x blocks_y BMFR blocks
+__device__ __forceinline__ void grid_block(const BmfrGeom& g, int& bx, int& by) {
+    const uint32_t n = gridDim.x * gridDim.y;
+    const uint32_t b = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, n);
+    bx = (int)(b % (uint32_t)g.blocks_x);
+    by = (int)(b / (uint32_t)g.blocks_x);
+}
+
 template <int kRows, bool kF64, int kThreads>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bmfr_fit_gram(BmfrGeom g,
                                                                                                   pbrt_bmfr_images im) {
+    int bx, by;
+    grid_block(g, bx, by);
     if constexpr (kF64)
-        bmfr_fit_raw_gram_block<kRows, kThreads>(g, im, blockIdx.x, blockIdx.y);
+        bmfr_fit_raw_gram_block<kRows, kThreads>(g, im, bx, by);
     else
-        bmfr_fit_gram_f16_block<kRows, kThreads>(g, im, blockIdx.x, blockIdx.y);
+        bmfr_fit_gram_f16_block<kRows, kThreads>(g, im, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -907,7 +927,9 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
 }
 
 __global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
-    bmfr_post_block(g, im, blockIdx.x, blockIdx.y);
+    int bx, by;
+    grid_block(g, bx, by);
+    bmfr_post_block(g, im, bx, by);
 }
 
 // ---------------------------------------------------------------------------
@@ -935,10 +957,11 @@ __device__ __forceinline__ uint32_t batch_frame_of(const BmfrBatchFrame* frames,
 template <int kRows, bool kF64>
 __global__ void __launch_bounds__(kFitThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
-    const uint32_t f = batch_frame_of(frames, n, blockIdx.x);
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t f = batch_frame_of(frames, n, b);
     const BmfrGeom g = frames[f].g;
     const pbrt_bmfr_images im = frames[f].im;
-    const uint32_t local = blockIdx.x - frames[f].first_block;
+    const uint32_t local = b - frames[f].first_block;
     const int bx = (int)(local % (uint32_t)g.blocks_x), by = (int)(local / (uint32_t)g.blocks_x);
     if constexpr (kF64)
         bmfr_fit_raw_gram_block<kRows, kFitThreads>(g, im, bx, by);
@@ -954,10 +977,11 @@ __global__ void __launch_bounds__(256) k_bmfr_noise_batched(const BmfrBatchFrame
 }
 
 __global__ void __launch_bounds__(1024) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
-    const uint32_t f = batch_frame_of(frames, n, blockIdx.x);
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t f = batch_frame_of(frames, n, b);
     const BmfrGeom g = frames[f].g;
     const pbrt_bmfr_images im = frames[f].im;
-    const uint32_t local = blockIdx.x - frames[f].first_block;
+    const uint32_t local = b - frames[f].first_block;
     bmfr_post_block(g, im, (int)(local % (uint32_t)g.blocks_x), (int)(local / (uint32_t)g.blocks_x));
 }
 
