@@ -881,6 +881,26 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
     }
     PixelIn p = load_pixel(g, im, gx, gy);
     const size_t pix = (size_t)p.img_y * g.width + p.img_x;
+    // the loads that do not need the block's depth range go out before its barrier,
+    // so their latency overlaps the reduction (nothing here reads what post writes:
+    // the history read is the other denoised plane)
+    const size_t plane = (size_t)g.width * g.height;
+    bool accept = false;
+    float pixel_spp = 1.0f;
+    vec3 prev = make_vec3(0.0f), alb = make_vec3(0.0f);
+    if (p.inside) {
+        const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
+        const float mux = h2f(mv.x), muy = h2f(mv.y);
+        accept = mux >= 0.0f;
+        pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
+        alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
+        if (g.frame > 0 && accept) {
+            const uint16_t* layer = im.denoised + 4 * plane * (g.frame & 1u);
+            const int w = g.width;
+            prev = xyz(sample_bilinear([&](int sx, int sy) { return load_half4(layer, (size_t)sy * w + sx); },
+                                       g.width, g.height, mux, muy));
+        }
+    }
     // block min / max of depth: one barrier, every thread folds the waves in order
     {
         const float wmin = wave_reduce_dpp(p.depth, OpMin()), wmax = wave_reduce_dpp(p.depth, OpMax());
@@ -896,10 +916,6 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
         mx = fmaxf(mx, part_max[w]);
     }
     if (!p.inside) return;
-    const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
-    float mux = h2f(mv.x), muy = h2f(mv.y);
-    bool accept = mux >= 0.0f;
-    float pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
     float z = (p.depth - mn) / (mx - mn + 1e-6f);
     float fx = (float)lx / (float)(g.bs - 1), fy = (float)ly / (float)(g.bs - 1);
     float f[kFeatures] = {1.0f, p.normal.x, p.normal.y, p.normal.z, fx, fy, z, fx * fx, fy * fy, z * z};
@@ -907,19 +923,10 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
 #pragma unroll
     for (int i = 0; i < kFeatures; ++i) den += make_vec3(wts[3 * i], wts[3 * i + 1], wts[3 * i + 2]) * f[i];
     den = vclamp(den, 0.0f, 10.0f);
-    vec3 prev = make_vec3(0.0f);
     float alpha = 1.0f;
-    const size_t plane = (size_t)g.width * g.height;
-    if (g.frame > 0 && accept) {
-        const uint16_t* layer = im.denoised + 4 * plane * (g.frame & 1u);
-        const int w = g.width;
-        prev = xyz(sample_bilinear([&](int sx, int sy) { return load_half4(layer, (size_t)sy * w + sx); }, g.width,
-                                   g.height, mux, muy));
-        alpha = fmaxf(1.0f / pixel_spp, 0.1f);
-    }
+    if (g.frame > 0 && accept) alpha = fmaxf(1.0f / pixel_spp, 0.1f);
     den = alpha * den + (1.0f - alpha) * prev;
     store_half4(im.denoised + 4 * plane * ((g.frame & 1u) ^ 1u), pix, make_vec4(den, 1.0f));
-    vec3 alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
     vec3 t = alb * den;
     t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
     t = vclamp(make_vec3(gamma_454545(t.x), gamma_454545(t.y), gamma_454545(t.z)), 0.0f, 1.0f);
