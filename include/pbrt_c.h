@@ -424,8 +424,10 @@ typedef struct pbrt_bmfr_images {
 
 /* Workspace layout (each part 256-byte aligned): fp16 feature buffer [13][Hp][Wp]
  * (Householder fit only), float weights [blocks_y][blocks_x][30] (feature-major,
- * rgb inner), then the frame's noise table float [10][block_size^2] (Gram fits:
- * the addRandom values, computed once per frame). */
+ * rgb inner), the frame's noise table float [10][block_size^2] (Gram fits: the
+ * addRandom values, computed once per frame), then float depth ranges
+ * [blocks_y][blocks_x][2] (each block's depth min and max, written by the fit's
+ * pass and read by the post pass). */
 int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* params, size_t* out_bytes);
 int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images, void* stream);
 
@@ -436,7 +438,7 @@ int32_t pbrt_bmfr(const pbrt_bmfr_params* params, const pbrt_bmfr_images* images
  * scratch (the fit's weights, plus the fp16 feature array for the Householder fit)
  * lives in a device array of PBRT_BMFR_NOALLOC_SCRATCH_BYTES that the library holds
  * on each GPU it runs on until the process ends; a frame needing more is rejected
- * with PBRT_ERR_INVALID_ARG (Gram fits: 30 floats per block, i.e. up to 3840x2160 at
+ * with PBRT_ERR_INVALID_ARG (Gram fits: 32 floats per block, i.e. up to 3840x2160 at
  * 16-pixel blocks; Householder: about 26 bytes per padded pixel, i.e. up to ~640x480).
  * Every call on one GPU shares that array: calls that overlap on one GPU (two streams,
  * two threads) race.  Nothing in the scratch needs clearing between calls. */
@@ -474,8 +476,8 @@ typedef struct pbrt_bmfr_batch_frame {
 
 int32_t pbrt_bmfr_batched_workspace_size(const pbrt_bmfr_batch_frame* frames, uint32_t count, size_t* out_bytes);
 /* Workspace layout: frame table, then per frame (each 256-byte aligned) float
- * weights [blocks_y][blocks_x][30] and the frame's noise table [10][block_size^2],
- * in frame order. */
+ * weights [blocks_y][blocks_x][30], the frame's noise table [10][block_size^2] and
+ * its blocks' depth ranges [blocks_y][blocks_x][2], in frame order. */
 int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32_t count, void* workspace,
                           size_t workspace_bytes, void* stream);
 

@@ -97,9 +97,10 @@ def check_batch(torch, frames, fit):
     a256 = lambda x: (x + 255) // 256 * 256
     nbs = [(f["w"] // 32 + 2) * (f["h"] // 32 + 2) * 30 for f in frames]
     noise = a256(4 * 10 * 32 * 32)  # the frame's noise table [10][32 * 32]
-    # workspace: frame table, then per frame its weights and its noise table, each in
-    # its own 256-byte aligned region
-    off = ws.numel() - sum(a256(4 * nb) + noise for nb in nbs)
+    ranges = [a256(8 * (nb // 30)) for nb in nbs]  # each block's depth min and max
+    # workspace: frame table, then per frame its weights, its noise table and its
+    # blocks' depth ranges, each in its own 256-byte aligned region
+    off = ws.numel() - sum(a256(4 * nb) + noise + r for nb, r in zip(nbs, ranges))
     for i, f in enumerate(frames):
         wb = ws[off:off + 4 * nbs[i]].view(torch.float32)
         fb = a256(13 * (f["w"] // 32 + 2) * 32 * (f["h"] // 32 + 2) * 32 * 2)
@@ -107,7 +108,7 @@ def check_batch(torch, frames, fit):
         assert torch.equal(wb, w1), f"frame {i} weights"
         assert torch.equal(batched[i][0], f["denoised"]), f"frame {i} denoised"
         assert torch.equal(batched[i][1], f["final"]), f"frame {i} final"
-        off += a256(4 * nbs[i]) + noise
+        off += a256(4 * nbs[i]) + noise + ranges[i]
     return t_batch, t_single
 
 

@@ -45,6 +45,7 @@ struct BmfrGeom {
     int fit_threads;
     uint16_t* features;  // [13][hp][wp] fp16
     float* weights;      // [blocks_y][blocks_x][30]
+    float* ranges;       // [blocks_y][blocks_x][2]: the block's depth min and max, from the fit's pass
     float* noise;        // [10][bs*bs]: the noise of feature c for each block pixel (Gram fits), in
                          // the fit's load order: entry c * bs^2 + t * kSub + s for thread t's s-th pixel
 };
@@ -198,6 +199,13 @@ __device__ __forceinline__ PixelIn load_pixel(const BmfrGeom& g, const pbrt_bmfr
     return p;
 }
 
+// The block's depth range, for the post pass (it normalises the depth feature again).
+__device__ __forceinline__ void store_range(const BmfrGeom& g, int bx, int by, float mn, float mx) {
+    float* r = g.ranges + 2 * ((size_t)by * g.blocks_x + bx);
+    r[0] = mn;
+    r[1] = mx;
+}
+
 // ---------------------------------------------------------------------------
 // bmfrPre: 13 features of every padded pixel to the fp16 feature buffer
 // ---------------------------------------------------------------------------
@@ -208,6 +216,7 @@ __global__ void __launch_bounds__(1024) k_bmfr_pre(BmfrGeom g, pbrt_bmfr_images 
     PixelIn p = load_pixel(g, im, gx, gy);
     float mn = block_reduce(p.depth, OpMin(), scratch);
     float mx = block_reduce(p.depth, OpMax(), scratch);
+    if (threadIdx.x == 0) store_range(g, blockIdx.x, blockIdx.y, mn, mx);
     float z = (p.depth - mn) / (mx - mn + 1e-6f);
     float px = (float)lx / (float)(g.bs - 1), py = (float)ly / (float)(g.bs - 1);
     float f[kAlphaSize] = {1.0f, p.normal.x, p.normal.y, p.normal.z, px, py, z, px * px, py * py, z * z,
@@ -393,6 +402,7 @@ __device__ __forceinline__ void bmfr_fit_gram_f16_block(const BmfrGeom& g, const
         mn = fminf(mn, S.depth_min[w]);
         mx = fmaxf(mx, S.depth_max[w]);
     }
+    if (tid == 0) store_range(g, bx, by, mn, mx);
     // this thread's noise values, kSub consecutive floats per feature
     float noise[kFeatures][kSub];
 #pragma unroll
@@ -676,6 +686,7 @@ __device__ __forceinline__ void bmfr_fit_raw_gram_block(const BmfrGeom& g, const
         mn = fminf(mn, S.depth_min[w]);
         mx = fmaxf(mx, S.depth_max[w]);
     }
+    if (tid == 0) store_range(g, bx, by, mn, mx);
     // 2. features, fp16-quantised (the reference's R16F feature buffer), + noise; the
     //    noise of feature c for this thread's pixels is one vector load
 #pragma unroll
@@ -870,52 +881,37 @@ __device__ __forceinline__ float gamma_454545(float x) {
     return __builtin_amdgcn_exp2f(0.454545f * __builtin_amdgcn_logf(x));
 }
 
-__device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx, int by) {
-    __shared__ float part_min[16], part_max[16];
-    __shared__ float wts[30];
-    const int lx = threadIdx.x % g.bs, ly = threadIdx.x / g.bs;
+
+// Rows per workgroup of k_bmfr_post: a block's rows are cut into workgroups of at
+// most 256 threads.
+__host__ __device__ __forceinline__ int post_rows(int bs) { return bs * bs > 256 ? 256 / bs : bs; }
+
+// bmfrPost per pixel: the block's depth range is the one its fit recorded, so a
+// workgroup needs no reduction and no barrier; workgroup w of the launch takes
+// rows [row0, row0 + post_rows) of one block (XCD-contiguous order, as the fit),
+// one thread per padded pixel, and the padding pixels leave at once.  The weights
+// are uniform over the workgroup: scalar loads, and a non-finite weight (an
+// exponent of all ones) becomes 0 with integer operations.
+// Pixel (lx, ly) of block blk of a frame (the per-thread part of both post launches).
+__device__ __forceinline__ void bmfr_post_pixel(const BmfrGeom& g, const pbrt_bmfr_images& im, uint32_t blk, int lx,
+                                                int ly) {
+    const int bx = (int)(blk % (uint32_t)g.blocks_x), by = (int)(blk / (uint32_t)g.blocks_x);
     const int gx = bx * g.bs + lx, gy = by * g.bs + ly;
-    if (threadIdx.x < 30) {
-        float w = g.weights[((size_t)by * g.blocks_x + bx) * 30 + threadIdx.x];
-        wts[threadIdx.x] = (isinf(w) || isnan(w)) ? 0.0f : w;
-    }
-    PixelIn p = load_pixel(g, im, gx, gy);
+    const int ax = gx - g.offx, ay = gy - g.offy;
+    if (ax != mirror_coord(ax, g.width) || ay != mirror_coord(ay, g.height)) return;
+    const PixelIn p = load_pixel(g, im, gx, gy);
     const size_t pix = (size_t)p.img_y * g.width + p.img_x;
-    // the loads that do not need the block's depth range go out before its barrier,
-    // so their latency overlaps the reduction (nothing here reads what post writes:
-    // the history read is the other denoised plane)
-    const size_t plane = (size_t)g.width * g.height;
-    bool accept = false;
-    float pixel_spp = 1.0f;
-    vec3 prev = make_vec3(0.0f), alb = make_vec3(0.0f);
-    if (p.inside) {
-        const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
-        const float mux = h2f(mv.x), muy = h2f(mv.y);
-        accept = mux >= 0.0f;
-        pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
-        alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
-        if (g.frame > 0 && accept) {
-            const uint16_t* layer = im.denoised + 4 * plane * (g.frame & 1u);
-            const int w = g.width;
-            prev = xyz(sample_bilinear([&](int sx, int sy) { return load_half4(layer, (size_t)sy * w + sx); },
-                                       g.width, g.height, mux, muy));
-        }
+    const float mn = g.ranges[2 * blk], mx = g.ranges[2 * blk + 1];
+    float wts[30];
+#pragma unroll
+    for (int i = 0; i < 30; ++i) {
+        const uint32_t u = __float_as_uint(g.weights[(size_t)blk * 30 + i]);
+        wts[i] = __uint_as_float((u & 0x7F800000u) == 0x7F800000u ? 0u : u);
     }
-    // block min / max of depth: one barrier, every thread folds the waves in order
-    {
-        const float wmin = wave_reduce_dpp(p.depth, OpMin()), wmax = wave_reduce_dpp(p.depth, OpMax());
-        if ((threadIdx.x & 63) == 0) {
-            part_min[threadIdx.x >> 6] = wmin;
-            part_max[threadIdx.x >> 6] = wmax;
-        }
-    }
-    __syncthreads();
-    float mn = part_min[0], mx = part_max[0];
-    for (int w = 1, nw = (int)(blockDim.x >> 6); w < nw; ++w) {
-        mn = fminf(mn, part_min[w]);
-        mx = fmaxf(mx, part_max[w]);
-    }
-    if (!p.inside) return;
+    const ushort2 mv = reinterpret_cast<const ushort2*>(im.motion)[pix];
+    const float mux = h2f(mv.x), muy = h2f(mv.y);
+    const bool accept = mux >= 0.0f;
+    const float pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
     float z = (p.depth - mn) / (mx - mn + 1e-6f);
     float fx = (float)lx / (float)(g.bs - 1), fy = (float)ly / (float)(g.bs - 1);
     float f[kFeatures] = {1.0f, p.normal.x, p.normal.y, p.normal.z, fx, fy, z, fx * fx, fy * fy, z * z};
@@ -923,20 +919,30 @@ __device__ __forceinline__ void bmfr_post_block(const BmfrGeom& g, const pbrt_bm
 #pragma unroll
     for (int i = 0; i < kFeatures; ++i) den += make_vec3(wts[3 * i], wts[3 * i + 1], wts[3 * i + 2]) * f[i];
     den = vclamp(den, 0.0f, 10.0f);
+    vec3 prev = make_vec3(0.0f);
     float alpha = 1.0f;
-    if (g.frame > 0 && accept) alpha = fmaxf(1.0f / pixel_spp, 0.1f);
+    const size_t plane = (size_t)g.width * g.height;
+    if (g.frame > 0 && accept) {
+        const uint16_t* layer = im.denoised + 4 * plane * (g.frame & 1u);
+        const int wd = g.width;
+        prev = xyz(sample_bilinear([&](int sx, int sy) { return load_half4(layer, (size_t)sy * wd + sx); }, g.width,
+                                   g.height, mux, muy));
+        alpha = fmaxf(1.0f / pixel_spp, 0.1f);
+    }
     den = alpha * den + (1.0f - alpha) * prev;
     store_half4(im.denoised + 4 * plane * ((g.frame & 1u) ^ 1u), pix, make_vec4(den, 1.0f));
+    vec3 alb = xyz(load_rgba8(im.albedo, pix)) + make_vec3(1e-6f);
     vec3 t = alb * den;
     t = make_vec3(fmaxf(t.x, 0.0f), fmaxf(t.y, 0.0f), fmaxf(t.z, 0.0f));
     t = vclamp(make_vec3(gamma_454545(t.x), gamma_454545(t.y), gamma_454545(t.z)), 0.0f, 1.0f);
     store_bgra8(im.final_image, pix, make_vec4(t, 1.0f));
 }
 
-__global__ void __launch_bounds__(1024) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
-    int bx, by;
-    grid_block(g, bx, by);
-    bmfr_post_block(g, im, bx, by);
+__global__ void __launch_bounds__(256) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
+    const int rows = post_rows(g.bs), tiles = g.bs / rows;
+    const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
+    bmfr_post_pixel(g, im, w / (uint32_t)tiles, threadIdx.x % g.bs,
+                    (int)(w % (uint32_t)tiles) * rows + threadIdx.x / g.bs);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,13 +989,13 @@ __global__ void __launch_bounds__(256) k_bmfr_noise_batched(const BmfrBatchFrame
     if (e < kFeatures * g.bs * g.bs) noise_table_entry(g.frame, g.bs, kFitThreads, e, g.noise);
 }
 
-__global__ void __launch_bounds__(1024) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
-    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
+    const int rows = post_rows(frames[0].g.bs), tiles = frames[0].g.bs / rows;  // one block size per batch
+    const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
+    const uint32_t b = w / (uint32_t)tiles;
     const uint32_t f = batch_frame_of(frames, n, b);
-    const BmfrGeom g = frames[f].g;
-    const pbrt_bmfr_images im = frames[f].im;
-    const uint32_t local = b - frames[f].first_block;
-    bmfr_post_block(g, im, (int)(local % (uint32_t)g.blocks_x), (int)(local / (uint32_t)g.blocks_x));
+    bmfr_post_pixel(frames[f].g, frames[f].im, b - frames[f].first_block, threadIdx.x % frames[f].g.bs,
+                    (int)(w % (uint32_t)tiles) * rows + threadIdx.x / frames[f].g.bs);
 }
 
 // pixelOffsets of bmfrGeneral.comp:16
@@ -1068,6 +1074,7 @@ static void launch_gram_fit(bool f64, int bs, uint32_t threads, dim3 grid, hipSt
 static size_t feature_bytes(const BmfrGeom& g) { return (size_t)kAlphaSize * g.wp * g.hp * sizeof(uint16_t); }
 static size_t weight_bytes(const BmfrGeom& g) { return (size_t)g.blocks_x * g.blocks_y * 30 * sizeof(float); }
 static size_t noise_bytes(const BmfrGeom& g) { return (size_t)kFeatures * g.bs * g.bs * sizeof(float); }
+static size_t range_bytes(const BmfrGeom& g) { return (size_t)g.blocks_x * g.blocks_y * 2 * sizeof(float); }
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static void require_bmfr_images(const pbrt_bmfr_images* im, const char* who) {
@@ -1079,10 +1086,11 @@ static void require_bmfr_images(const pbrt_bmfr_images* im, const char* who) {
 // pre + fit + post of one frame with the feature array (Householder fit only), the
 // weights and the noise table (Gram fits only) at the given device addresses.
 static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, BmfrGeom g, uint16_t* features,
-                        float* weights, float* noise, hipStream_t st) {
+                        float* weights, float* noise, float* ranges, hipStream_t st) {
     g.features = features;
     g.weights = weights;
     g.noise = noise;
+    g.ranges = ranges;
     dim3 grid(g.blocks_x, g.blocks_y);
     const int threads = g.bs * g.bs;
     if (p.fit == PBRT_BMFR_FIT_HOUSEHOLDER) {
@@ -1100,7 +1108,8 @@ static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, B
                            g.noise);
         launch_gram_fit(p.fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, (uint32_t)threads, grid, st, g, im);
     }
-    hipLaunchKernelGGL(k_bmfr_post, grid, dim3(threads), 0, st, g, im);
+    const int rows = post_rows(g.bs);
+    hipLaunchKernelGGL(k_bmfr_post, dim3(g.blocks_x * g.blocks_y * (g.bs / rows)), dim3(g.bs * rows), 0, st, g, im);
     PBRT_HIP(hipGetLastError());
 }
 
@@ -1137,7 +1146,7 @@ extern "C" int32_t pbrt_bmfr_workspace_size(const pbrt_bmfr_params* p, size_t* o
     return guard([&] {
         PBRT_REQUIRE(p && out, "pbrt_bmfr_workspace_size: NULL argument");
         BmfrGeom g = make_geom(*p);
-        *out = align256(feature_bytes(g)) + align256(weight_bytes(g)) + noise_bytes(g);
+        *out = align256(feature_bytes(g)) + align256(weight_bytes(g)) + align256(noise_bytes(g)) + range_bytes(g);
     });
 }
 
@@ -1147,11 +1156,12 @@ extern "C" int32_t pbrt_bmfr(const pbrt_bmfr_params* p, const pbrt_bmfr_images* 
         require_bmfr_images(im, "pbrt_bmfr");
         PBRT_REQUIRE(im->workspace, "pbrt_bmfr: missing workspace");
         BmfrGeom g = make_geom(*p);
-        const size_t fb = align256(feature_bytes(g)), wb = align256(weight_bytes(g));
-        PBRT_REQUIRE(im->workspace_bytes >= fb + wb + noise_bytes(g), "pbrt_bmfr: workspace too small");
+        const size_t fb = align256(feature_bytes(g)), wb = align256(weight_bytes(g)), nb = align256(noise_bytes(g));
+        PBRT_REQUIRE(im->workspace_bytes >= fb + wb + nb + range_bytes(g), "pbrt_bmfr: workspace too small");
         char* ws = static_cast<char*>(im->workspace);
         launch_bmfr(*p, *im, g, reinterpret_cast<uint16_t*>(ws), reinterpret_cast<float*>(ws + fb),
-                    reinterpret_cast<float*>(ws + fb + wb), (hipStream_t)stream);
+                    reinterpret_cast<float*>(ws + fb + wb), reinterpret_cast<float*>(ws + fb + wb + nb),
+                    (hipStream_t)stream);
     });
 }
 
@@ -1160,19 +1170,21 @@ extern "C" int32_t pbrt_bmfr_noalloc(const pbrt_bmfr_params* p, const pbrt_bmfr_
         PBRT_REQUIRE(p && im, "pbrt_bmfr_noalloc: NULL argument");
         require_bmfr_images(im, "pbrt_bmfr_noalloc");
         BmfrGeom g = make_geom(*p);
-        // the Gram fits keep the weights and the frame's noise table; the Householder
-        // fit the fp16 feature array and the weights.  Every byte used is written
-        // before it is read.
+        // the Gram fits keep the weights, the frame's noise table and the blocks'
+        // depth ranges; the Householder fit the fp16 feature array, the weights and
+        // the ranges.  Every byte used is written before it is read.
         const bool hh = p->fit == PBRT_BMFR_FIT_HOUSEHOLDER;
         const size_t fb = hh ? align256(feature_bytes(g)) : 0, wb = align256(weight_bytes(g));
-        const size_t need = fb + wb + (hh ? 0 : noise_bytes(g));
+        const size_t nb = hh ? 0 : align256(noise_bytes(g));
+        const size_t need = fb + wb + nb + range_bytes(g);
         if (need > kNoallocScratchBytes)
             fail(PBRT_ERR_INVALID_ARG, "pbrt_bmfr_noalloc: this frame needs " + std::to_string(need) +
                                            " bytes of scratch, more than the " + std::to_string(kNoallocScratchBytes) +
                                            " the library holds per device (use pbrt_bmfr with a workspace)");
         char* base = device_scratch();
         launch_bmfr(*p, *im, g, reinterpret_cast<uint16_t*>(base), reinterpret_cast<float*>(base + fb),
-                    reinterpret_cast<float*>(base + fb + wb), (hipStream_t)stream);
+                    reinterpret_cast<float*>(base + fb + wb), reinterpret_cast<float*>(base + fb + wb + nb),
+                    (hipStream_t)stream);
     });
 }
 
@@ -1257,10 +1269,13 @@ BatchPlan plan_batch(const pbrt_bmfr_batch_frame* frames, uint32_t count, char* 
         off = align256(off + weight_bytes(e.g));
         e.g.noise = workspace ? reinterpret_cast<float*>(workspace + off) : nullptr;
         off = align256(off + noise_bytes(e.g));
+        e.g.ranges = workspace ? reinterpret_cast<float*>(workspace + off) : nullptr;
+        off = align256(off + range_bytes(e.g));
         e.first_block = (uint32_t)blocks;
         blocks += (uint64_t)e.g.blocks_x * e.g.blocks_y;
     }
-    PBRT_REQUIRE(blocks < (1ull << 31), "pbrt_bmfr_batched: too many blocks for one launch");
+    // up to four post workgroups per block: the launch grid stays below 2^31
+    PBRT_REQUIRE(blocks < (1ull << 29), "pbrt_bmfr_batched: too many blocks for one launch");
     plan.blocks = (uint32_t)blocks;
     plan.total_bytes = off;
     return plan;
@@ -1343,7 +1358,9 @@ extern "C" int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32
             else
                 hipLaunchKernelGGL((k_bmfr_fit_gram_batched<64, false>), grid, dim3(kFitThreads), 0, st, table, count);
         }
-        hipLaunchKernelGGL(k_bmfr_post_batched, grid, dim3(plan.bs * plan.bs), 0, st, table, count);
+        const int rows = post_rows(plan.bs);
+        hipLaunchKernelGGL(k_bmfr_post_batched, dim3(plan.blocks * (uint32_t)(plan.bs / rows)), dim3(plan.bs * rows), 0,
+                           st, table, count);
         PBRT_HIP(hipGetLastError());
     });
 }
