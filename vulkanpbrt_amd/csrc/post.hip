@@ -94,8 +94,23 @@ __device__ __forceinline__ vec3 rgb_to_ycocg(vec3 c) {
 }
 
 // taa.comp:48-104.  vec3(1/0) folds to the integer 0x7FFFFFFF (2^31 as float).
+// The tile's denoised pixels and their one-pixel halo are converted to YCoCg once,
+// into LDS, instead of once per neighbourhood they fall in (nine times).
 __global__ void __launch_bounds__(kTile* kTile) k_taa(uint32_t width, uint32_t height, uint32_t frame,
                                                      pbrt_taa_images im) {
+    constexpr int kHalo = kTile + 2;
+    __shared__ float ycc[3][kHalo * kHalo];
+    const int x0 = (int)blockIdx.x * kTile - 1, y0 = (int)blockIdx.y * kTile - 1;
+    for (int i = threadIdx.x; i < kHalo * kHalo; i += kTile * kTile) {
+        const int sx = x0 + i % kHalo, sy = y0 + i / kHalo;
+        if (sx >= 0 && sy >= 0 && sx < (int)width && sy < (int)height) {
+            const vec3 c = rgb_to_ycocg(xyz(load_bgra8(im.denoised, (size_t)sy * width + sx)));
+            ycc[0][i] = c.x;
+            ycc[1][i] = c.y;
+            ycc[2][i] = c.z;
+        }
+    }
+    __syncthreads();
     const int x = blockIdx.x * kTile + threadIdx.x % kTile;
     const int y = blockIdx.y * kTile + threadIdx.x / kTile;
     if (x >= (int)width || y >= (int)height) return;
@@ -114,8 +129,8 @@ __global__ void __launch_bounds__(kTile* kTile) k_taa(uint32_t width, uint32_t h
         for (int dx = -1; dx <= 1; ++dx) {
             int sx = x + dx, sy = y + dy;
             if (sx >= 0 && sy >= 0 && sx < (int)width && sy < (int)height) {
-                vec3 sc = (dx == 0 && dy == 0) ? cur : xyz(load_bgra8(im.denoised, (size_t)sy * width + sx));
-                sc = rgb_to_ycocg(sc);
+                const int h = (sy - y0) * kHalo + (sx - x0);
+                const vec3 sc = make_vec3(ycc[0][h], ycc[1][h], ycc[2][h]);
                 if (dx == 0 || dy == 0) {
                     min_cross = vmin(min_cross, sc);
                     max_cross = vmax(max_cross, sc);
