@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--quality-spp", type=int, default=4096, help="samples per pixel of the references")
     ap.add_argument("--graph", dest="graph", action="store_true", default=None,
                     help="capture each frame (trace, accumulate, denoise, TAA) into one hipGraph and launch it "
-                         "(default for the city; the per-kernel breakdown then comes from an eager pass over "
+                         "(default for the atrium and the city; the per-kernel breakdown then comes from an eager pass over "
                          "the same frames after the timed region)")
     ap.add_argument("--eager", dest="graph", action="store_false", help="launch the frame's kernels one by one")
     ap.add_argument("--pipeline", dest="pipeline", action="store_true", default=False,
@@ -117,7 +117,7 @@ WORKLOADS = {
             width=256, height=256, depth=2, seed=1, tag="cornell-256"),
     2: dict(name="atrium (~262k tris, 8 textures, 3 area quads + sun), 1920x1080, 1 spp, 4 bounces, "
                  "accumulator + BMFR 32x32 + TAA over the 64-frame camera spline",
-            width=1920, height=1080, depth=4, seed=2, tag="atrium-1080p"),
+            width=1920, height=1080, depth=4, seed=2, graph=True, tag="atrium-1080p"),
     5: dict(name="city stress (~4.9M tris: 4096 towers, 65,536 emissive window quads = 131k triangle lights, "
                  "UNIFORM light sampling), 3840x2160, 1 spp, 8 bounces with Russian roulette, accumulator + "
                  "BMFR + TAA over the 128-frame camera path",
@@ -259,7 +259,7 @@ def main():
             "mean": {n: round(v, 5) for n, v in q["mean"].items()},
             "per_frame": {str(f): {n: round(v, 5) for n, v in row.items()} for f, row in q["frames"].items()}}}))
         return
-    use_graph = cfg.get("graph", False) if args.graph is None else args.graph
+    use_graph = (cfg.get("graph", False) and not args.pipeline) if args.graph is None else args.graph
     graph = None
     if use_graph:
         graph = C.c_void_p()
