@@ -387,7 +387,7 @@ enum pbrt_bmfr_fit {
      * against the noised constant column in FP64 and scaled by powers of two per
      * block, their Gram matrix is built on FP16 MFMA (hi/lo split, FP32
      * accumulation inside each 32-row chunk, FP64 across chunks) and solved by
-     * FP64 Cholesky.  Accurate on well-conditioned blocks only: on planar or
+     * FP64 elimination.  Accurate on well-conditioned blocks only: on planar or
      * sky blocks cond(A^T A) reaches 1e10 and the fit error grows to ~1e-1. */
     PBRT_BMFR_FIT_GRAM_F16 = 1,
     /* as PBRT_BMFR_FIT_GRAM_F16 but the Gram matrix is accumulated by FP64 MFMA

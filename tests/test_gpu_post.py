@@ -230,11 +230,26 @@ def test_bmfr_autotune_picks_a_measured_tile_shape(cuda, oracle_lib, frames, bs)
     assert [c[0] for c in cands] == [256, 512, 1024] and all(c[1] > 0 for c in cands)
     assert tun.best.fit_threads == min(cands, key=lambda c: c[1])[0]
     assert tun.best.fit == capi.BMFR_FIT_GRAM_F64 and tun.best.block_size == bs
-    # the tuned shape runs and matches the default one to FP64 rounding of the fit
-    _, w_best, _, _ = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16),
-                               tun.best.fit_threads)
+    # the tuned shape runs and fits the same values as the default one, to the FP64
+    # rounding of the fit.  The weights themselves are compared loosely: on a
+    # near-collinear block (depth almost linear in the block coordinates) two Gram
+    # sums that differ only in their order of addition give weights that differ along
+    # the normal equations' near-null directions, which the fitted values do not see.
+    feats, w_best, _, _ = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16),
+                                   tun.best.fit_threads)
     _, w_def, _, _ = run_bmfr(cuda, capi.BMFR_FIT_GRAM_F64, bs, 1, inp, np.zeros((2, H, W, 4), np.uint16))
-    assert np.allclose(w_best, w_def, rtol=1e-4, atol=1e-5)
+    nby, nbx = w_def.shape[:2]
+    errs, werrs = [], []
+    for by in range(nby):
+        for bx in range(nbx):
+            A = oracle_lib.bmfr_block_matrix(W, H, bs, 1, feats, bx, by).astype(np.float64)[:, :10]
+            g_best = A @ w_best[by, bx].reshape(10, 3).astype(np.float64)
+            g_def = A @ w_def[by, bx].reshape(10, 3).astype(np.float64)
+            errs.append(np.abs(g_best - g_def).max() / max(1e-2, np.abs(g_def).max()))
+            werrs.append(np.abs(w_best[by, bx] - w_def[by, bx]).max() / np.abs(w_def[by, bx]).max())
+    print(f"tuned vs default shape: fitted values rel diff max {max(errs):.3e}, weights (per block, rel to "
+          f"its largest) max {max(werrs):.3e}")
+    assert max(errs) < 1e-6 and max(werrs) < 1e-3
     p = capi.BmfrParams(W, H, bs, 1, capi.BMFR_FIT_GRAM_F64, 384)
     n = C.c_size_t()
     assert capi.lib().pbrt_bmfr_workspace_size(C.byref(p), C.byref(n)) == -1  # not a supported size

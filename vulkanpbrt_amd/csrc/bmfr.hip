@@ -14,11 +14,11 @@
 //   k_bmfr_fit_gram<F64>     pre + fit fused, features in registers: the Gram matrix
 //                            of the raw columns on v_mfma_f64_16x16x4f64, the
 //                            projection against the noised constant column a0 as a
-//                            rank-1 downdate in FP64, equilibrated FP64 Cholesky
+//                            rank-1 downdate in FP64, equilibrated FP64 elimination
 //   k_bmfr_fit_gram<F16>     the columns projected against a0 in FP64 per row, scaled
 //                            by powers of two, split into fp16 hi + lo parts whose
 //                            Gram v_mfma_f32_16x16x32_f16 accumulates (FP32 inside a
-//                            32-row chunk, FP64 across chunks); FP64 Cholesky
+//                            32-row chunk, FP64 across chunks); FP64 elimination
 //   The BMFR features are nearly collinear on planar blocks (depth is almost a
 //   linear function of the block coordinates; only the 1e-4 noise separates
 //   them), so cond(A^T A) reaches ~1e10: an FP32-accumulated Gram loses the fit
@@ -319,6 +319,46 @@ constexpr int kFitThreads = 256;
 // FP16 fit: wave-private staging of 64 rows (one per lane) as column-major fp16 hi / lo
 // planes, the per-wave partial Gram matrices, and the per-wave partial sums of the
 // block reductions (each reduction costs one workgroup barrier).
+// The 9 x 9 normal equations of the features with the three colours as right-hand
+// sides, solved in one wave with column j of the system [G_ff | G_fc] in lane j's
+// m[0..8] (lanes 0..8 the features, 9..11 the colours; other lanes carry copies).
+// Gaussian elimination without row exchanges: on a symmetric positive semidefinite
+// matrix its pivots are the diagonals of the successive Schur complements (the
+// squares of the Cholesky diagonal) and it is backward stable as Cholesky is.  A
+// pivot at or below 1e-15 x maxdiag drops its feature: it eliminates nothing and its
+// weight is 0, which solves the system without that feature.  Every pivot step is
+// one division and independent multiply-adds across the lanes (a Cholesky column
+// step is a square root, a division and a chain of j dependent products), and the
+// back-substitution runs in the three colour lanes at once.  Returns in lanes 9..11
+// the weights w[0..8] of colour lane - 9.
+__device__ __forceinline__ void solve_feature_block(const double (&m_in)[kFeatures - 1], double maxdiag,
+                                                    double (&w)[kFeatures - 1]) {
+    constexpr int n = kFeatures - 1;
+    double m[n], inv_piv[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i) m[i] = m_in[i];
+#pragma unroll
+    for (int p = 0; p < n; ++p) {
+        const double piv = read_lane(m[p], p);
+        const bool kp = piv > 1e-15 * maxdiag && piv > 0.0;
+        const double inv = kp ? 1.0 / piv : 0.0;
+        inv_piv[p] = inv;
+#pragma unroll
+        for (int i = p + 1; i < n; ++i) m[i] = fma(-(read_lane(m[i], p) * inv), m[p], m[i]);
+    }
+    // m is now the upper triangle U (lane j: U[i][j] for i <= j) and, in the colour
+    // lanes, the eliminated right-hand sides; r takes the substitution's updates
+    double r[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i) r[i] = m[i];
+#pragma unroll
+    for (int p = n - 1; p >= 0; --p) {
+        w[p] = r[p] * inv_piv[p];
+#pragma unroll
+        for (int i = 0; i < p; ++i) r[i] = fma(-read_lane(m[i], p), w[p], r[i]);
+    }
+}
+
 template <int kWaves>
 struct GramShared {
     struct {
@@ -331,9 +371,6 @@ struct GramShared {
     double dots_part[kWaves][kAlphaSize];
     float amax_part[kWaves][12];
     double G[12][12];
-    double Lc[kFeatures - 1][kFeatures - 1];
-    double inv_diag[kFeatures - 1];
-    int keep[kFeatures - 1];
 };
 
 // LDS written by some lanes of a wave and then read by others of the same wave.
@@ -351,7 +388,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // The FP16 fit: the columns are projected against the noised constant column a0 in
 // FP64, rounded to fp32, scaled by powers of two and split into fp16 hi + lo parts
 // whose Gram matrix v_mfma_f32_16x16x32_f16 accumulates (FP32 inside a 32-row chunk,
-// FP64 across chunks); FP64 Cholesky then solves for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+// FP64 across chunks); FP64 elimination then solves for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
 template <int kRows, int kThreads>
 __device__ __forceinline__ void bmfr_fit_gram_f16_block(const BmfrGeom& g, const pbrt_bmfr_images& im, int bx,
                                                         int by) {
@@ -547,61 +584,21 @@ __device__ __forceinline__ void bmfr_fit_gram_f16_block(const BmfrGeom& g, const
     }
     __syncthreads();
     if (wave != 0) return;
-    // 6. FP64 Cholesky of the 9 x 9 feature block in wave 0, lane i holding row i
-    //    (a pivot below 1e-15 of the largest diagonal drops its feature), then one
-    //    lane per colour solves for w1..w9 and w0 = a0.(y - A'w') / a0.a0.
+    // 6. FP64 solve of the 9 x 9 feature block in wave 0 (a pivot below 1e-15 of
+    //    the largest diagonal drops its feature), then in the colour lanes
+    //    w0 = a0.(y - A'w') / a0.a0.
     constexpr int n = kFeatures - 1;
     double maxdiag = 0.0;
 #pragma unroll
     for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, S.G[i][i]);
-    const int li = lane < n ? lane : 0;
-    double grow[n], lrow[n];
+    const int lj = lane < 12 ? lane : 0;
+    double m[n], w[n];
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-        grow[k] = S.G[li][k];
-        lrow[k] = 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < n; ++j) {
-        double dj = grow[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) dj -= lrow[k] * lrow[k];
-        dj = read_lane(dj, j);
-        const bool kp = dj > 1e-15 * maxdiag && dj > 0.0;
-        const double ljj = kp ? sqrt(dj) : 1.0;
-        const double inv = 1.0 / ljj;
-        double v = grow[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) v -= lrow[k] * read_lane(lrow[k], j);
-        lrow[j] = lane == j ? ljj : (lane > j && kp ? v * inv : 0.0);
-        if (lane == 0) {
-            S.keep[j] = kp ? 1 : 0;
-            S.inv_diag[j] = kp ? inv : 0.0;
-        }
-    }
-    if (lane < n) {
-#pragma unroll
-        for (int k = 0; k < n; ++k) S.Lc[lane][k] = lrow[k];
-    }
-    wave_lds_sync();
-    if (lane < 3) {
-        const int c = lane;
+    for (int i = 0; i < n; ++i) m[i] = S.G[i][lj];
+    solve_feature_block(m, maxdiag, w);
+    if (lane >= n && lane < n + 3) {
+        const int c = lane - n;
         float* out = g.weights + ((size_t)by * g.blocks_x + bx) * 30;
-        double z[n], w[n];
-#pragma unroll
-        for (int i = 0; i < n; ++i) {
-            double v = S.G[i][9 + c];
-#pragma unroll
-            for (int k = 0; k < i; ++k) v -= S.Lc[i][k] * z[k];
-            z[i] = v * S.inv_diag[i];
-        }
-#pragma unroll
-        for (int i = n - 1; i >= 0; --i) {
-            double v = z[i];
-#pragma unroll
-            for (int k = i + 1; k < n; ++k) v -= S.Lc[k][i] * w[k];
-            w[i] = v * S.inv_diag[i];
-        }
         double w0 = dot(10 + c);
 #pragma unroll
         for (int i = 0; i < n; ++i) w0 -= w[i] * dot(i + 1);
@@ -612,14 +609,17 @@ __device__ __forceinline__ void bmfr_fit_gram_f16_block(const BmfrGeom& g, const
     }
 }
 
-// Four waves per SIMD (at most 128 VGPRs): three blocks of 256 threads per CU left
-// the per-block latency (load, reductions, Cholesky) uncovered.
+// Occupancy: the F64 fit of 256-thread workgroups runs five waves per SIMD (at most
+// 96 VGPRs, five blocks per CU: 27 KB of LDS each), the other fits four (at most
+// 128); three blocks of 256 threads per CU left the per-block latency (load,
+// reductions, solve) uncovered, and with four a block whose wave 0 is still solving
+// keeps the CU from starting the next block.
 // The FP64 fit.  The Gram matrix R of the raw columns [a0 | a1..a9 | r g b] (fp32
 // values: their products are exact in FP64) is accumulated on v_mfma_f64_16x16x4f64;
 // the normal equations of the features projected against the noised constant column
 // a0 follow in FP64 by the rank-1 downdate G = R[1:,1:] - R[0,1:] R[0,1:]^T / R[0,0].
-// The 9 x 9 feature block of G is equilibrated by its diagonal, factored by Cholesky
-// (a pivot below 1e-15 drops its feature) and solved for the three colours;
+// The 9 x 9 feature block of G is equilibrated by its diagonal and solved for the three
+// colours by elimination (a pivot below 1e-15 drops its feature);
 // w0 = (R[0,10+c] - sum_i w_i R[0,1+i]) / R[0,0].  Against FP64 least squares on the
 // same noised blocks this is more accurate than projecting each row first and
 // rounding the projected columns to fp32 (fitted values within ~1e-8 instead of
@@ -633,8 +633,6 @@ struct RawGramShared {
     float coord[32];
     double r0[kAlphaSize];  // row 0 of R: a0 . a_c
     double G[12][12];       // projected normal equations (9 features | 3 colours)
-    double Lc[kFeatures - 1][kFeatures - 1];
-    double inv_diag[kFeatures - 1];
     double eq[kFeatures - 1];  // equilibration 1 / sqrt(G_ii)
 };
 
@@ -775,60 +773,26 @@ __device__ __forceinline__ void bmfr_fit_raw_gram_block(const BmfrGeom& g, const
     }
     __syncthreads();
     if (wave != 0) return;
-    // 5. Cholesky of the equilibrated 9 x 9 feature block in wave 0, lane i holding row i
+    // 5. the 9 x 9 feature block equilibrated by its diagonal (the colour columns
+    //    by the feature rows' factor only), solved in wave 0
     constexpr int n = kFeatures - 1;
-    const int li = lane < n ? lane : 0;
-    double eqi = S.G[li][li] > 0.0 ? 1.0 / sqrt(S.G[li][li]) : 0.0;
-    if (lane < n) S.eq[lane] = eqi;
+    const int lj = lane < 12 ? lane : 0;
+    const double gjj = S.G[lj][lj];
+    const double eqj = lj < n ? (gjj > 0.0 ? 1.0 / sqrt(gjj) : 0.0) : 1.0;
+    if (lane < n) S.eq[lane] = eqj;
     wave_lds_sync();
-    double grow[n], lrow[n];
+    double maxdiag = 0.0, m[n], w[n];
 #pragma unroll
-    for (int k = 0; k < n; ++k) {
-        grow[k] = S.G[li][k] * eqi * S.eq[k];
-        lrow[k] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double e = S.eq[i];
+        maxdiag = fmax(maxdiag, S.G[i][i] * e * e);
+        m[i] = S.G[i][lj] * e * eqj;
     }
-    double maxdiag = 0.0;
-#pragma unroll
-    for (int i = 0; i < n; ++i) maxdiag = fmax(maxdiag, S.G[i][i] * S.eq[i] * S.eq[i]);
-#pragma unroll
-    for (int j = 0; j < n; ++j) {
-        double dj = grow[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) dj -= lrow[k] * lrow[k];
-        dj = read_lane(dj, j);
-        const bool kp = dj > 1e-15 * maxdiag && dj > 0.0;
-        const double ljj = kp ? sqrt(dj) : 1.0;
-        const double inv = 1.0 / ljj;
-        double v = grow[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) v -= lrow[k] * read_lane(lrow[k], j);
-        lrow[j] = lane == j ? ljj : (lane > j && kp ? v * inv : 0.0);
-        if (lane == 0) S.inv_diag[j] = kp ? inv : 0.0;
-    }
-    if (lane < n) {
-#pragma unroll
-        for (int k = 0; k < n; ++k) S.Lc[lane][k] = lrow[k];
-    }
-    wave_lds_sync();
-    // 6. one lane per colour: forward and back substitution, unequilibrate, w0
-    if (lane < 3) {
-        const int c = lane;
+    solve_feature_block(m, maxdiag, w);
+    // 6. the colour lanes: unequilibrate, w0
+    if (lane >= n && lane < n + 3) {
+        const int c = lane - n;
         float* out = g.weights + ((size_t)by * g.blocks_x + bx) * 30;
-        double z[n], w[n];
-#pragma unroll
-        for (int i = 0; i < n; ++i) {
-            double v = S.G[i][9 + c] * S.eq[i];
-#pragma unroll
-            for (int k = 0; k < i; ++k) v -= S.Lc[i][k] * z[k];
-            z[i] = v * S.inv_diag[i];
-        }
-#pragma unroll
-        for (int i = n - 1; i >= 0; --i) {
-            double v = z[i];
-#pragma unroll
-            for (int k = i + 1; k < n; ++k) v -= S.Lc[k][i] * w[k];
-            w[i] = v * S.inv_diag[i];
-        }
         double w0 = S.r0[10 + c];
 #pragma unroll
         for (int i = 0; i < n; ++i) {
@@ -861,8 +825,8 @@ __device__ __forceinline__ void grid_block(const BmfrGeom& g, int& bx, int& by) 
 }
 
 template <int kRows, bool kF64, int kThreads>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bmfr_fit_gram(BmfrGeom g,
-                                                                                                  pbrt_bmfr_images im) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kF64 && kThreads == 256 ? 5 : 4)))
+k_bmfr_fit_gram(BmfrGeom g, pbrt_bmfr_images im) {
     int bx, by;
     grid_block(g, bx, by);
     if constexpr (kF64)
@@ -968,7 +932,7 @@ __device__ __forceinline__ uint32_t batch_frame_of(const BmfrBatchFrame* frames,
 }
 
 template <int kRows, bool kF64>
-__global__ void __launch_bounds__(kFitThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kFitThreads) __attribute__((amdgpu_waves_per_eu(kF64 ? 5 : 4)))
 k_bmfr_fit_gram_batched(const BmfrBatchFrame* frames, uint32_t n) {
     const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t f = batch_frame_of(frames, n, b);
