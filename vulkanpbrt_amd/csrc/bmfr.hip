@@ -848,7 +848,7 @@ __device__ __forceinline__ float gamma_454545(float x) {
 
 // Rows per workgroup of k_bmfr_post: a block's rows are cut into workgroups of at
 // most 256 threads.
-__host__ __device__ __forceinline__ int post_rows(int bs) { return bs * bs > 256 ? 256 / bs : bs; }
+__host__ __device__ constexpr int post_rows(int bs) { return bs * bs > 256 ? 256 / bs : bs; }
 
 // bmfrPost per pixel: the block's depth range is the one its fit recorded, so a
 // workgroup needs no reduction and no barrier; workgroup w of the launch takes
@@ -857,10 +857,11 @@ __host__ __device__ __forceinline__ int post_rows(int bs) { return bs * bs > 256
 // are uniform over the workgroup: scalar loads, and a non-finite weight (an
 // exponent of all ones) becomes 0 with integer operations.
 // Pixel (lx, ly) of block blk of a frame (the per-thread part of both post launches).
+template <int kBs>
 __device__ __forceinline__ void bmfr_post_pixel(const BmfrGeom& g, const pbrt_bmfr_images& im, uint32_t blk, int lx,
                                                 int ly) {
     const int bx = (int)(blk % (uint32_t)g.blocks_x), by = (int)(blk / (uint32_t)g.blocks_x);
-    const int gx = bx * g.bs + lx, gy = by * g.bs + ly;
+    const int gx = bx * kBs + lx, gy = by * kBs + ly;
     const int ax = gx - g.offx, ay = gy - g.offy;
     if (ax != mirror_coord(ax, g.width) || ay != mirror_coord(ay, g.height)) return;
     const PixelIn p = load_pixel(g, im, gx, gy);
@@ -877,7 +878,7 @@ __device__ __forceinline__ void bmfr_post_pixel(const BmfrGeom& g, const pbrt_bm
     const bool accept = mux >= 0.0f;
     const float pixel_spp = unorm8_to_float(im.spp[pix]) * 256.0f;
     float z = (p.depth - mn) / (mx - mn + 1e-6f);
-    float fx = (float)lx / (float)(g.bs - 1), fy = (float)ly / (float)(g.bs - 1);
+    float fx = div_small<kBs - 1>((float)lx), fy = div_small<kBs - 1>((float)ly);
     float f[kFeatures] = {1.0f, p.normal.x, p.normal.y, p.normal.z, fx, fy, z, fx * fx, fy * fy, z * z};
     vec3 den = make_vec3(0.0f);
 #pragma unroll
@@ -902,11 +903,13 @@ __device__ __forceinline__ void bmfr_post_pixel(const BmfrGeom& g, const pbrt_bm
     store_bgra8(im.final_image, pix, make_vec4(t, 1.0f));
 }
 
+// kBs: the block size (8, 16 or 32), so that the pixel's place in its block is a
+// shift and a mask of the thread index
+template <int kBs>
 __global__ void __launch_bounds__(256) k_bmfr_post(BmfrGeom g, pbrt_bmfr_images im) {
-    const int rows = post_rows(g.bs), tiles = g.bs / rows;
+    constexpr uint32_t rows = post_rows(kBs), tiles = kBs / rows;
     const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
-    bmfr_post_pixel(g, im, w / (uint32_t)tiles, threadIdx.x % g.bs,
-                    (int)(w % (uint32_t)tiles) * rows + threadIdx.x / g.bs);
+    bmfr_post_pixel<kBs>(g, im, w / tiles, threadIdx.x % kBs, (int)((w % tiles) * rows + threadIdx.x / kBs));
 }
 
 // ---------------------------------------------------------------------------
@@ -953,13 +956,14 @@ __global__ void __launch_bounds__(256) k_bmfr_noise_batched(const BmfrBatchFrame
     if (e < kFeatures * g.bs * g.bs) noise_table_entry(g.frame, g.bs, kFitThreads, e, g.noise);
 }
 
+template <int kBs>  // one block size per batch
 __global__ void __launch_bounds__(256) k_bmfr_post_batched(const BmfrBatchFrame* frames, uint32_t n) {
-    const int rows = post_rows(frames[0].g.bs), tiles = frames[0].g.bs / rows;  // one block size per batch
+    constexpr uint32_t rows = post_rows(kBs), tiles = kBs / rows;
     const uint32_t w = xcd_contiguous(blockIdx.x, gridDim.x);
-    const uint32_t b = w / (uint32_t)tiles;
+    const uint32_t b = w / tiles;
     const uint32_t f = batch_frame_of(frames, n, b);
-    bmfr_post_pixel(frames[f].g, frames[f].im, b - frames[f].first_block, threadIdx.x % frames[f].g.bs,
-                    (int)(w % (uint32_t)tiles) * rows + threadIdx.x / frames[f].g.bs);
+    bmfr_post_pixel<kBs>(frames[f].g, frames[f].im, b - frames[f].first_block, threadIdx.x % kBs,
+                         (int)((w % tiles) * rows + threadIdx.x / kBs));
 }
 
 // pixelOffsets of bmfrGeneral.comp:16
@@ -1073,7 +1077,13 @@ static void launch_bmfr(const pbrt_bmfr_params& p, const pbrt_bmfr_images& im, B
         launch_gram_fit(p.fit == PBRT_BMFR_FIT_GRAM_F64, g.bs, (uint32_t)threads, grid, st, g, im);
     }
     const int rows = post_rows(g.bs);
-    hipLaunchKernelGGL(k_bmfr_post, dim3(g.blocks_x * g.blocks_y * (g.bs / rows)), dim3(g.bs * rows), 0, st, g, im);
+    const dim3 post_grid(g.blocks_x * g.blocks_y * (g.bs / rows)), post_block(g.bs * rows);
+    if (g.bs == 32)
+        hipLaunchKernelGGL(k_bmfr_post<32>, post_grid, post_block, 0, st, g, im);
+    else if (g.bs == 16)
+        hipLaunchKernelGGL(k_bmfr_post<16>, post_grid, post_block, 0, st, g, im);
+    else
+        hipLaunchKernelGGL(k_bmfr_post<8>, post_grid, post_block, 0, st, g, im);
     PBRT_HIP(hipGetLastError());
 }
 
@@ -1323,8 +1333,13 @@ extern "C" int32_t pbrt_bmfr_batched(const pbrt_bmfr_batch_frame* frames, uint32
                 hipLaunchKernelGGL((k_bmfr_fit_gram_batched<64, false>), grid, dim3(kFitThreads), 0, st, table, count);
         }
         const int rows = post_rows(plan.bs);
-        hipLaunchKernelGGL(k_bmfr_post_batched, dim3(plan.blocks * (uint32_t)(plan.bs / rows)), dim3(plan.bs * rows), 0,
-                           st, table, count);
+        const dim3 post_grid(plan.blocks * (uint32_t)(plan.bs / rows)), post_block(plan.bs * rows);
+        if (plan.bs == 32)
+            hipLaunchKernelGGL(k_bmfr_post_batched<32>, post_grid, post_block, 0, st, table, count);
+        else if (plan.bs == 16)
+            hipLaunchKernelGGL(k_bmfr_post_batched<16>, post_grid, post_block, 0, st, table, count);
+        else
+            hipLaunchKernelGGL(k_bmfr_post_batched<8>, post_grid, post_block, 0, st, table, count);
         PBRT_HIP(hipGetLastError());
     });
 }

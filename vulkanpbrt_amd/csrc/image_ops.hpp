@@ -23,7 +23,20 @@ __device__ __forceinline__ uint8_t unorm8_rne(float x) {
     x = fminf(fmaxf(x, 0.0f), 1.0f);
     return (uint8_t)rintf(x * 255.0f);
 }
-__device__ __forceinline__ float unorm8_to_float(uint32_t c) { return (float)c / 255.0f; }
+// c / kD rounded to nearest for a small integer c (0 <= c <= 255, 1 <= kD <= 255):
+// q = c * r with r = 1/kD rounded to nearest, then one correction by the residual
+// c - q*kD, which the inner FMA computes exactly.  With r within half an ulp of 1/kD
+// and q within one ulp of c/kD the corrected quotient is the correctly rounded one
+// (Markstein's theorem); every pair (c, kD) this code divides is also checked against
+// IEEE division on the host.  Three instructions against a full division's ten.
+template <int kD>
+__device__ __forceinline__ float div_small(float c) {
+    constexpr float r = 1.0f / (float)kD;
+    const float q = c * r;
+    return fmaf(fmaf(-q, (float)kD, c), r, q);
+}
+
+__device__ __forceinline__ float unorm8_to_float(uint32_t c) { return div_small<255>((float)c); }
 
 // mirror() of bmfrGeneral.comp and bfr.comp: reflect an index that lies at most
 // one image size outside [0, s)
